@@ -1,0 +1,146 @@
+/* zgl -- MI355X-native Gauss-Legendre-quadrature Zernike and spherical-
+ * harmonic transforms: the C ABI.
+ *
+ * This is the thin boundary between host code and the HIP kernels for gfx950.
+ * Only plain pointers, sizes and enums cross it. It replaces the two class
+ * templates of zest (sebsassi/zest) that form the transform hot path:
+ *
+ *   zest::zt::GLQTransformer<ZernikeNorm, SHNorm, SHPhase, LonLatRadLayout<>>
+ *       include/zernike_glq_transformer.hpp:511-965
+ *   zest::st::GLQTransformer<SHNorm, SHPhase, LonLatLayout<> | LatLonLayout<>>
+ *       include/sh_glq_transformer.hpp:513-1213
+ *
+ * Memory layouts are zest's, unchanged:
+ *   Zernike grid   row-major [lon = 2N-1][lat = N+1][rad = round8(N+1)] doubles
+ *                  (include/zernike_glq_transformer.hpp:50-131)
+ *   SH grid        LonLat [lon = 2L-1][lat = round8(L)] or
+ *                  LatLon [lat = L][lon = round8(2L-1)] doubles
+ *                  (include/sh_glq_transformer.hpp:44-195)
+ *   Zernike coeffs {cos, sin} double pairs at (n+1)(n+3)(2n+1)/24 + l*l/4 + m
+ *                  (include/layout.hpp:384-421)
+ *   SH coeffs      {cos, sin} double pairs at l(l+1)/2 + m
+ *                  (include/layout.hpp:202-239)
+ *
+ * Convention values are zest's enum values (include/zernike_conventions.hpp:35,
+ * include/sh_conventions.hpp:35-48).
+ *
+ * Semantics follow zest: the plan adapts to the grid order of each call
+ * (zest's resize(), include/zernike_glq_transformer.hpp:580-632); the transform
+ * works at order min(exp_order, grid_order); forward writes only that many
+ * leading coefficients of each expansion and leaves the rest untouched
+ * (include/zernike_glq_transformer.hpp:649-654); backward writes the whole grid.
+ * With ZGL_MEM_DEVICE the work is enqueued on `stream` and the call returns
+ * without waiting; with ZGL_MEM_HOST the call is synchronous, as zest is.
+ * A plan must not be used from two threads at once (zest's instances are not
+ * thread-safe either); distinct plans are independent.
+ */
+#ifndef ZGL_H
+#define ZGL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum zgl_status {
+    ZGL_SUCCESS = 0,
+    ZGL_ERROR_INVALID_VALUE = 1,
+    ZGL_ERROR_OUT_OF_MEMORY = 2,
+    ZGL_ERROR_HIP = 3,
+    ZGL_ERROR_NOT_SUPPORTED = 4
+} zgl_status;
+
+typedef enum zgl_kind {
+    ZGL_ZERNIKE = 0, /* zt::GLQTransformer: BallGLQGrid <-> RealZernikeExpansion */
+    ZGL_SH = 1       /* st::GLQTransformer: SphereGLQGrid <-> RealSHExpansion */
+} zgl_kind;
+
+/* zest::zt::ZernikeNorm (include/zernike_conventions.hpp:35) */
+enum { ZGL_ZNORM_NORMED = 0, ZGL_ZNORM_UNNORMED = 1 };
+/* zest::st::SHNorm (include/sh_conventions.hpp:40-46) */
+enum { ZGL_SHNORM_GEO = 0, ZGL_SHNORM_QM = 1 };
+/* zest::st::SHPhase (include/sh_conventions.hpp:35) */
+enum { ZGL_PHASE_NONE = -1, ZGL_PHASE_CS = 1 };
+/* SH grid layouts (include/sh_glq_transformer.hpp:44-195) */
+enum { ZGL_LAYOUT_LONLAT = 0, ZGL_LAYOUT_LATLON = 1 };
+
+typedef enum zgl_precision {
+    ZGL_FP64 = 0 /* FP64 throughout: the drop-in form */
+} zgl_precision;
+
+typedef enum zgl_mem {
+    ZGL_MEM_DEVICE = 0, /* grid and coefficient pointers are device memory */
+    ZGL_MEM_HOST = 1    /* host memory; staged through device buffers, synchronous */
+} zgl_mem;
+
+typedef struct zgl_plan zgl_plan;
+
+/* Replaces GLQTransformer(size_t order) and GLQTransformer()
+ * (include/zernike_glq_transformer.hpp:523-570,
+ *  include/sh_glq_transformer.hpp:525-570). order 0 is allowed.
+ * `layout` is ignored for ZGL_ZERNIKE (zest's only Zernike grid layout is
+ * LonLatRadLayout). Tables are built on the current HIP device. */
+zgl_status zgl_plan_create(zgl_plan **plan, zgl_kind kind, int zernike_norm,
+                           int sh_norm, int sh_phase, int layout, size_t order,
+                           zgl_precision precision);
+
+zgl_status zgl_plan_destroy(zgl_plan *plan);
+
+/* Replaces resize(order) (include/zernike_glq_transformer.hpp:580-632,
+ * include/sh_glq_transformer.hpp:580-624). Tables of earlier orders stay
+ * cached, so alternating orders does not rebuild them. */
+zgl_status zgl_plan_resize(zgl_plan *plan, size_t order);
+
+/* Replaces order() (include/zernike_glq_transformer.hpp:575). */
+size_t zgl_plan_order(const zgl_plan *plan);
+
+/* Number of grid doubles and coefficient pairs of one instance at `order`
+ * for this plan's kind and layout (BallGLQGrid::size,
+ * RealZernikeExpansion::size, SphereGLQGrid::size, RealSHExpansion::size). */
+size_t zgl_grid_size(const zgl_plan *plan, size_t order);
+size_t zgl_coeff_size(const zgl_plan *plan, size_t order);
+
+/* Replaces forward_transform(grid_span, expansion_span)
+ * (include/zernike_glq_transformer.hpp:640-655,
+ *  include/sh_glq_transformer.hpp:632-647), batched over `batch`
+ * independent instances. grid_stride and coeff_stride are the distances in
+ * doubles between consecutive instances (0 means packed:
+ * zgl_grid_size(order) and 2 * zgl_coeff_size(exp_order)). `stream` is a
+ * hipStream_t (NULL = the null stream). */
+zgl_status zgl_forward(zgl_plan *plan, const double *grid, size_t grid_order,
+                       double *coeffs, size_t exp_order, size_t batch,
+                       ptrdiff_t grid_stride, ptrdiff_t coeff_stride,
+                       zgl_mem mem, void *stream);
+
+/* Replaces backward_transform(expansion_span, grid_span)
+ * (include/zernike_glq_transformer.hpp:663-679,
+ *  include/sh_glq_transformer.hpp:655-668). */
+zgl_status zgl_backward(zgl_plan *plan, const double *coeffs, size_t exp_order,
+                        double *grid, size_t grid_order, size_t batch,
+                        ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
+                        zgl_mem mem, void *stream);
+
+/* Stage timing. No counterpart in the reference: it serves benchmarks and
+ * profilers. While enabled, every kernel launch of the plan is bracketed by
+ * HIP events on the launch stream; zgl_profile_stats() waits for them and
+ * returns, per stage, the launch count and the summed kernel time. */
+typedef struct zgl_stage_stats {
+    char name[32];
+    long long launches;
+    long long instances; /* instances processed by those launches */
+    double total_ms;
+} zgl_stage_stats;
+
+zgl_status zgl_profile_enable(zgl_plan *plan, int enable);
+zgl_status zgl_profile_stats(zgl_plan *plan, zgl_stage_stats *out, int max_stages,
+                             int *num_stages);
+
+/* Message of the last error on the calling thread (zest reports errors by
+ * exceptions; the C ABI returns a status and keeps the text here). */
+const char *zgl_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZGL_H */

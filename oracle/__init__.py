@@ -1,0 +1,292 @@
+"""ctypes bindings of the test-side checkers.
+
+TEST INFRASTRUCTURE ONLY. Only tests/, ``__graft_entry__.smoke()`` and
+``bench.py``'s cpu_baseline leg import this package, and only to check or to
+time the CPU reference beside the GPU. The product (``zest_amd``) never imports
+it.
+
+Two libraries are bound:
+
+* ``liboracle.so`` -- the C restatement of zest's GLQ transforms
+  (``oracle/zgl_oracle.c``), always built by ``oracle/Makefile``.
+* ``_ref/libzest_ref.so`` -- the upstream zest library compiled from its own
+  sources behind ``oracle/zest_ref_shim.cpp``. Present when it was built in a
+  container holding the zest sources; ``REF`` is ``None`` otherwise.
+
+Convention arguments use zest's enum values: ``zernike_norm`` 0 = normed,
+1 = unnormed; ``sh_norm`` 0 = geo, 1 = qm; ``sh_phase`` -1 = none, +1 = cs;
+SH ``layout`` 0 = LonLat (zest's default), 1 = LatLon.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_c_size = ctypes.c_size_t
+_c_int = ctypes.c_int
+_ptr = ctypes.c_void_p
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _load_oracle():
+    path = os.path.join(_HERE, "liboracle.so")
+    lib = ctypes.CDLL(path)
+    lib.zo_gl_nodes_weights.argtypes = [_c_size, _ptr, _ptr]
+    lib.zo_gl_packed.argtypes = [_c_size, _ptr, _ptr]
+    lib.zo_plm_table.argtypes = [_c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_radial_table.argtypes = [_c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_rdft_forward.argtypes = [_c_size, _c_size, _ptr, _c_size, ctypes.c_double, _ptr]
+    lib.zo_rdft_backward.argtypes = [_c_size, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_zt_forward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_zt_backward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_st_forward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_st_backward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    return lib
+
+
+def _load_ref():
+    path = os.path.join(_HERE, "_ref", "libzest_ref.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.zref_zt_create.restype = _ptr
+    lib.zref_zt_create.argtypes = [_c_int, _c_int, _c_int, _c_size]
+    lib.zref_zt_destroy.argtypes = [_ptr]
+    lib.zref_zt_forward.argtypes = [_ptr, _ptr, _c_size, _ptr, _c_size]
+    lib.zref_zt_backward.argtypes = [_ptr, _ptr, _c_size, _ptr, _c_size]
+    lib.zref_st_create.restype = _ptr
+    lib.zref_st_create.argtypes = [_c_int, _c_int, _c_int, _c_size]
+    lib.zref_st_destroy.argtypes = [_ptr]
+    lib.zref_st_forward.argtypes = [_ptr, _ptr, _c_size, _ptr, _c_size]
+    lib.zref_st_backward.argtypes = [_ptr, _ptr, _c_size, _ptr, _c_size]
+    lib.zref_gl_nodes_weights.argtypes = [_c_size, _c_int, _ptr, _ptr]
+    lib.zref_gl_angles.argtypes = [_c_size, _ptr]
+    lib.zref_plm_table.argtypes = [_c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zref_radial_table.argtypes = [_c_int, _c_size, _ptr, _c_size, _ptr]
+    return lib
+
+
+ORACLE = _load_oracle()
+REF = _load_ref()
+
+
+# ---- shapes (include/zernike_glq_transformer.hpp:97-126,
+#      include/sh_glq_transformer.hpp:48-195) ----
+def zt_shape(order: int) -> tuple[int, int, int]:
+    lon = 2 * order - min(1, order)
+    return (lon, order + 1, (order + 1 + 7) // 8 * 8)
+
+
+def zt_num_coeffs(order: int) -> int:
+    return (order + 1) * (order + 3) * (2 * order + 1) // 24
+
+
+def st_shape(order: int, layout: int = 0) -> tuple[int, int]:
+    lon = 2 * order - min(1, order)
+    if layout == 1:
+        return (order, (lon + 7) // 8 * 8)
+    return (lon, (order + 7) // 8 * 8)
+
+
+def st_num_coeffs(order: int) -> int:
+    return order * (order + 1) // 2
+
+
+# ---- oracle primitives ----
+def gl_nodes_weights(n: int):
+    z = np.zeros(n)
+    w = np.zeros(n)
+    ORACLE.zo_gl_nodes_weights(n, _p(z), _p(w))
+    return z, w
+
+
+def gl_packed(n: int):
+    nu = (n + 1) // 2
+    z = np.zeros(nu)
+    w = np.zeros(nu)
+    ORACLE.zo_gl_packed(n, _p(z), _p(w))
+    return z, w
+
+
+def plm_table(sh_norm: int, sh_phase: int, order: int, z: np.ndarray) -> np.ndarray:
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    out = np.zeros((order * (order + 1) // 2, z.size))
+    ORACLE.zo_plm_table(sh_norm, sh_phase, order, _p(z), z.size, _p(out))
+    return out
+
+
+def radial_table(zernike_norm: int, order: int, r: np.ndarray) -> np.ndarray:
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(((order + 1) ** 2 // 4, r.size))
+    ORACLE.zo_radial_table(zernike_norm, order, _p(r), r.size, _p(out))
+    return out
+
+
+def rdft_forward(x: np.ndarray, nfreq: int, scale: float = 1.0) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[0]
+    cols = x.size // n
+    out = np.zeros((nfreq, cols, 2))
+    ORACLE.zo_rdft_forward(n, cols, _p(x), nfreq, scale, _p(out))
+    return out
+
+
+def rdft_backward(x: np.ndarray, n: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nfreq, cols = x.shape[0], x.shape[1]
+    out = np.zeros((n, cols))
+    ORACLE.zo_rdft_backward(n, cols, _p(x), nfreq, _p(out))
+    return out
+
+
+def zt_forward(grid: np.ndarray, order: int, exp_order: int | None = None,
+               zernike_norm: int = 1, sh_norm: int = 0, sh_phase: int = -1,
+               out: np.ndarray | None = None) -> np.ndarray:
+    exp_order = order if exp_order is None else exp_order
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    assert grid.size == np.prod(zt_shape(order))
+    if out is None:
+        out = np.zeros((zt_num_coeffs(exp_order), 2))
+    ORACLE.zo_zt_forward(zernike_norm, sh_norm, sh_phase, order, _p(grid), exp_order, _p(out))
+    return out
+
+
+def zt_backward(coeffs: np.ndarray, order: int, exp_order: int | None = None,
+                zernike_norm: int = 1, sh_norm: int = 0, sh_phase: int = -1) -> np.ndarray:
+    exp_order = order if exp_order is None else exp_order
+    coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+    assert coeffs.size >= 2 * zt_num_coeffs(min(exp_order, order))
+    grid = np.zeros(zt_shape(order))
+    ORACLE.zo_zt_backward(zernike_norm, sh_norm, sh_phase, exp_order, _p(coeffs), order, _p(grid))
+    return grid
+
+
+def st_forward(grid: np.ndarray, order: int, exp_order: int | None = None,
+               sh_norm: int = 0, sh_phase: int = -1, layout: int = 0,
+               out: np.ndarray | None = None) -> np.ndarray:
+    exp_order = order if exp_order is None else exp_order
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    assert grid.size == np.prod(st_shape(order, layout))
+    if out is None:
+        out = np.zeros((st_num_coeffs(exp_order), 2))
+    ORACLE.zo_st_forward(sh_norm, sh_phase, layout, order, _p(grid), exp_order, _p(out))
+    return out
+
+
+def st_backward(coeffs: np.ndarray, order: int, exp_order: int | None = None,
+                sh_norm: int = 0, sh_phase: int = -1, layout: int = 0) -> np.ndarray:
+    exp_order = order if exp_order is None else exp_order
+    coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+    grid = np.zeros(st_shape(order, layout))
+    ORACLE.zo_st_backward(sh_norm, sh_phase, layout, exp_order, _p(coeffs), order, _p(grid))
+    return grid
+
+
+# ---- the upstream zest library (compiled reference) ----
+class ZestZernike:
+    """One zest ``zt::GLQTransformer`` instance (keeps its tables between calls)."""
+
+    def __init__(self, order: int, zernike_norm: int = 1, sh_norm: int = 0, sh_phase: int = -1):
+        if REF is None:
+            raise RuntimeError("oracle/_ref/libzest_ref.so is not built")
+        self.order = order
+        self._h = REF.zref_zt_create(zernike_norm, sh_norm, sh_phase, order)
+        if not self._h:
+            raise MemoryError("zest transformer allocation failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            REF.zref_zt_destroy(self._h)
+            self._h = None
+
+    def forward(self, grid: np.ndarray, order: int | None = None, exp_order: int | None = None,
+                out: np.ndarray | None = None) -> np.ndarray:
+        order = self.order if order is None else order
+        exp_order = order if exp_order is None else exp_order
+        grid = np.ascontiguousarray(grid, dtype=np.float64)
+        if out is None:
+            out = np.zeros((zt_num_coeffs(exp_order), 2))
+        REF.zref_zt_forward(self._h, _p(grid), order, _p(out), exp_order)
+        return out
+
+    def backward(self, coeffs: np.ndarray, order: int | None = None,
+                 exp_order: int | None = None, out: np.ndarray | None = None) -> np.ndarray:
+        order = self.order if order is None else order
+        exp_order = order if exp_order is None else exp_order
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        if out is None:
+            out = np.zeros(zt_shape(order))
+        REF.zref_zt_backward(self._h, _p(coeffs), exp_order, _p(out), order)
+        return out
+
+
+class ZestSH:
+    """One zest ``st::GLQTransformer`` instance."""
+
+    def __init__(self, order: int, sh_norm: int = 0, sh_phase: int = -1, layout: int = 0):
+        if REF is None:
+            raise RuntimeError("oracle/_ref/libzest_ref.so is not built")
+        self.order = order
+        self.layout = layout
+        self._h = REF.zref_st_create(sh_norm, sh_phase, layout, order)
+        if not self._h:
+            raise MemoryError("zest transformer allocation failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            REF.zref_st_destroy(self._h)
+            self._h = None
+
+    def forward(self, grid: np.ndarray, order: int | None = None, exp_order: int | None = None,
+                out: np.ndarray | None = None) -> np.ndarray:
+        order = self.order if order is None else order
+        exp_order = order if exp_order is None else exp_order
+        grid = np.ascontiguousarray(grid, dtype=np.float64)
+        if out is None:
+            out = np.zeros((st_num_coeffs(exp_order), 2))
+        REF.zref_st_forward(self._h, _p(grid), order, _p(out), exp_order)
+        return out
+
+    def backward(self, coeffs: np.ndarray, order: int | None = None,
+                 exp_order: int | None = None, out: np.ndarray | None = None) -> np.ndarray:
+        order = self.order if order is None else order
+        exp_order = order if exp_order is None else exp_order
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        if out is None:
+            out = np.zeros(st_shape(order, self.layout))
+        REF.zref_st_backward(self._h, _p(coeffs), exp_order, _p(out), order)
+        return out
+
+
+def ref_gl_nodes_weights(n: int, packed: bool = False):
+    size = (n + 1) // 2 if packed else n
+    z = np.zeros(size)
+    w = np.zeros(size)
+    REF.zref_gl_nodes_weights(n, int(packed), _p(z), _p(w))
+    return z, w
+
+
+def ref_gl_angles(n: int) -> np.ndarray:
+    a = np.zeros(n)
+    REF.zref_gl_angles(n, _p(a))
+    return a
+
+
+def ref_plm_table(sh_norm: int, sh_phase: int, order: int, z: np.ndarray) -> np.ndarray:
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    out = np.zeros((order * (order + 1) // 2, z.size))
+    REF.zref_plm_table(sh_norm, sh_phase, order, _p(z), z.size, _p(out))
+    return out
+
+
+def ref_radial_table(zernike_norm: int, order: int, r: np.ndarray) -> np.ndarray:
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(((order + 1) ** 2 // 4, r.size))
+    REF.zref_radial_table(zernike_norm, order, _p(r), r.size, _p(out))
+    return out

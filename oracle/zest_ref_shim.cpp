@@ -1,0 +1,236 @@
+// C-ABI shim over the upstream zest library (sebsassi/zest), compiled from the
+// zest sources where they lie (the headers of include/ plus
+// src/plm_recursion.cpp and src/radial_zernike_recursion.cpp) into
+// oracle/_ref/libzest_ref.so by oracle/Makefile.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the real reference: tests use it to
+// pin the C restatement in oracle/zgl_oracle.c, and bench.py times it as the
+// CPU baseline. Nothing in the product (zest_amd/, include/) links or loads it.
+//
+// Every entry point wraps one zest call:
+//   zt::GLQTransformer::forward_transform / backward_transform
+//       (include/zernike_glq_transformer.hpp:640-679)
+//   st::GLQTransformer::forward_transform / backward_transform
+//       (include/sh_glq_transformer.hpp:632-668)
+//   gl::gl_nodes_and_weights (include/gauss_legendre.hpp:1549-1562)
+//   st::PlmRecursion::plm_real (include/plm_recursion.hpp:108-113)
+//   zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-154)
+#include <cstddef>
+#include <cstring>
+#include <new>
+#include <span>
+#include <vector>
+
+#include "zernike_glq_transformer.hpp"
+#include "sh_glq_transformer.hpp"
+#include "gauss_legendre.hpp"
+#include "plm_recursion.hpp"
+#include "radial_zernike_recursion.hpp"
+
+namespace {
+
+using zest::st::SHNorm;
+using zest::st::SHPhase;
+using zest::zt::ZernikeNorm;
+
+struct ZtBase {
+    virtual ~ZtBase() = default;
+    virtual void forward(const double* grid, std::size_t grid_order,
+                         double* coeffs, std::size_t exp_order) = 0;
+    virtual void backward(const double* coeffs, std::size_t exp_order,
+                          double* grid, std::size_t grid_order) = 0;
+};
+
+template <ZernikeNorm zn, SHNorm sn, SHPhase sp>
+struct Zt final : ZtBase {
+    zest::zt::GLQTransformer<zn, sn, sp> t;
+    explicit Zt(std::size_t order) : t(order) {}
+    void forward(const double* grid, std::size_t grid_order, double* coeffs,
+                 std::size_t exp_order) override {
+        using Grid = zest::zt::BallGLQGridSpan<const double>;
+        using Exp = zest::zt::RealZernikeSpan<std::array<double, 2>, zn, sn, sp>;
+        t.forward_transform(Grid(grid, grid_order),
+                            Exp(reinterpret_cast<std::array<double, 2>*>(coeffs), exp_order));
+    }
+    void backward(const double* coeffs, std::size_t exp_order, double* grid,
+                  std::size_t grid_order) override {
+        using Grid = zest::zt::BallGLQGridSpan<double>;
+        using Exp = zest::zt::RealZernikeSpan<const std::array<double, 2>, zn, sn, sp>;
+        t.backward_transform(
+            Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), exp_order),
+            Grid(grid, grid_order));
+    }
+};
+
+struct StBase {
+    virtual ~StBase() = default;
+    virtual void forward(const double* grid, std::size_t grid_order,
+                         double* coeffs, std::size_t exp_order) = 0;
+    virtual void backward(const double* coeffs, std::size_t exp_order,
+                          double* grid, std::size_t grid_order) = 0;
+};
+
+template <SHNorm sn, SHPhase sp, typename Layout>
+struct St final : StBase {
+    zest::st::GLQTransformer<sn, sp, Layout> t;
+    explicit St(std::size_t order) : t(order) {}
+    void forward(const double* grid, std::size_t grid_order, double* coeffs,
+                 std::size_t exp_order) override {
+        using Grid = zest::st::SphereGLQGridSpan<const double, Layout>;
+        using Exp = zest::st::RealSHSpan<std::array<double, 2>, sn, sp>;
+        t.forward_transform(Grid(grid, grid_order),
+                            Exp(reinterpret_cast<std::array<double, 2>*>(coeffs), exp_order));
+    }
+    void backward(const double* coeffs, std::size_t exp_order, double* grid,
+                  std::size_t grid_order) override {
+        using Grid = zest::st::SphereGLQGridSpan<double, Layout>;
+        using Exp = zest::st::RealSHSpan<const std::array<double, 2>, sn, sp>;
+        t.backward_transform(
+            Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), exp_order),
+            Grid(grid, grid_order));
+    }
+};
+
+template <ZernikeNorm zn, SHNorm sn>
+ZtBase* make_zt_phase(int phase, std::size_t order) {
+    if (phase > 0) return new Zt<zn, sn, SHPhase::cs>(order);
+    return new Zt<zn, sn, SHPhase::none>(order);
+}
+
+template <ZernikeNorm zn>
+ZtBase* make_zt_norm(int sh_norm, int phase, std::size_t order) {
+    if (sh_norm == 1) return make_zt_phase<zn, SHNorm::qm>(phase, order);
+    return make_zt_phase<zn, SHNorm::geo>(phase, order);
+}
+
+template <SHNorm sn, typename Layout>
+StBase* make_st_phase(int phase, std::size_t order) {
+    if (phase > 0) return new St<sn, SHPhase::cs, Layout>(order);
+    return new St<sn, SHPhase::none, Layout>(order);
+}
+
+template <typename Layout>
+StBase* make_st_layout(int sh_norm, int phase, std::size_t order) {
+    if (sh_norm == 1) return make_st_phase<SHNorm::qm, Layout>(phase, order);
+    return make_st_phase<SHNorm::geo, Layout>(phase, order);
+}
+
+}  // namespace
+
+extern "C" {
+
+// zernike_norm: 0 = normed, 1 = unnormed; sh_norm: 0 = geo, 1 = qm;
+// sh_phase: -1 = none, +1 = cs (the values of zest's enums).
+void* zref_zt_create(int zernike_norm, int sh_norm, int sh_phase, std::size_t order) {
+    try {
+        if (zernike_norm == 1)
+            return make_zt_norm<ZernikeNorm::unnormed>(sh_norm, sh_phase, order);
+        return make_zt_norm<ZernikeNorm::normed>(sh_norm, sh_phase, order);
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void zref_zt_destroy(void* h) { delete static_cast<ZtBase*>(h); }
+
+void zref_zt_forward(void* h, const double* grid, std::size_t grid_order,
+                     double* coeffs, std::size_t exp_order) {
+    static_cast<ZtBase*>(h)->forward(grid, grid_order, coeffs, exp_order);
+}
+
+void zref_zt_backward(void* h, const double* coeffs, std::size_t exp_order,
+                      double* grid, std::size_t grid_order) {
+    static_cast<ZtBase*>(h)->backward(coeffs, exp_order, grid, grid_order);
+}
+
+// layout: 0 = LonLatLayout (zest's default), 1 = LatLonLayout.
+void* zref_st_create(int sh_norm, int sh_phase, int layout, std::size_t order) {
+    try {
+        if (layout == 1)
+            return make_st_layout<zest::st::LatLonLayout<>>(sh_norm, sh_phase, order);
+        return make_st_layout<zest::st::LonLatLayout<>>(sh_norm, sh_phase, order);
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void zref_st_destroy(void* h) { delete static_cast<StBase*>(h); }
+
+void zref_st_forward(void* h, const double* grid, std::size_t grid_order,
+                     double* coeffs, std::size_t exp_order) {
+    static_cast<StBase*>(h)->forward(grid, grid_order, coeffs, exp_order);
+}
+
+void zref_st_backward(void* h, const double* coeffs, std::size_t exp_order,
+                      double* grid, std::size_t grid_order) {
+    static_cast<StBase*>(h)->backward(coeffs, exp_order, grid, grid_order);
+}
+
+// Gauss-Legendre nodes (cos style) and weights; packed != 0 selects the
+// z >= 0 half, as zest's st::GLQTransformer uses it.
+void zref_gl_nodes_weights(std::size_t num_nodes, int packed, double* nodes,
+                           double* weights) {
+    const std::size_t parity = num_nodes & 1;
+    if (packed) {
+        const std::size_t n = zest::gl::PackedLayout::size(num_nodes);
+        std::span<double> sn(nodes, n), sw(weights, n);
+        zest::gl::gl_nodes_and_weights<zest::gl::PackedLayout, zest::gl::GLNodeStyle::cos>(
+            sn, sw, parity);
+    } else {
+        std::span<double> sn(nodes, num_nodes), sw(weights, num_nodes);
+        zest::gl::gl_nodes_and_weights<zest::gl::UnpackedLayout, zest::gl::GLNodeStyle::cos>(
+            sn, sw, parity);
+    }
+}
+
+// Gauss-Legendre nodes in angle style (colatitudes), unpacked, as
+// BallGLQGridPoints / SphereGLQGridPoints sample them.
+void zref_gl_angles(std::size_t num_nodes, double* angles) {
+    std::span<double> s(angles, num_nodes);
+    zest::gl::gl_nodes<zest::gl::UnpackedLayout, zest::gl::GLNodeStyle::angle>(
+        s, num_nodes & 1);
+}
+
+// P_lm(z_i) table, layout [l(l+1)/2 + m][nz] (PlmVecSpan).
+void zref_plm_table(int sh_norm, int sh_phase, std::size_t order, const double* z,
+                    std::size_t nz, double* out) {
+    zest::st::PlmRecursion rec(order);
+    std::span<const double> zs(z, nz);
+    const std::size_t size = zest::st::PlmLayout::size(order) * nz;
+    std::span<double> buf(out, size);
+    if (sh_norm == 1) {
+        if (sh_phase > 0) {
+            zest::st::PlmVecSpan<double, SHNorm::qm, SHPhase::cs> p(buf, order, nz);
+            rec.plm_real(zs, p);
+        } else {
+            zest::st::PlmVecSpan<double, SHNorm::qm, SHPhase::none> p(buf, order, nz);
+            rec.plm_real(zs, p);
+        }
+    } else {
+        if (sh_phase > 0) {
+            zest::st::PlmVecSpan<double, SHNorm::geo, SHPhase::cs> p(buf, order, nz);
+            rec.plm_real(zs, p);
+        } else {
+            zest::st::PlmVecSpan<double, SHNorm::geo, SHPhase::none> p(buf, order, nz);
+            rec.plm_real(zs, p);
+        }
+    }
+}
+
+// R_nl(r_i) table, layout [((n+1)^2 >> 2) + (l >> 1)][nr] (RadialZernikeVecSpan).
+void zref_radial_table(int zernike_norm, std::size_t order, const double* r,
+                       std::size_t nr, double* out) {
+    zest::zt::RadialZernikeRecursion rec(order);
+    std::span<const double> rs(r, nr);
+    const std::size_t size = zest::zt::RadialZernikeLayout::size(order) * nr;
+    std::span<double> buf(out, size);
+    if (zernike_norm == 1) {
+        zest::zt::RadialZernikeVecSpan<double, ZernikeNorm::unnormed> z(buf, order, nr);
+        rec.zernike<ZernikeNorm::unnormed>(rs, z);
+    } else {
+        zest::zt::RadialZernikeVecSpan<double, ZernikeNorm::normed> z(buf, order, nr);
+        rec.zernike<ZernikeNorm::normed>(rs, z);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,595 @@
+/* CPU restatement of zest's Gauss-Legendre-quadrature (GLQ) transforms, FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the oracle the HIP library in
+ * zest_amd/ is checked against. Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, and only as the checker; nothing in the
+ * product links or calls it.
+ *
+ * It restates, stage by stage, the algorithm of the reference
+ * (sebsassi/zest) for:
+ *   zt::GLQTransformer  include/zernike_glq_transformer.hpp:640-679, 710-949
+ *   st::GLQTransformer  include/sh_glq_transformer.hpp:632-668, 757-1201
+ * with the numerics of
+ *   st::PlmRecursion         include/plm_recursion.hpp:229-354, src/plm_recursion.cpp:181-247
+ *   zt::RadialZernikeRecursion include/radial_zernike_recursion.hpp:151-288,
+ *                            src/radial_zernike_recursion.cpp:91-116
+ *   pocketfft r2c/c2r semantics (sign, scale, Hermitian half, Im X_0 ignored)
+ *                            include/pocketfft.hpp:3293-3418
+ * The Gauss-Legendre nodes are solved by Newton's method on the Legendre
+ * three-term recurrence instead of zest's tables / Bogaert formulas
+ * (include/gauss_legendre.hpp:1549-1562); both are accurate to a few ulp.
+ * The longitude transform is a direct DFT, not an FFT.
+ *
+ * Pinned by tests/test_oracle.py against: the upstream zest library compiled
+ * from its sources (oracle/_ref), numpy.polynomial.legendre.leggauss,
+ * scipy.special (lpmv, eval_jacobi), numpy.fft, closed forms of P_lm (l <= 4)
+ * and R_nl (n <= 6), the known-answer transforms of zest's own tests, and the
+ * committed golden fixtures in tests/golden/.
+ */
+#include "zgl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ZO_PI 3.14159265358979323846264338327950288
+
+/* ---- grid shapes (include/zernike_glq_transformer.hpp:97-126,
+ *      include/sh_glq_transformer.hpp:48-195, include/alignment.hpp:57-61) */
+static size_t round8(size_t n) { return (n + 7) & ~(size_t)7; }
+
+size_t zo_zt_lon_size(size_t order) { return 2 * order - (order ? 1 : 0); }
+size_t zo_zt_lat_size(size_t order) { return order + 1; }
+size_t zo_zt_rad_size(size_t order) { return round8(order + 1); }
+size_t zo_zt_num_coeffs(size_t order) {
+    return (order + 1) * (order + 3) * (2 * order + 1) / 24; /* include/layout.hpp:400-406 */
+}
+size_t zo_st_lon_size(int layout, size_t order) {
+    size_t n = 2 * order - (order ? 1 : 0);
+    return layout == 1 ? round8(n) : n;
+}
+size_t zo_st_lat_size(int layout, size_t order) {
+    return layout == 1 ? order : round8(order);
+}
+
+static size_t tri_idx(size_t l, size_t m) { return ((l * (l + 1)) >> 1) + m; }
+static size_t rad_idx(size_t n, size_t l) { return (((n + 1) * (n + 1)) >> 2) + (l >> 1); }
+static size_t tet_idx(size_t n, size_t l, size_t m) {
+    return (n + 1) * (n + 3) * (2 * n + 1) / 24 + ((l * l) >> 2) + m;
+}
+
+/* ---- Gauss-Legendre nodes and weights --------------------------------------
+ * Unpacked, ascending z (nodes[0] ~ -1), cos style, as
+ * gl::gl_nodes_and_weights<UnpackedLayout, GLNodeStyle::cos> returns them. */
+static void legendre_p_dp(size_t n, double x, double *p, double *dp) {
+    double p0 = 1.0, p1 = x;
+    if (n == 0) { *p = 1.0; *dp = 0.0; return; }
+    for (size_t k = 1; k < n; ++k) {
+        double p2 = ((double)(2 * k + 1) * x * p1 - (double)k * p0) / (double)(k + 1);
+        p0 = p1;
+        p1 = p2;
+    }
+    *p = p1;
+    *dp = (double)n * (x * p1 - p0) / (x * x - 1.0);
+}
+
+void zo_gl_nodes_weights(size_t n, double *nodes, double *weights) {
+    if (n == 0) return;
+    const size_t half = n / 2;
+    for (size_t k = 0; k < (n + 1) / 2; ++k) {
+        /* k-th largest root; Tricomi's initial guess */
+        double theta = ZO_PI * ((double)k + 0.75) / ((double)n + 0.5);
+        double x = cos(theta);
+        double p, dp;
+        for (int it = 0; it < 100; ++it) {
+            legendre_p_dp(n, x, &p, &dp);
+            double dx = p / dp;
+            x -= dx;
+            if (fabs(dx) <= 1e-17 * (fabs(x) + 1e-300)) break;
+        }
+        legendre_p_dp(n, x, &p, &dp);
+        double w = 2.0 / ((1.0 - x * x) * dp * dp);
+        if ((n & 1) && k == half) { x = 0.0; }
+        nodes[n - 1 - k] = x;
+        nodes[k] = -x;
+        weights[n - 1 - k] = w;
+        weights[k] = w;
+    }
+}
+
+/* Packed layout (include/gauss_legendre.hpp:40-52): the (n+1)/2 nodes z >= 0,
+ * ascending, with their weights (the centre node keeps its full weight). */
+void zo_gl_packed(size_t n, double *nodes, double *weights) {
+    if (n == 0) return;
+    double *zn = (double *)malloc(n * sizeof(double));
+    double *zw = (double *)malloc(n * sizeof(double));
+    zo_gl_nodes_weights(n, zn, zw);
+    const size_t nu = (n + 1) / 2;
+    for (size_t i = 0; i < nu; ++i) {
+        nodes[i] = zn[n / 2 + i];
+        weights[i] = zw[n / 2 + i];
+    }
+    free(zn);
+    free(zw);
+}
+
+/* ---- associated Legendre table ---------------------------------------------
+ * include/plm_recursion.hpp:229-354 (vectorised plm_impl, complex_norm = sqrt2)
+ * with the coefficients of src/plm_recursion.cpp:181-247.
+ * out layout: [l(l+1)/2 + m][nz]. */
+void zo_plm_table(int sh_norm, int sh_phase, size_t order, const double *z,
+                  size_t nz, double *out) {
+    if (order == 0) return;
+    const double inv_sqrt_4pi = 0.5 / sqrt(ZO_PI);
+    const double phase = (double)sh_phase;
+    const int qm = (sh_norm == 1);
+    double *sqrl = (double *)calloc(2 * order + 2, sizeof(double));
+    double *alm = (double *)calloc(order * order + 1, sizeof(double));
+    double *blm = (double *)calloc(order * order + 1, sizeof(double));
+    double *u = (double *)malloc(nz * sizeof(double));
+    double *us = (double *)malloc(nz * sizeof(double));
+    for (size_t l = 1; l < 2 * order; ++l) sqrl[l] = sqrt((double)l);
+    for (size_t l = 2; l < order; ++l) {
+        size_t ind = tri_idx(l, 0);
+        alm[ind] = sqrl[2 * l - 1] * sqrl[2 * l + 1] / (double)l;
+        blm[ind] = ((double)l - 1.0) * sqrl[2 * l + 1] / (sqrl[2 * l - 3] * (double)l);
+        for (size_t m = 1; m + 1 < l; ++m) {
+            ind = tri_idx(l, m);
+            alm[ind] = sqrl[2 * l - 1] * sqrl[2 * l + 1] / (sqrl[l - m] * sqrl[l + m]);
+            blm[ind] = sqrl[2 * l + 1] * sqrl[l + m - 1] * sqrl[l - m - 1] /
+                       (sqrl[l - m] * sqrl[l + m] * sqrl[2 * l - 3]);
+        }
+    }
+#define P(ind, i) out[(size_t)(ind) * nz + (i)]
+    for (size_t i = 0; i < nz; ++i) {
+        u[i] = sqrt((1.0 - z[i]) * (1.0 + z[i]));
+        P(0, i) = qm ? inv_sqrt_4pi : 1.0;
+    }
+    if (order == 1) goto done;
+    for (size_t i = 0; i < nz; ++i)
+        P(1, i) = qm ? z[i] * (sqrl[3] * inv_sqrt_4pi) : z[i] * sqrl[3];
+    for (size_t l = 2; l < order; ++l) {
+        size_t ind = tri_idx(l, 0);
+        for (size_t i = 0; i < nz; ++i)
+            P(ind, i) = alm[ind] * z[i] * P(ind - l, i) - blm[ind] * P(ind - 2 * l + 1, i);
+    }
+    {
+        const double underflow_compensation = 1.0e-280;
+        double pmm = qm ? underflow_compensation * sqrt(2.0) * inv_sqrt_4pi
+                        : underflow_compensation * sqrt(2.0);
+        for (size_t i = 0; i < nz; ++i) us[i] = 1.0 / underflow_compensation;
+        for (size_t m = 1; m + 1 < order; ++m) {
+            for (size_t i = 0; i < nz; ++i) us[i] *= u[i];
+            pmm *= phase * sqrl[2 * m + 1] / sqrl[2 * m];
+            for (size_t i = 0; i < nz; ++i) P(tri_idx(m, m), i) = pmm;
+            for (size_t i = 0; i < nz; ++i)
+                P(tri_idx(m + 1, m), i) = z[i] * (sqrl[2 * m + 3] * pmm);
+            for (size_t l = m + 2; l < order; ++l) {
+                size_t ind = tri_idx(l, m);
+                for (size_t i = 0; i < nz; ++i)
+                    P(ind, i) = z[i] * alm[ind] * P(ind - l, i) - blm[ind] * P(ind - 2 * l + 1, i);
+                for (size_t i = 0; i < nz; ++i) P(ind - 2 * l + 1, i) *= us[i];
+            }
+            for (size_t i = 0; i < nz; ++i) P(tri_idx(order - 1, m), i) *= us[i];
+            for (size_t i = 0; i < nz; ++i) P(tri_idx(order - 2, m), i) *= us[i];
+        }
+        for (size_t i = 0; i < nz; ++i) us[i] *= u[i];
+        for (size_t i = 0; i < nz; ++i)
+            P(tri_idx(order - 1, order - 1), i) =
+                us[i] * (phase * pmm * sqrl[2 * order - 1] / sqrl[2 * order - 2]);
+    }
+#undef P
+done:
+    free(sqrl); free(alm); free(blm); free(u); free(us);
+}
+
+/* ---- radial Zernike table ----------------------------------------------------
+ * include/radial_zernike_recursion.hpp:151-288, coefficients
+ * src/radial_zernike_recursion.cpp:91-116.
+ * out layout: [((n+1)^2 >> 2) + (l >> 1)][nr]. */
+void zo_radial_table(int zernike_norm, size_t order, const double *r, size_t nr,
+                     double *out) {
+    if (order == 0) return;
+    const int normed = (zernike_norm == 0);
+    const double sqrt3 = 1.7320508075688772935274463;
+    const double sqrt5 = 2.2360679774997896964091737;
+    const double sqrt7 = 2.6457513110645905905016158;
+    const size_t nk = ((order + 1) * (order + 1)) >> 2;
+    double *norms = (double *)malloc(order * sizeof(double));
+    double *k1 = (double *)calloc(nk, sizeof(double));
+    double *k2 = (double *)calloc(nk, sizeof(double));
+    double *k3 = (double *)calloc(nk, sizeof(double));
+    for (size_t n = 0; n < order; ++n) {
+        double dn = (double)n;
+        norms[n] = sqrt(2.0 * dn + 3.0);
+        for (size_t l = n & 1; l < n; l += 2) {
+            double dl = (double)l;
+            double c0 = (dn - dl) * (dn + dl + 1.0) * (2.0 * dn - 3.0);
+            double c1 = (2.0 * dn - 1.0) * (2.0 * dn + 1) * (2.0 * dn - 3.0);
+            double c2 = -0.5 * (2.0 * dn - 1.0) *
+                        ((2 * dl + 1.0) * (2.0 * dl + 1.0) + (2.0 * dn + 1) * (2.0 * dn - 3.0));
+            double c3 = -(dn - dl - 2.0) * (dn + dl - 1.0) * (2.0 * dn + 1);
+            double inv = 1.0 / c0;
+            size_t ind = rad_idx(n, l);
+            k1[ind] = c1 * inv;
+            k2[ind] = c2 * inv;
+            k3[ind] = c3 * inv;
+        }
+    }
+#define R(n, l, i) out[rad_idx((n), (l)) * nr + (i)]
+    for (size_t i = 0; i < nr; ++i) R(0, 0, i) = 1.0;
+    if (order == 1) {
+        if (normed) for (size_t i = 0; i < nr; ++i) R(0, 0, i) *= sqrt3;
+        goto done;
+    }
+    for (size_t i = 0; i < nr; ++i) R(1, 1, i) = r[i];
+    if (order == 2) {
+        if (normed) for (size_t i = 0; i < nr; ++i) { R(0, 0, i) *= sqrt3; R(1, 1, i) *= sqrt5; }
+        goto done;
+    }
+    for (size_t i = 0; i < nr; ++i) R(2, 2, i) = r[i] * r[i];
+    for (size_t i = 0; i < nr; ++i) R(2, 0, i) = 2.5 * R(2, 2, i) - 1.5;
+    if (order == 3) {
+        if (normed)
+            for (size_t i = 0; i < nr; ++i) {
+                R(0, 0, i) *= sqrt3; R(1, 1, i) *= sqrt5; R(2, 0, i) *= sqrt7; R(2, 2, i) *= sqrt7;
+            }
+        goto done;
+    }
+    for (size_t i = 0; i < nr; ++i) R(3, 1, i) = (3.5 * R(2, 2, i) - 2.5) * r[i];
+    for (size_t i = 0; i < nr; ++i) R(3, 3, i) = R(2, 2, i) * r[i];
+    for (size_t n = 4; n < order; ++n) {
+        for (size_t l = n & 1; l + 4 <= n; l += 2) {
+            size_t ind = rad_idx(n, l);
+            for (size_t i = 0; i < nr; ++i)
+                R(n, l, i) = (k2[ind] + k1[ind] * R(2, 2, i)) * R(n - 2, l, i) + k3[ind] * R(n - 4, l, i);
+            if (normed) {
+                /* R22 stays unnormalised while it serves as r^2 */
+                double nm = (n == 6 && l == 2) ? 1.0 : norms[n - 4];
+                for (size_t i = 0; i < nr; ++i) R(n - 4, l, i) *= nm;
+            }
+        }
+        double dn = (double)n;
+        for (size_t i = 0; i < nr; ++i) R(n, n, i) = r[i] * R(n - 1, n - 1, i);
+        for (size_t i = 0; i < nr; ++i)
+            R(n, n - 2, i) = (dn + 0.5) * R(n, n, i) - (dn - 0.5) * R(n - 2, n - 2, i);
+    }
+    if (normed) {
+        if (order > 6) for (size_t i = 0; i < nr; ++i) R(2, 2, i) *= sqrt7;
+        for (size_t n = order - 4; n < order; ++n)
+            for (size_t l = n & 1; l <= n; l += 2)
+                for (size_t i = 0; i < nr; ++i) R(n, l, i) *= norms[n];
+    }
+#undef R
+done:
+    free(norms); free(k1); free(k2); free(k3);
+}
+
+/* ---- real DFT along the outermost axis --------------------------------------
+ * Forward: out[m][c] = scale * sum_i in[i][c] e^{-2 pi i m i / n}, m < nfreq
+ * (pocketfft::r2c with fct = scale, include/pocketfft.hpp:3488-3500).
+ * Backward: out[i][c] = Re X_0 + 2 sum_{1 <= m < nfreq, m < n/2...} Re(X_m e^{+...})
+ * plus Re X_{n/2} (-1)^i for even n (pocketfft::c2r, fct = 1,
+ * include/pocketfft.hpp:3520-3532): the imaginary parts of X_0 and of the
+ * Nyquist bin are ignored. Complex data is interleaved {re, im}. */
+void zo_rdft_forward(size_t n, size_t cols, const double *in, size_t nfreq,
+                     double scale, double *out) {
+    double *cs = (double *)malloc(n * sizeof(double));
+    double *sn = (double *)malloc(n * sizeof(double));
+    for (size_t t = 0; t < n; ++t) {
+        cs[t] = cos(2.0 * ZO_PI * (double)t / (double)n);
+        sn[t] = sin(2.0 * ZO_PI * (double)t / (double)n);
+    }
+    double *acc = (double *)malloc(2 * cols * sizeof(double));
+    for (size_t m = 0; m < nfreq; ++m) {
+        memset(acc, 0, 2 * cols * sizeof(double));
+        for (size_t i = 0; i < n; ++i) {
+            size_t t = (m * i) % n;
+            const double c = cs[t], s = sn[t];
+            const double *row = in + i * cols;
+            for (size_t k = 0; k < cols; ++k) {
+                acc[2 * k] += row[k] * c;
+                acc[2 * k + 1] -= row[k] * s;
+            }
+        }
+        for (size_t k = 0; k < 2 * cols; ++k) out[m * 2 * cols + k] = scale * acc[k];
+        if (m == 0)
+            for (size_t k = 0; k < cols; ++k) out[2 * k + 1] = 0.0;
+        if ((n & 1) == 0 && m == n / 2)
+            for (size_t k = 0; k < cols; ++k) out[m * 2 * cols + 2 * k + 1] = 0.0;
+    }
+    free(cs); free(sn); free(acc);
+}
+
+void zo_rdft_backward(size_t n, size_t cols, const double *in, size_t nfreq,
+                      double *out) {
+    double *cs = (double *)malloc(n * sizeof(double));
+    double *sn = (double *)malloc(n * sizeof(double));
+    for (size_t t = 0; t < n; ++t) {
+        cs[t] = cos(2.0 * ZO_PI * (double)t / (double)n);
+        sn[t] = sin(2.0 * ZO_PI * (double)t / (double)n);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        double *row = out + i * cols;
+        for (size_t k = 0; k < cols; ++k) row[k] = in[2 * k];
+        for (size_t m = 1; m < nfreq; ++m) {
+            const int nyq = ((n & 1) == 0 && 2 * m == n);
+            if (2 * m > n) break;
+            size_t t = (m * i) % n;
+            const double c = cs[t], s = sn[t];
+            const double *x = in + m * 2 * cols;
+            if (nyq) {
+                for (size_t k = 0; k < cols; ++k) row[k] += x[2 * k] * c;
+            } else {
+                for (size_t k = 0; k < cols; ++k)
+                    row[k] += 2.0 * (x[2 * k] * c - x[2 * k + 1] * s);
+            }
+        }
+    }
+    free(cs); free(sn);
+}
+
+/* ---- Zernike GLQ transforms ------------------------------------------------- */
+typedef struct {
+    size_t order, nphi, nth, nr, nf;
+    double *lat_z, *lat_w, *rad_r, *rad_w, *plm, *rad;
+} zt_tables;
+
+/* include/zernike_glq_transformer.hpp:526-556 */
+static void zt_tables_init(zt_tables *t, int zn, int sn, int sp, size_t order) {
+    t->order = order;
+    t->nphi = zo_zt_lon_size(order);
+    t->nth = zo_zt_lat_size(order);
+    t->nr = zo_zt_rad_size(order);
+    t->nf = (t->nphi >> 1) + 1;
+    t->lat_z = (double *)malloc(t->nth * sizeof(double));
+    t->lat_w = (double *)malloc(t->nth * sizeof(double));
+    t->rad_r = (double *)malloc(t->nr * sizeof(double));
+    t->rad_w = (double *)malloc(t->nr * sizeof(double));
+    zo_gl_nodes_weights(t->nth, t->lat_z, t->lat_w);
+    zo_gl_nodes_weights(t->nr, t->rad_r, t->rad_w);
+    for (size_t k = 0; k < t->nr; ++k) t->rad_r[k] = 0.5 * (1.0 + t->rad_r[k]);
+    t->plm = (double *)calloc(((order * (order + 1)) >> 1) * t->nth + 1, sizeof(double));
+    t->rad = (double *)calloc((((order + 1) * (order + 1)) >> 2) * t->nr + 1, sizeof(double));
+    zo_plm_table(sn, sp, order, t->lat_z, t->nth, t->plm);
+    zo_radial_table(zn, order, t->rad_r, t->nr, t->rad);
+}
+
+static void zt_tables_free(zt_tables *t) {
+    free(t->lat_z); free(t->lat_w); free(t->rad_r); free(t->rad_w);
+    free(t->plm); free(t->rad);
+}
+
+/* forward_transform: include/zernike_glq_transformer.hpp:640-655 */
+int zo_zt_forward(int zn, int sn, int sp, size_t N, const double *grid,
+                  size_t exp_order, double *coeffs) {
+    if (N == 0) return 0;
+    zt_tables t;
+    zt_tables_init(&t, zn, sn, sp, N);
+    const size_t cols = t.nth * t.nr;
+    const double shnorm = (sn == 1) ? 1.0 : 1.0 / (4.0 * ZO_PI);
+    /* integrate_longitudinal, 710-720 */
+    const double prefactor = shnorm * 0.5 * (2.0 * ZO_PI / (double)t.nphi);
+    double *ffts = (double *)malloc(t.nf * cols * 2 * sizeof(double));
+    zo_rdft_forward(t.nphi, cols, grid, t.nf, prefactor, ffts);
+    /* apply_weights, 722-750 */
+    for (size_t m = 0; m < t.nf; ++m)
+        for (size_t j = 0; j < t.nth; ++j)
+            for (size_t k = 0; k < t.nr; ++k) {
+                double *x = ffts + ((m * t.nth + j) * t.nr + k) * 2;
+                double w = t.lat_w[j] * t.rad_r[k] * t.rad_r[k] * t.rad_w[k];
+                x[0] = w * x[0];
+                x[1] = -w * x[1];
+            }
+    const size_t Nm = exp_order < N ? exp_order : N;
+    /* integrate_latitudinal, 778-817 */
+    double *flm = (double *)calloc(((N * (N + 1)) >> 1) * t.nr * 2, sizeof(double));
+    for (size_t l = 0; l < Nm; ++l)
+        for (size_t m = 0; m <= l; ++m) {
+            double *f = flm + tri_idx(l, m) * t.nr * 2;
+            const double *p = t.plm + tri_idx(l, m) * t.nth;
+            for (size_t j = 0; j < t.nth; ++j) {
+                const double *x = ffts + (m * t.nth + j) * t.nr * 2;
+                for (size_t k = 0; k < t.nr; ++k) {
+                    f[2 * k] += p[j] * x[2 * k];
+                    f[2 * k + 1] += p[j] * x[2 * k + 1];
+                }
+            }
+        }
+    /* integrate_radial, 819-865 */
+    memset(coeffs, 0, zo_zt_num_coeffs(Nm) * 2 * sizeof(double));
+    for (size_t n = 0; n < Nm; ++n) {
+        const double norm = (zn == 1) ? (double)(2 * n + 3) : 1.0;
+        for (size_t l = n & 1; l <= n; l += 2) {
+            const double *rr = t.rad + rad_idx(n, l) * t.nr;
+            for (size_t m = 0; m <= l; ++m) {
+                const double *f = flm + tri_idx(l, m) * t.nr * 2;
+                double *c = coeffs + tet_idx(n, l, m) * 2;
+                for (size_t k = 0; k < t.nr; ++k) {
+                    c[0] += rr[k] * f[2 * k];
+                    c[1] += rr[k] * f[2 * k + 1];
+                }
+                if (zn == 1) { c[0] *= norm; c[1] *= norm; }
+            }
+        }
+    }
+    free(ffts); free(flm);
+    zt_tables_free(&t);
+    return 0;
+}
+
+/* backward_transform: include/zernike_glq_transformer.hpp:663-679 */
+int zo_zt_backward(int zn, int sn, int sp, size_t exp_order, const double *coeffs,
+                   size_t N, double *grid) {
+    if (N == 0) return 0;
+    zt_tables t;
+    zt_tables_init(&t, zn, sn, sp, N);
+    const size_t cols = t.nth * t.nr;
+    const size_t Nm = exp_order < N ? exp_order : N;
+    /* sum_n, 867-899 */
+    double *flm = (double *)calloc(((N * (N + 1)) >> 1) * t.nr * 2, sizeof(double));
+    for (size_t n = 0; n < Nm; ++n)
+        for (size_t l = n & 1; l <= n; l += 2) {
+            const double *rr = t.rad + rad_idx(n, l) * t.nr;
+            for (size_t m = 0; m <= l; ++m) {
+                const double *c = coeffs + tet_idx(n, l, m) * 2;
+                double *f = flm + tri_idx(l, m) * t.nr * 2;
+                for (size_t k = 0; k < t.nr; ++k) {
+                    f[2 * k] += rr[k] * c[0];
+                    f[2 * k + 1] += rr[k] * c[1];
+                }
+            }
+        }
+    /* sum_l, 901-941 */
+    double *ffts = (double *)calloc(t.nf * cols * 2, sizeof(double));
+    for (size_t l = 0; l < Nm; ++l)
+        for (size_t m = 0; m <= l; ++m) {
+            const double *f = flm + tri_idx(l, m) * t.nr * 2;
+            const double *p = t.plm + tri_idx(l, m) * t.nth;
+            const double mf = m > 0 ? 0.5 : 1.0;
+            for (size_t j = 0; j < t.nth; ++j) {
+                const double w = mf * p[j];
+                double *x = ffts + (m * t.nth + j) * t.nr * 2;
+                for (size_t k = 0; k < t.nr; ++k) {
+                    x[2 * k] += w * f[2 * k];
+                    x[2 * k + 1] += -w * f[2 * k + 1];
+                }
+            }
+        }
+    /* sum_m, 943-949 */
+    zo_rdft_backward(t.nphi, cols, ffts, t.nf, grid);
+    free(ffts); free(flm);
+    zt_tables_free(&t);
+    return 0;
+}
+
+/* ---- spherical-harmonic GLQ transforms ---------------------------------------
+ * Grid stored as [lon][lat] (LonLat) or [lat][lon] (LatLon). Internally both
+ * are handled as a [lon][lat] spectrum, which only reorders the same sums. */
+typedef struct {
+    size_t order, nphi, nth, nu, nf;
+    double *z, *w, *plm;
+} st_tables;
+
+/* include/sh_glq_transformer.hpp:525-570 */
+static void st_tables_init(st_tables *t, int sn, int sp, int layout, size_t order) {
+    t->order = order;
+    t->nphi = zo_st_lon_size(layout, order);
+    t->nth = zo_st_lat_size(layout, order);
+    t->nu = (t->nth + 1) >> 1;
+    t->nf = (t->nphi >> 1) + 1;
+    t->z = (double *)malloc(t->nu * sizeof(double));
+    t->w = (double *)malloc(t->nu * sizeof(double));
+    zo_gl_packed(t->nth, t->z, t->w);
+    t->plm = (double *)calloc(((order * (order + 1)) >> 1) * t->nu + 1, sizeof(double));
+    zo_plm_table(sn, sp, order, t->z, t->nu, t->plm);
+}
+
+static void st_tables_free(st_tables *t) { free(t->z); free(t->w); free(t->plm); }
+
+/* forward_transform: include/sh_glq_transformer.hpp:632-647 */
+int zo_st_forward(int sn, int sp, int layout, size_t L, const double *grid,
+                  size_t exp_order, double *coeffs) {
+    if (L == 0) return 0;
+    st_tables t;
+    st_tables_init(&t, sn, sp, layout, L);
+    const double shnorm = (sn == 1) ? 1.0 : 1.0 / (4.0 * ZO_PI);
+    /* integrate_longitudinal, 757-765 */
+    const double prefactor = shnorm * (2.0 * ZO_PI) / (double)t.nphi;
+    double *g = (double *)malloc(t.nphi * t.nth * sizeof(double));
+    if (layout == 1) {
+        for (size_t j = 0; j < t.nth; ++j)
+            for (size_t i = 0; i < t.nphi; ++i) g[i * t.nth + j] = grid[j * t.nphi + i];
+    } else {
+        memcpy(g, grid, t.nphi * t.nth * sizeof(double));
+    }
+    double *ffts = (double *)malloc(t.nf * t.nth * 2 * sizeof(double)); /* [m][lat] */
+    zo_rdft_forward(t.nphi, t.nth, g, t.nf, prefactor, ffts);
+    /* fft_to_symm_asymm, 810-884: sa[par][m][i] */
+    const size_t north = t.nth >> 1, south = t.nu - 1;
+    double *sa = (double *)calloc(2 * t.nf * t.nu * 2, sizeof(double));
+    for (size_t m = 0; m < t.nf; ++m) {
+        const double sign = (m & 1) ? -1.0 : 1.0;
+        for (size_t i = 0; i < t.nu; ++i) {
+            const double *S = ffts + (m * t.nth + south - i) * 2;
+            const double *Nn = ffts + (m * t.nth + north + i) * 2;
+            const double w = t.w[i];
+            double sw[2] = {w * S[0], -w * S[1]};
+            double nw[2] = {w * Nn[0], -w * Nn[1]};
+            double *s0 = sa + ((0 * t.nf + m) * t.nu + i) * 2;
+            double *s1 = sa + ((1 * t.nf + m) * t.nu + i) * 2;
+            s0[0] = nw[0] + sign * sw[0];
+            s0[1] = nw[1] + sign * sw[1];
+            s1[0] = nw[0] - sign * sw[0];
+            s1[1] = nw[1] - sign * sw[1];
+        }
+    }
+    /* integrate_latitudinal, 886-974 */
+    const size_t Lm = exp_order < L ? exp_order : L;
+    for (size_t l = 0; l < Lm; ++l)
+        for (size_t m = 0; m <= l; ++m) {
+            const double *p = t.plm + tri_idx(l, m) * t.nu;
+            const double *x = sa + (((l & 1) * t.nf + m) * t.nu) * 2;
+            double c0 = 0.0, c1 = 0.0;
+            for (size_t i = 0; i < t.nu; ++i) {
+                c0 += p[i] * x[2 * i];
+                c1 += p[i] * x[2 * i + 1];
+            }
+            coeffs[tri_idx(l, m) * 2] = c0;
+            coeffs[tri_idx(l, m) * 2 + 1] = c1;
+        }
+    free(g); free(ffts); free(sa);
+    st_tables_free(&t);
+    return 0;
+}
+
+/* backward_transform: include/sh_glq_transformer.hpp:655-668 */
+int zo_st_backward(int sn, int sp, int layout, size_t exp_order, const double *coeffs,
+                   size_t L, double *grid) {
+    if (L == 0) return 0;
+    st_tables t;
+    st_tables_init(&t, sn, sp, layout, L);
+    const size_t Lm = exp_order < L ? exp_order : L;
+    /* sum_l, 976-1051 */
+    double *sa = (double *)calloc(2 * t.nf * t.nu * 2, sizeof(double));
+    for (size_t l = 0; l < Lm; ++l)
+        for (size_t m = 0; m <= l; ++m) {
+            const double *p = t.plm + tri_idx(l, m) * t.nu;
+            const double *c = coeffs + tri_idx(l, m) * 2;
+            double *x = sa + (((l & 1) * t.nf + m) * t.nu) * 2;
+            const double mf = m > 0 ? 0.5 : 1.0;
+            for (size_t i = 0; i < t.nu; ++i) {
+                const double w = mf * p[i];
+                x[2 * i] += w * c[0];
+                x[2 * i + 1] += -w * c[1];
+            }
+        }
+    /* symm_asymm_to_fft, 1136-1193 */
+    const size_t north = t.nth >> 1, south = t.nu - 1;
+    double *ffts = (double *)calloc(t.nf * t.nth * 2, sizeof(double));
+    for (size_t m = 0; m < t.nf; ++m) {
+        const double sign = (m & 1) ? -1.0 : 1.0;
+        for (size_t i = 0; i < t.nu; ++i) {
+            const double *s0 = sa + ((0 * t.nf + m) * t.nu + i) * 2;
+            const double *s1 = sa + ((1 * t.nf + m) * t.nu + i) * 2;
+            double *Nn = ffts + (m * t.nth + north + i) * 2;
+            double *S = ffts + (m * t.nth + south - i) * 2;
+            Nn[0] = s0[0] + s1[0];
+            Nn[1] = s0[1] + s1[1];
+            S[0] = sign * (s0[0] - s1[0]);
+            S[1] = sign * (s0[1] - s1[1]);
+        }
+    }
+    /* sum_m, 1195-1201 */
+    double *g = (double *)malloc(t.nphi * t.nth * sizeof(double));
+    zo_rdft_backward(t.nphi, t.nth, ffts, t.nf, g);
+    if (layout == 1) {
+        for (size_t j = 0; j < t.nth; ++j)
+            for (size_t i = 0; i < t.nphi; ++i) grid[j * t.nphi + i] = g[i * t.nth + j];
+    } else {
+        memcpy(grid, g, t.nphi * t.nth * sizeof(double));
+    }
+    free(g); free(ffts); free(sa);
+    st_tables_free(&t);
+    return 0;
+}

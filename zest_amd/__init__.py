@@ -1,0 +1,180 @@
+"""zest_amd: MI355X-native Gauss-Legendre-quadrature Zernike and spherical-
+harmonic transforms.
+
+The product is ``libzgl.so`` -- hand-written HIP kernels for gfx950 behind the
+C ABI declared in ``include/zgl.h`` -- together with the C++20 drop-in header
+``include/zest_amd/glq_transformer.hpp``. This module is a thin ctypes binding
+of that C ABI used by the tests and the benchmark; it moves no data itself and
+has no CPU fallback: importing it without the built library raises.
+
+Convention arguments take zest's enum values: ``zernike_norm`` 0 = normed,
+1 = unnormed; ``sh_norm`` 0 = geo, 1 = qm; ``sh_phase`` -1 = none, +1 = cs; SH
+``layout`` 0 = LonLat, 1 = LatLon.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzgl.so")
+
+ZERNIKE, SH = 0, 1
+FP64 = 0
+MEM_DEVICE, MEM_HOST = 0, 1
+
+_STATUS = {0: "success", 1: "invalid value", 2: "out of memory", 3: "HIP error", 4: "not supported"}
+
+
+class ZglError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"zgl {_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, pd = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_ssize_t
+    lib.zgl_plan_create.argtypes = [ctypes.POINTER(vp), i32, i32, i32, i32, i32, sz, i32]
+    lib.zgl_plan_create.restype = i32
+    lib.zgl_plan_destroy.argtypes = [vp]
+    lib.zgl_plan_destroy.restype = i32
+    lib.zgl_plan_resize.argtypes = [vp, sz]
+    lib.zgl_plan_resize.restype = i32
+    lib.zgl_plan_order.argtypes = [vp]
+    lib.zgl_plan_order.restype = sz
+    lib.zgl_grid_size.argtypes = [vp, sz]
+    lib.zgl_grid_size.restype = sz
+    lib.zgl_coeff_size.argtypes = [vp, sz]
+    lib.zgl_coeff_size.restype = sz
+    lib.zgl_forward.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_forward.restype = i32
+    lib.zgl_backward.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_backward.restype = i32
+    lib.zgl_last_error_string.argtypes = []
+    lib.zgl_last_error_string.restype = ctypes.c_char_p
+    return lib
+
+
+LIB = _load()
+
+# The C ABI entry points of include/zgl.h.
+EXPORTS = (
+    "zgl_plan_create", "zgl_plan_destroy", "zgl_plan_resize", "zgl_plan_order",
+    "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
+    "zgl_last_error_string",
+)
+
+
+def _check(status: int):
+    if status != 0:
+        raise ZglError(status, LIB.zgl_last_error_string().decode())
+
+
+def zt_shape(order: int) -> tuple[int, int, int]:
+    """BallGLQGrid shape [lon][lat][rad] (include/zernike_glq_transformer.hpp:70-126)."""
+    return (2 * order - min(1, order), order + 1, (order + 1 + 7) // 8 * 8)
+
+
+def zt_num_coeffs(order: int) -> int:
+    """RealZernikeExpansion size (include/layout.hpp:400-406)."""
+    return (order + 1) * (order + 3) * (2 * order + 1) // 24
+
+
+def st_shape(order: int, layout: int = 0) -> tuple[int, int]:
+    """SphereGLQGrid shape (include/sh_glq_transformer.hpp:44-195)."""
+    lon = 2 * order - min(1, order)
+    if layout == 1:
+        return (order, (lon + 7) // 8 * 8)
+    return (lon, (order + 7) // 8 * 8)
+
+
+def st_num_coeffs(order: int) -> int:
+    return order * (order + 1) // 2
+
+
+def _ptr_and_mem(x):
+    """(pointer, memory kind) of a torch tensor or numpy array."""
+    if isinstance(x, np.ndarray):
+        if not x.flags.c_contiguous or x.dtype != np.float64:
+            raise ValueError("arrays must be C-contiguous float64")
+        return x.ctypes.data, MEM_HOST
+    import torch
+    if not isinstance(x, torch.Tensor):
+        raise TypeError("expected a numpy array or a torch tensor")
+    if x.dtype != torch.float64 or not x.is_contiguous():
+        raise ValueError("tensors must be contiguous float64")
+    return x.data_ptr(), (MEM_DEVICE if x.is_cuda else MEM_HOST)
+
+
+class Plan:
+    """One transform plan (the counterpart of one zest GLQTransformer)."""
+
+    def __init__(self, kind: int, order: int, zernike_norm: int = 1, sh_norm: int = 0,
+                 sh_phase: int = -1, layout: int = 0, precision: int = FP64):
+        self.kind = kind
+        self.layout = layout if kind == SH else 0
+        self._h = ctypes.c_void_p()
+        _check(LIB.zgl_plan_create(ctypes.byref(self._h), kind, zernike_norm, sh_norm,
+                                   sh_phase, layout, order, precision))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            LIB.zgl_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def order(self) -> int:
+        return LIB.zgl_plan_order(self._h)
+
+    def resize(self, order: int):
+        _check(LIB.zgl_plan_resize(self._h, order))
+
+    def grid_size(self, order: int) -> int:
+        return LIB.zgl_grid_size(self._h, order)
+
+    def coeff_size(self, order: int) -> int:
+        return LIB.zgl_coeff_size(self._h, order)
+
+    def grid_shape(self, order: int):
+        return zt_shape(order) if self.kind == ZERNIKE else st_shape(order, self.layout)
+
+    def forward(self, grid, coeffs, grid_order: int, exp_order: int | None = None,
+                batch: int = 1, grid_stride: int = 0, coeff_stride: int = 0, stream=None):
+        """Enqueue forward transforms of `batch` instances (zgl_forward)."""
+        exp_order = grid_order if exp_order is None else exp_order
+        gp, gm = _ptr_and_mem(grid)
+        cp, cm = _ptr_and_mem(coeffs)
+        if gm != cm:
+            raise ValueError("grid and coefficients must live in the same memory")
+        _check(LIB.zgl_forward(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
+                               coeff_stride, gm, _stream_ptr(stream, gm)))
+
+    def backward(self, coeffs, grid, grid_order: int, exp_order: int | None = None,
+                 batch: int = 1, coeff_stride: int = 0, grid_stride: int = 0, stream=None):
+        """Enqueue backward transforms of `batch` instances (zgl_backward)."""
+        exp_order = grid_order if exp_order is None else exp_order
+        cp, cm = _ptr_and_mem(coeffs)
+        gp, gm = _ptr_and_mem(grid)
+        if gm != cm:
+            raise ValueError("grid and coefficients must live in the same memory")
+        _check(LIB.zgl_backward(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
+                                grid_stride, gm, _stream_ptr(stream, gm)))
+
+
+def _stream_ptr(stream, mem):
+    if mem == MEM_HOST:
+        return None
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)

@@ -1,0 +1,439 @@
+// The C ABI of include/zgl.h: plans, table caching, chunked pipelines and
+// host/device staging around the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "zgl_internal.hpp"
+
+namespace zgl {
+
+void check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    zgl_status st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+                        ? ZGL_ERROR_OUT_OF_MEMORY
+                        : ZGL_ERROR_HIP;
+    throw Error{st, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+namespace {
+
+thread_local std::string g_last_error;
+
+zgl_status fail(zgl_status s, const std::string& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+constexpr size_t kMaxOrder = 4096;
+
+size_t env_mb(const char* name, size_t dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    char* end = nullptr;
+    const unsigned long long x = std::strtoull(v, &end, 10);
+    return (end && *end == 0 && x > 0) ? (size_t)x : dflt;
+}
+
+}  // namespace
+}  // namespace zgl
+
+struct zgl_plan {
+    zgl_kind kind;
+    int znorm, shnorm, phase, layout;
+    zgl_precision precision;
+    size_t order = 0;
+    std::map<size_t, zgl::OrderTables> tables;
+    double2* spec = nullptr;
+    size_t spec_cap = 0;  // double2 elements
+    double2* flm = nullptr;
+    size_t flm_cap = 0;
+    double* dgrid = nullptr;
+    size_t dgrid_cap = 0;  // doubles
+    double* dcoef = nullptr;
+    size_t dcoef_cap = 0;  // doubles
+    // stage timing (zgl_profile_enable)
+    bool profiling = false;
+    struct Rec { int stage; int nb; hipEvent_t start, stop; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+};
+
+namespace zgl {
+namespace {
+
+size_t grid_size(const zgl_plan* p, size_t order) {
+    if (order == 0) return 0;
+    const size_t lon_min = 2 * order - 1;
+    if (p->kind == ZGL_ZERNIKE) return lon_min * (order + 1) * ((order + 1 + 7) / 8 * 8);
+    if (p->layout == ZGL_LAYOUT_LATLON) return order * ((lon_min + 7) / 8 * 8);
+    return lon_min * ((order + 7) / 8 * 8);
+}
+
+size_t coeff_size(const zgl_plan* p, size_t order) {
+    if (p->kind == ZGL_ZERNIKE) return (order + 1) * (order + 3) * (2 * order + 1) / 24;
+    return order * (order + 1) / 2;
+}
+
+const OrderTables& tables_for(zgl_plan* p, size_t order) {
+    auto it = p->tables.find(order);
+    if (it != p->tables.end()) return it->second;
+    OrderTables t;
+    try {
+        build_tables(t, p->kind, p->znorm, p->shnorm, p->phase, p->layout, (int)order, nullptr);
+    } catch (...) {
+        free_tables(t);
+        throw;
+    }
+    return p->tables.emplace(order, t).first->second;
+}
+
+template <class T>
+void ensure(T*& ptr, size_t& cap, size_t need) {
+    if (need <= cap) return;
+    if (ptr) check_hip(hipFree(ptr), "free scratch");
+    ptr = nullptr;
+    cap = 0;
+    check_hip(hipMalloc(&ptr, need * sizeof(T)), "scratch allocation");
+    cap = need;
+}
+
+struct Chunking {
+    int nb;
+    long spec_stride;  // double2
+    long flm_stride;   // double2
+};
+
+Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch) {
+    Chunking c;
+    c.spec_stride = (long)g.nm * 2 * g.jh * g.ncol;
+    c.flm_stride = p->kind == ZGL_ZERNIKE ? (long)g.order * (g.order + 1) / 2 * g.ncol : 0;
+    const size_t per = (size_t)(c.spec_stride + c.flm_stride) * sizeof(double2);
+    size_t budget, cap;
+    if (p->kind == ZGL_ZERNIKE) {
+        // Keep a chunk's intermediates within the 256 MB Infinity Cache.
+        budget = env_mb("ZGL_SCRATCH_MB", 192) << 20;
+        cap = 64;
+    } else {
+        // Instances form the GEMM column axis: larger chunks reuse the P table more.
+        budget = env_mb("ZGL_SH_SCRATCH_MB", 1024) << 20;
+        cap = 512;
+    }
+    size_t nb = per ? budget / per : cap;
+    nb = std::max<size_t>(1, std::min({nb, cap, batch}));
+    c.nb = (int)nb;
+    return c;
+}
+
+enum Stage { ST_FFT_FWD, ST_LEG_FWD, ST_RAD_FWD, ST_RAD_BWD, ST_LEG_BWD, ST_FFT_BWD, ST_COUNT };
+const char* const kStageNames[ST_COUNT] = {"fft_forward", "legendre_forward", "radial_forward",
+                                           "radial_backward", "legendre_backward", "fft_backward"};
+
+hipEvent_t pooled_event(zgl_plan* p) {
+    if (!p->pool.empty()) {
+        hipEvent_t e = p->pool.back();
+        p->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    check_hip(hipEventCreate(&e), "event create");
+    return e;
+}
+
+// Runs `launch` between two events on stream s when the plan is profiling.
+template <class F>
+void timed(zgl_plan* p, Stage st, int nb, hipStream_t s, F&& launch) {
+    if (!p->profiling) {
+        launch();
+        return;
+    }
+    zgl_plan::Rec r{st, nb, pooled_event(p), pooled_event(p)};
+    check_hip(hipEventRecord(r.start, s), "event record");
+    launch();
+    check_hip(hipEventRecord(r.stop, s), "event record");
+    p->recs.push_back(r);
+}
+
+double shnorm_value(int sh_norm) { return sh_norm == ZGL_SHNORM_QM ? 1.0 : 1.0 / (4.0 * M_PI); }
+
+void run_forward(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
+                 size_t batch, long gs, long cs, hipStream_t s) {
+    const OrderTables& t = tables_for(p, N);
+    const Geom& g = t.g;
+    const Chunking ch = chunking(p, g, batch);
+    ensure(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb);
+    if (ch.flm_stride) ensure(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb);
+    FftLaunch L;
+    L.g = &g;
+    L.t = &t;
+    L.layout_kind = (p->kind == ZGL_SH && p->layout == ZGL_LAYOUT_LATLON) ? GRID_ROWS : GRID_COLS;
+    L.nm_used = (int)nt;
+    const double twopi_n = 2.0 * M_PI / (double)g.nphi;
+    L.scale = p->kind == ZGL_ZERNIKE ? shnorm_value(p->shnorm) * 0.5 * twopi_n
+                                     : shnorm_value(p->shnorm) * twopi_n;
+    L.center_factor = p->kind == ZGL_ZERNIKE ? 1.0 : 2.0;
+    for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
+        const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
+        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
+        if (p->kind == ZGL_ZERNIKE) {
+            ZtGemmArgs a{&t, (int)nt, p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0, p->spec, ch.spec_stride,
+                         p->flm, ch.flm_stride, coeffs + b0 * cs, cs, nb};
+            timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward(a, s); });
+            timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward(a, s); });
+        } else {
+            StGemmArgs a{&t, (int)nt, p->spec, ch.spec_stride, coeffs + b0 * cs, cs, nb};
+            timed(p, ST_LEG_FWD, nb, s, [&] { st_legendre_forward(a, s); });
+        }
+    }
+}
+
+void run_backward(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
+                  size_t batch, long cs, long gs, hipStream_t s) {
+    const OrderTables& t = tables_for(p, N);
+    const Geom& g = t.g;
+    const Chunking ch = chunking(p, g, batch);
+    ensure(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb);
+    if (ch.flm_stride) ensure(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb);
+    FftLaunch L;
+    L.g = &g;
+    L.t = &t;
+    L.layout_kind = (p->kind == ZGL_SH && p->layout == ZGL_LAYOUT_LATLON) ? GRID_ROWS : GRID_COLS;
+    L.nm_used = (int)nt;
+    L.scale = 1.0;
+    L.center_factor = 1.0;
+    for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
+        const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
+        if (p->kind == ZGL_ZERNIKE) {
+            ZtGemmArgs a{&t, (int)nt, 0, p->spec, ch.spec_stride, p->flm, ch.flm_stride,
+                         const_cast<double*>(coeffs) + b0 * cs, cs, nb};
+            timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward(a, s); });
+            timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward(a, s); });
+        } else {
+            StGemmArgs a{&t, (int)nt, p->spec, ch.spec_stride, const_cast<double*>(coeffs) + b0 * cs,
+                         cs, nb};
+            timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward(a, s); });
+        }
+        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward(L, p->spec, ch.spec_stride, grid + b0 * gs, gs, nb, s); });
+    }
+}
+
+template <class F>
+zgl_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const Error& e) {
+        return fail(e.status, e.what);
+    } catch (const std::bad_alloc&) {
+        return fail(ZGL_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(ZGL_ERROR_HIP, "unexpected exception");
+    }
+}
+
+zgl_status check_conventions(zgl_kind kind, int zn, int sn, int sp, int layout) {
+    if (kind != ZGL_ZERNIKE && kind != ZGL_SH) return fail(ZGL_ERROR_INVALID_VALUE, "unknown transform kind");
+    if (zn != ZGL_ZNORM_NORMED && zn != ZGL_ZNORM_UNNORMED)
+        return fail(ZGL_ERROR_INVALID_VALUE, "zernike_norm must be 0 (normed) or 1 (unnormed)");
+    if (sn != ZGL_SHNORM_GEO && sn != ZGL_SHNORM_QM)
+        return fail(ZGL_ERROR_INVALID_VALUE, "sh_norm must be 0 (geo) or 1 (qm)");
+    if (sp != ZGL_PHASE_NONE && sp != ZGL_PHASE_CS)
+        return fail(ZGL_ERROR_INVALID_VALUE, "sh_phase must be -1 (none) or +1 (cs)");
+    if (kind == ZGL_SH && layout != ZGL_LAYOUT_LONLAT && layout != ZGL_LAYOUT_LATLON)
+        return fail(ZGL_ERROR_INVALID_VALUE, "layout must be 0 (LonLat) or 1 (LatLon)");
+    return ZGL_SUCCESS;
+}
+
+}  // namespace
+}  // namespace zgl
+
+using namespace zgl;
+
+extern "C" {
+
+zgl_status zgl_plan_create(zgl_plan** plan, zgl_kind kind, int zernike_norm, int sh_norm,
+                           int sh_phase, int layout, size_t order, zgl_precision precision) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan pointer is NULL");
+    *plan = nullptr;
+    zgl_status st = check_conventions(kind, zernike_norm, sh_norm, sh_phase, layout);
+    if (st != ZGL_SUCCESS) return st;
+    if (precision != ZGL_FP64) return fail(ZGL_ERROR_NOT_SUPPORTED, "unsupported precision");
+    if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
+    return guarded([&]() -> zgl_status {
+        zgl_plan* p = new zgl_plan;
+        p->kind = kind;
+        p->znorm = zernike_norm;
+        p->shnorm = sh_norm;
+        p->phase = sh_phase;
+        p->layout = kind == ZGL_SH ? layout : 0;
+        p->precision = precision;
+        p->order = order;
+        try {
+            if (order) tables_for(p, order);
+        } catch (...) {
+            zgl_plan_destroy(p);
+            throw;
+        }
+        *plan = p;
+        return ZGL_SUCCESS;
+    });
+}
+
+zgl_status zgl_plan_destroy(zgl_plan* plan) {
+    if (!plan) return ZGL_SUCCESS;
+    (void)hipDeviceSynchronize();
+    for (auto& kv : plan->tables) free_tables(kv.second);
+    for (auto& r : plan->recs) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    for (auto e : plan->pool) (void)hipEventDestroy(e);
+    for (void* q : {(void*)plan->spec, (void*)plan->flm, (void*)plan->dgrid, (void*)plan->dcoef})
+        if (q) (void)hipFree(q);
+    delete plan;
+    return ZGL_SUCCESS;
+}
+
+zgl_status zgl_plan_resize(zgl_plan* plan, size_t order) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
+    return guarded([&]() -> zgl_status {
+        if (order) tables_for(plan, order);
+        plan->order = order;
+        return ZGL_SUCCESS;
+    });
+}
+
+size_t zgl_plan_order(const zgl_plan* plan) { return plan ? plan->order : 0; }
+
+size_t zgl_grid_size(const zgl_plan* plan, size_t order) { return plan ? grid_size(plan, order) : 0; }
+
+size_t zgl_coeff_size(const zgl_plan* plan, size_t order) { return plan ? coeff_size(plan, order) : 0; }
+
+zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, double* coeffs,
+                       size_t exp_order, size_t batch, ptrdiff_t grid_stride,
+                       ptrdiff_t coeff_stride, zgl_mem mem, void* stream) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
+    if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    const size_t nt = std::min(exp_order, grid_order);
+    const long gsz = (long)grid_size(plan, grid_order);
+    const long csz = 2 * (long)coeff_size(plan, exp_order);
+    const long gs = grid_stride ? (long)grid_stride : gsz;
+    const long cs = coeff_stride ? (long)coeff_stride : csz;
+    if (batch > 1 && (gs < gsz || cs < 2 * (long)coeff_size(plan, nt)))
+        return fail(ZGL_ERROR_INVALID_VALUE, "instance strides overlap");
+    return guarded([&]() -> zgl_status {
+        if (grid_order) tables_for(plan, grid_order);
+        plan->order = grid_order;
+        if (batch == 0 || nt == 0) return ZGL_SUCCESS;
+        if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (mem == ZGL_MEM_DEVICE) {
+            run_forward(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+            check_hip(hipGetLastError(), "forward");
+            return ZGL_SUCCESS;
+        }
+        const size_t ncw = 2 * coeff_size(plan, nt);  // doubles written per instance
+        ensure(plan->dgrid, plan->dgrid_cap, (size_t)gsz * batch);
+        ensure(plan->dcoef, plan->dcoef_cap, ncw * batch);
+        check_hip(hipMemcpy2DAsync(plan->dgrid, gsz * sizeof(double), grid, gs * sizeof(double),
+                                   gsz * sizeof(double), batch, hipMemcpyHostToDevice, s),
+                  "grid upload");
+        run_forward(plan, plan->dgrid, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+        check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, ncw * sizeof(double),
+                                   ncw * sizeof(double), batch, hipMemcpyDeviceToHost, s),
+                  "coefficient download");
+        check_hip(hipStreamSynchronize(s), "forward");
+        return ZGL_SUCCESS;
+    });
+}
+
+zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, double* grid,
+                        size_t grid_order, size_t batch, ptrdiff_t coeff_stride,
+                        ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
+    if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    const size_t nt = std::min(exp_order, grid_order);
+    const long gsz = (long)grid_size(plan, grid_order);
+    const long csz = 2 * (long)coeff_size(plan, exp_order);
+    const long gs = grid_stride ? (long)grid_stride : gsz;
+    const long cs = coeff_stride ? (long)coeff_stride : csz;
+    if (batch > 1 && gs < gsz) return fail(ZGL_ERROR_INVALID_VALUE, "instance strides overlap");
+    return guarded([&]() -> zgl_status {
+        if (grid_order) tables_for(plan, grid_order);
+        plan->order = grid_order;
+        if (batch == 0 || grid_order == 0) return ZGL_SUCCESS;
+        if (!grid || (!coeffs && nt > 0)) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (mem == ZGL_MEM_DEVICE) {
+            run_backward(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+            check_hip(hipGetLastError(), "backward");
+            return ZGL_SUCCESS;
+        }
+        const size_t ncr = 2 * coeff_size(plan, nt);  // doubles read per instance
+        ensure(plan->dgrid, plan->dgrid_cap, (size_t)gsz * batch);
+        ensure(plan->dcoef, plan->dcoef_cap, std::max<size_t>(ncr, 1) * batch);
+        if (ncr)
+            check_hip(hipMemcpy2DAsync(plan->dcoef, ncr * sizeof(double), coeffs, cs * sizeof(double),
+                                       ncr * sizeof(double), batch, hipMemcpyHostToDevice, s),
+                      "coefficient upload");
+        run_backward(plan, plan->dcoef, nt, plan->dgrid, grid_order, batch, (long)ncr, gsz, s);
+        check_hip(hipMemcpy2DAsync(grid, gs * sizeof(double), plan->dgrid, gsz * sizeof(double),
+                                   gsz * sizeof(double), batch, hipMemcpyDeviceToHost, s),
+                  "grid download");
+        check_hip(hipStreamSynchronize(s), "backward");
+        return ZGL_SUCCESS;
+    });
+}
+
+zgl_status zgl_profile_enable(zgl_plan* plan, int enable) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    plan->profiling = enable != 0;
+    return ZGL_SUCCESS;
+}
+
+zgl_status zgl_profile_stats(zgl_plan* plan, zgl_stage_stats* out, int max_stages, int* num_stages) {
+    if (!plan || !num_stages || (max_stages > 0 && !out))
+        return fail(ZGL_ERROR_INVALID_VALUE, "NULL argument");
+    return guarded([&]() -> zgl_status {
+        long long launches[ST_COUNT] = {}, inst[ST_COUNT] = {};
+        double ms[ST_COUNT] = {};
+        for (auto& r : plan->recs) {
+            check_hip(hipEventSynchronize(r.stop), "event wait");
+            float t = 0.f;
+            check_hip(hipEventElapsedTime(&t, r.start, r.stop), "event time");
+            launches[r.stage] += 1;
+            inst[r.stage] += r.nb;
+            ms[r.stage] += t;
+            plan->pool.push_back(r.start);
+            plan->pool.push_back(r.stop);
+        }
+        plan->recs.clear();
+        int n = 0;
+        for (int st = 0; st < ST_COUNT; ++st) {
+            if (!launches[st]) continue;
+            if (n < max_stages) {
+                zgl_stage_stats& o = out[n];
+                std::snprintf(o.name, sizeof(o.name), "%s", kStageNames[st]);
+                o.launches = launches[st];
+                o.instances = inst[st];
+                o.total_ms = ms[st];
+            }
+            ++n;
+        }
+        *num_stages = n;
+        return ZGL_SUCCESS;
+    });
+}
+
+const char* zgl_last_error_string(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

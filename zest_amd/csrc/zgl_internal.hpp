@@ -1,0 +1,135 @@
+// Internal declarations shared by the zgl HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "zgl.h"
+
+namespace zgl {
+
+constexpr int kMaxPass = 16;
+
+// Error carrying a zgl_status across internal helpers.
+struct Error {
+    zgl_status status;
+    std::string what;
+};
+
+void check_hip(hipError_t e, const char* what);  // throws Error
+
+// Mixed-radix DIT plan of one real-DFT length (device tables + pass list).
+struct FftDesc {
+    int n;
+    int npass;
+    int radix[kMaxPass];  // radix of each pass, in execution order
+    int L[kMaxPass];      // product of the radices before the pass
+};
+
+struct FftPlan {
+    FftDesc desc{};
+    double2* tw = nullptr;          // e^{-2 pi i t / n}, t < n
+    unsigned short* perm = nullptr; // LDS slot of input sample i
+    int tile = 16;                  // transforms (column pairs) per workgroup
+};
+
+// Geometry of one order for a plan, both transform kinds.
+struct Geom {
+    int order = 0;
+    int nphi = 0;   // FFT length (longitudes)
+    int nth = 0;    // latitudes
+    int ncol = 1;   // columns per latitude (radial nodes for Zernike, 1 for SH)
+    int jh = 0;     // half-latitude count (nth + 1) / 2
+    int nm = 0;     // longitudinal modes kept (= order)
+    int lat_stride = 0;  // distance between latitudes in the grid (doubles)
+    int lon_stride = 0;  // distance between longitudes in the grid (doubles)
+};
+
+// Device-resident tables of one order.
+struct OrderTables {
+    Geom g;
+    double* wlat = nullptr;  // [jh] weights of the north-half latitude nodes
+    double* wrad = nullptr;  // [ncol] r^2 v of the radial nodes (Zernike)
+    double* ptab = nullptr;  // P_lm(z_h), parity-split [m][par][l'][h]
+    double* rtab = nullptr;  // R_nl(r_k), zest layout [((n+1)^2>>2)+(l>>1)][nr]
+    FftPlan fft;
+    int4* work_lf = nullptr; int n_lf = 0;  // (problem, m0) tiles, forward Legendre
+    int4* work_lb = nullptr; int n_lb = 0;  // backward Legendre
+    int4* work_rf = nullptr; int n_rf = 0;  // forward radial
+    int4* work_rb = nullptr; int n_rb = 0;  // backward radial
+};
+
+// Offsets of the parity-split P table: rows l = m + par + 2 l' < order.
+__host__ __device__ inline int plm_count(int order, int m, int par) {
+    const int r = order - m - par;
+    return r > 0 ? (r + 1) / 2 : 0;
+}
+__host__ __device__ inline long plm_offset(int order, int m, int par, int jh) {
+    long rows = (long)m * order - ((long)m * (m - 1)) / 2;  // sum_{m'<m} (order - m')
+    if (par) rows += plm_count(order, m, 0);
+    return rows * jh;
+}
+__host__ __device__ inline long tri_idx(long l, long m) { return ((l * (l + 1)) >> 1) + m; }
+__host__ __device__ inline long rad_idx(long n, long l) { return (((n + 1) * (n + 1)) >> 2) + (l >> 1); }
+__host__ __device__ inline long tet_idx(long n, long l, long m) {
+    return (n + 1) * (n + 3) * (2 * n + 1) / 24 + ((l * l) >> 2) + m;
+}
+
+// ---- table generation (zgl_tables.hip) ----
+void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm,
+                  int sh_phase, int layout, int order, hipStream_t s);
+void free_tables(OrderTables& t);
+
+// ---- FFT (zgl_fft.hip) ----
+void fft_plan_create(FftPlan& p, int n, hipStream_t s);
+void fft_plan_destroy(FftPlan& p);
+
+enum GridLayoutKind { GRID_COLS = 0, GRID_ROWS = 1 };
+// GRID_COLS: sample i of latitude j, column c at i*lon_stride + j*lat_stride + c
+//            (Zernike LonLatRad and SH LonLat; the transform axis is the slow one).
+// GRID_ROWS: sample i of latitude j at j*lat_stride + i (SH LatLon).
+
+struct FftLaunch {
+    const Geom* g;
+    const OrderTables* t;
+    int layout_kind;
+    int nm_used;            // modes written (forward) or read (backward)
+    double scale;           // forward prefactor
+    double center_factor;   // weight factor of an equator node (zest's SH quirk: 2)
+};
+
+void fft_forward(const FftLaunch& L, const double* grid, long grid_stride,
+                 double2* spec, long spec_stride, int nb, hipStream_t s);
+void fft_backward(const FftLaunch& L, const double2* spec, long spec_stride,
+                  double* grid, long grid_stride, int nb, hipStream_t s);
+
+// ---- contractions (zgl_gemm.hip) ----
+struct ZtGemmArgs {
+    const OrderTables* t;
+    int nt;                 // truncated order min(exp_order, grid_order)
+    int zernike_unnormed;   // scale coefficients by 2n+3
+    double2* spec; long spec_stride;
+    double2* flm; long flm_stride;
+    double* coeffs; long coeff_stride;  // doubles
+    int nb;
+};
+void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
+void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
+void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);
+void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s);
+
+struct StGemmArgs {
+    const OrderTables* t;
+    int nt;
+    double2* spec; long spec_stride;
+    double* coeffs; long coeff_stride;  // doubles
+    int nb;
+};
+void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
+void st_legendre_backward(const StGemmArgs& a, hipStream_t s);
+
+}  // namespace zgl

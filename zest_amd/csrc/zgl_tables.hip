@@ -1,0 +1,302 @@
+// Device generation of the FP64 tables of one order: Gauss-Legendre nodes and
+// weights, the associated Legendre table P_lm(z_j) and the radial Zernike
+// table R_nl(r_k). All tables are built once per (plan, order) and cached.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "zgl_internal.hpp"
+
+namespace zgl {
+
+namespace {
+
+__device__ inline double sqrl(int k) { return sqrt((double)k); }
+
+// Legendre P_n and P_n' by the three-term recurrence.
+__device__ void legendre_p_dp(int n, double x, double& p, double& dp) {
+    double p0 = 1.0, p1 = x;
+    if (n == 0) { p = 1.0; dp = 0.0; return; }
+    for (int k = 1; k < n; ++k) {
+        double p2 = ((double)(2 * k + 1) * x * p1 - (double)k * p0) / (double)(k + 1);
+        p0 = p1;
+        p1 = p2;
+    }
+    p = p1;
+    dp = (double)n * (x * p1 - p0) / (x * x - 1.0);
+}
+
+// Gauss-Legendre nodes (ascending, cos style) and weights of n points:
+// Newton's method on P_n from Tricomi's initial guess, one thread per node
+// pair. Same node set as gl::gl_nodes_and_weights<UnpackedLayout, cos>
+// (include/gauss_legendre.hpp:1549-1562), which uses tables below 70 nodes
+// and Bogaert's formulas above; all agree to a few ulp.
+__global__ void gl_kernel(int n, double* nodes, double* weights) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (n + 1) / 2) return;
+    double x = cos(M_PI * ((double)k + 0.75) / ((double)n + 0.5));
+    double p, dp;
+    for (int it = 0; it < 100; ++it) {
+        legendre_p_dp(n, x, p, dp);
+        const double dx = p / dp;
+        x -= dx;
+        if (fabs(dx) <= 1e-17 * (fabs(x) + 1e-300)) break;
+    }
+    legendre_p_dp(n, x, p, dp);
+    const double w = 2.0 / ((1.0 - x * x) * dp * dp);
+    if ((n & 1) && k == n / 2) x = 0.0;
+    nodes[n - 1 - k] = x;
+    nodes[k] = -x;
+    weights[n - 1 - k] = w;
+    weights[k] = w;
+}
+
+// North-half weights w[nth/2 + h] and nodes z[nth/2 + h]; for Zernike also
+// the radial weights r_k^2 v_k with r_k = (1 + x_k) / 2
+// (include/zernike_glq_transformer.hpp:538-546, 741-746).
+__global__ void half_nodes_kernel(int nth, const double* z, const double* w, double* zh,
+                                  double* wh) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    const int jh = (nth + 1) / 2;
+    if (h >= jh) return;
+    zh[h] = z[nth / 2 + h];
+    wh[h] = w[nth / 2 + h];
+}
+
+__global__ void radial_nodes_kernel(int nr, double* x, const double* v, double* wr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    const double r = 0.5 * (1.0 + x[k]);
+    x[k] = r;
+    wr[k] = r * r * v[k];
+}
+
+// P_lm(z_h) for one (h, m): the column recursion of st::PlmRecursion::plm_impl
+// (include/plm_recursion.hpp:229-354) with its coefficients
+// (src/plm_recursion.cpp:181-247) and its 1e-280 underflow scaling, real
+// normalisation sqrt(2) for m > 0. Output in the parity-split layout
+// [m][par][l'][h], l = m + par + 2 l'.
+__global__ void plm_kernel(int order, int jh, int qm, double phase, const double* zh,
+                           double* ptab) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = gid % jh;
+    const int m = gid / jh;
+    if (m >= order) return;
+    const double inv_sqrt_4pi = 0.5 / sqrt(M_PI);
+    const double z = zh[h];
+    auto put = [&](int l, double v) {
+        const int par = (l - m) & 1;
+        const int lp = (l - m - par) >> 1;
+        ptab[plm_offset(order, m, par, jh) + (long)lp * jh + h] = v;
+    };
+    if (m == 0) {
+        double p2 = qm ? inv_sqrt_4pi : 1.0;
+        put(0, p2);
+        if (order == 1) return;
+        double p1 = qm ? z * (sqrl(3) * inv_sqrt_4pi) : z * sqrl(3);
+        put(1, p1);
+        for (int l = 2; l < order; ++l) {
+            const double a = sqrl(2 * l - 1) * sqrl(2 * l + 1) / (double)l;
+            const double b = ((double)l - 1.0) * sqrl(2 * l + 1) / (sqrl(2 * l - 3) * (double)l);
+            const double p = a * z * p1 - b * p2;
+            put(l, p);
+            p2 = p1;
+            p1 = p;
+        }
+        return;
+    }
+    const double u = sqrt((1.0 - z) * (1.0 + z));
+    double pmm = qm ? 1.0e-280 * sqrt(2.0) * inv_sqrt_4pi : 1.0e-280 * sqrt(2.0);
+    double us = 1.0 / 1.0e-280;
+    const int msteps = m < order - 1 ? m : order - 2;
+    for (int i = 1; i <= msteps; ++i) {
+        us *= u;
+        pmm *= phase * sqrl(2 * i + 1) / sqrl(2 * i);
+    }
+    if (m == order - 1) {
+        us *= u;
+        put(m, us * (phase * pmm * sqrl(2 * order - 1) / sqrl(2 * order - 2)));
+        return;
+    }
+    double p2 = pmm;
+    double p1 = z * (sqrl(2 * m + 3) * pmm);
+    put(m, p2 * us);
+    put(m + 1, p1 * us);
+    for (int l = m + 2; l < order; ++l) {
+        const double a = sqrl(2 * l - 1) * sqrl(2 * l + 1) / (sqrl(l - m) * sqrl(l + m));
+        const double b = sqrl(2 * l + 1) * sqrl(l + m - 1) * sqrl(l - m - 1) /
+                         (sqrl(l - m) * sqrl(l + m) * sqrl(2 * l - 3));
+        const double p = z * a * p1 - b * p2;
+        put(l, p * us);
+        p2 = p1;
+        p1 = p;
+    }
+}
+
+// R_nl(r_k) for one radial node: the recursion of
+// zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-288)
+// with the coefficients of src/radial_zernike_recursion.cpp:91-116. Every
+// recursion input is unnormalised there, so the normed table is the
+// unnormed one times sqrt(2n+3), applied at the end.
+__global__ void radial_kernel(int order, int nr, int normed, const double* rk, double* rtab) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr || order == 0) return;
+    const double r = rk[k];
+    auto R = [&](int n, int l) -> double& { return rtab[rad_idx(n, l) * nr + k]; };
+    R(0, 0) = 1.0;
+    if (order > 1) R(1, 1) = r;
+    if (order > 2) {
+        R(2, 2) = r * r;
+        R(2, 0) = 2.5 * R(2, 2) - 1.5;
+    }
+    if (order > 3) {
+        R(3, 1) = (3.5 * R(2, 2) - 2.5) * r;
+        R(3, 3) = R(2, 2) * r;
+    }
+    const double r2 = order > 2 ? R(2, 2) : r * r;
+    for (int n = 4; n < order; ++n) {
+        const double dn = (double)n;
+        for (int l = n & 1; l + 4 <= n; l += 2) {
+            const double dl = (double)l;
+            const double c0 = (dn - dl) * (dn + dl + 1.0) * (2.0 * dn - 3.0);
+            const double c1 = (2.0 * dn - 1.0) * (2.0 * dn + 1) * (2.0 * dn - 3.0);
+            const double c2 = -0.5 * (2.0 * dn - 1.0) *
+                              ((2 * dl + 1.0) * (2.0 * dl + 1.0) + (2.0 * dn + 1) * (2.0 * dn - 3.0));
+            const double c3 = -(dn - dl - 2.0) * (dn + dl - 1.0) * (2.0 * dn + 1);
+            const double inv = 1.0 / c0;
+            const double k1 = c1 * inv, k2 = c2 * inv, k3 = c3 * inv;
+            R(n, l) = (k2 + k1 * r2) * R(n - 2, l) + k3 * R(n - 4, l);
+        }
+        R(n, n) = r * R(n - 1, n - 1);
+        R(n, n - 2) = (dn + 0.5) * R(n, n) - (dn - 0.5) * R(n - 2, n - 2);
+    }
+    if (normed)
+        for (int n = 0; n < order; ++n) {
+            const double nm = sqrt(2.0 * n + 3.0);
+            for (int l = n & 1; l <= n; l += 2) R(n, l) *= nm;
+        }
+}
+
+template <class T>
+T* dev_alloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess)
+        throw Error{ZGL_ERROR_OUT_OF_MEMORY, "hipMalloc of a table failed"};
+    return static_cast<T*>(p);
+}
+
+int4* upload_work(const std::vector<int4>& w, hipStream_t s) {
+    int4* d = dev_alloc<int4>(w.size());
+    if (!w.empty())
+        check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s),
+                  "upload work list");
+    return d;
+}
+
+}  // namespace
+
+void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, int sh_phase,
+                  int layout, int order, hipStream_t s) {
+    Geom& g = t.g;
+    g.order = order;
+    const int lon_min = 2 * order - (order > 0 ? 1 : 0);
+    if (kind == ZGL_ZERNIKE) {
+        g.nphi = lon_min;
+        g.nth = order + 1;
+        g.ncol = (order + 1 + 7) / 8 * 8;
+        g.lat_stride = g.ncol;
+        g.lon_stride = g.nth * g.ncol;
+    } else if (layout == ZGL_LAYOUT_LATLON) {
+        g.nphi = (lon_min + 7) / 8 * 8;
+        g.nth = order;
+        g.ncol = 1;
+        g.lat_stride = g.nphi;
+        g.lon_stride = 1;
+    } else {
+        g.nphi = lon_min;
+        g.nth = (order + 7) / 8 * 8;
+        g.ncol = 1;
+        g.lat_stride = 1;
+        g.lon_stride = g.nth;
+    }
+    g.jh = (g.nth + 1) / 2;
+    g.nm = order;
+    if (order == 0) return;
+
+    // Latitude nodes and weights (all of them), then the north half.
+    double* z = dev_alloc<double>(g.nth);
+    double* w = dev_alloc<double>(g.nth);
+    gl_kernel<<<(g.nth + 1) / 2 / 64 + 1, 64, 0, s>>>(g.nth, z, w);
+    double* zh = dev_alloc<double>(g.jh);
+    t.wlat = dev_alloc<double>(g.jh);
+    half_nodes_kernel<<<g.jh / 64 + 1, 64, 0, s>>>(g.nth, z, w, zh, t.wlat);
+
+    // P table.
+    const long ptab_size = plm_offset(order, order - 1, 1, g.jh) + (long)plm_count(order, order - 1, 1) * g.jh;
+    t.ptab = dev_alloc<double>(ptab_size);
+    {
+        const long threads = (long)order * g.jh;
+        plm_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(
+            order, g.jh, sh_norm == ZGL_SHNORM_QM ? 1 : 0, (double)sh_phase, zh, t.ptab);
+    }
+
+    if (kind == ZGL_ZERNIKE) {
+        const int nr = g.ncol;
+        double* x = dev_alloc<double>(nr);
+        double* v = dev_alloc<double>(nr);
+        gl_kernel<<<(nr + 1) / 2 / 64 + 1, 64, 0, s>>>(nr, x, v);
+        t.wrad = dev_alloc<double>(nr);
+        radial_nodes_kernel<<<nr / 64 + 1, 64, 0, s>>>(nr, x, v, t.wrad);
+        const long rsize = (long)((order + 1) * (order + 1) / 4) * nr;
+        t.rtab = dev_alloc<double>(rsize);
+        radial_kernel<<<nr / 64 + 1, 64, 0, s>>>(order, nr, zernike_norm == ZGL_ZNORM_NORMED ? 1 : 0,
+                                                 x, t.rtab);
+        check_hip(hipGetLastError(), "table kernels");
+        check_hip(hipStreamSynchronize(s), "table generation");
+        check_hip(hipFree(x), "free");
+        check_hip(hipFree(v), "free");
+    }
+    check_hip(hipGetLastError(), "table kernels");
+    check_hip(hipStreamSynchronize(s), "table generation");
+    check_hip(hipFree(z), "free");
+    check_hip(hipFree(w), "free");
+    check_hip(hipFree(zh), "free");
+
+    fft_plan_create(t.fft, g.nphi, s);
+
+    // Work lists: (problem, first row of the 32-row tile).
+    std::vector<int4> lf, lb, rf, rb;
+    for (int m = 0; m < order; ++m)
+        for (int par = 0; par < 2; ++par) {
+            const int prob = 2 * m + par;
+            const int cnt = plm_count(order, m, par);
+            for (int m0 = 0; m0 < cnt; m0 += 32) lf.push_back(make_int4(prob, m0, 0, 0));
+            for (int m0 = 0; m0 < g.jh; m0 += 32) lb.push_back(make_int4(prob, m0, 0, 0));
+        }
+    if (kind == ZGL_ZERNIKE) {
+        for (int l = 0; l < order; ++l) {
+            const int ncount = (order - l + 1) / 2;
+            for (int m0 = 0; m0 < ncount; m0 += 32) rf.push_back(make_int4(l, m0, 0, 0));
+            for (int m0 = 0; m0 < 2 * (l + 1); m0 += 32) rb.push_back(make_int4(l, m0, 0, 0));
+        }
+    }
+    t.work_lf = upload_work(lf, s); t.n_lf = (int)lf.size();
+    t.work_lb = upload_work(lb, s); t.n_lb = (int)lb.size();
+    t.work_rf = upload_work(rf, s); t.n_rf = (int)rf.size();
+    t.work_rb = upload_work(rb, s); t.n_rb = (int)rb.size();
+    check_hip(hipStreamSynchronize(s), "work lists");
+}
+
+void free_tables(OrderTables& t) {
+    for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.rtab, (void*)t.work_lf,
+                    (void*)t.work_lb, (void*)t.work_rf, (void*)t.work_rb})
+        if (p) (void)hipFree(p);
+    fft_plan_destroy(t.fft);
+    t = OrderTables{};
+}
+
+}  // namespace zgl
