@@ -1,0 +1,87 @@
+"""The C ABI boundary without a GPU: the library loads, exports every entry
+point include/zgl.h declares, and rejects invalid arguments before touching
+the device. Also the host-side helpers (shapes, the portable generator)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "zgl.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(zgl_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import zest_amd
+    names = declared_symbols()
+    assert "zgl_forward" in names and "zgl_backward" in names
+    for name in names:
+        assert hasattr(zest_amd.LIB, name), name
+    assert set(zest_amd.EXPORTS) <= set(names)
+
+
+def test_invalid_arguments_fail_before_device_work():
+    import zest_amd
+    lib = zest_amd.LIB
+    h = ctypes.c_void_p()
+    assert lib.zgl_plan_create(None, 0, 1, 0, -1, 0, 4, 0) == 1
+    assert lib.zgl_plan_create(ctypes.byref(h), 7, 1, 0, -1, 0, 4, 0) == 1
+    assert lib.zgl_plan_create(ctypes.byref(h), 0, 2, 0, -1, 0, 4, 0) == 1
+    assert b"zernike_norm" in lib.zgl_last_error_string()
+    assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 5, -1, 0, 4, 0) == 1
+    assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, 0, 0, 4, 0) == 1
+    assert lib.zgl_plan_create(ctypes.byref(h), 1, 1, 0, -1, 3, 4, 0) == 1
+    assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, -1, 0, 4, 9) == 4
+    assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, -1, 0, 1 << 20, 0) == 1
+    assert not h.value
+    assert lib.zgl_forward(None, None, 4, None, 4, 1, 0, 0, 0, None) == 1
+    assert lib.zgl_backward(None, None, 4, None, 4, 1, 0, 0, 0, None) == 1
+    assert lib.zgl_plan_resize(None, 3) == 1
+    assert lib.zgl_plan_order(None) == 0
+    assert lib.zgl_plan_destroy(None) == 0
+
+
+def test_shapes_match_zest_layouts():
+    import oracle
+    import zest_amd
+    for order in (0, 1, 2, 7, 8, 32, 128, 255):
+        assert zest_amd.zt_shape(order) == oracle.zt_shape(order)
+        assert zest_amd.zt_num_coeffs(order) == oracle.zt_num_coeffs(order)
+        for layout in (0, 1):
+            assert zest_amd.st_shape(order, layout) == oracle.st_shape(order, layout)
+    # include/zernike_glq_transformer.hpp:107-126 and include/layout.hpp:400-406
+    assert zest_amd.zt_shape(128) == (255, 129, 136)
+    assert zest_amd.zt_num_coeffs(32) == 3128 and zest_amd.zt_num_coeffs(128) == 180960
+    assert zest_amd.st_shape(512) == (1023, 512) and zest_amd.st_shape(512, 1) == (512, 1024)
+
+
+def test_synthetic_generator_is_portable():
+    from zest_amd import synth
+    # SplitMix64 reference outputs for state 0 (Vigna's published generator)
+    z = synth.splitmix64(0, 3)
+    assert [int(v) for v in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    u = synth.uniform(5, 10000)
+    assert u.min() > 0.0 and u.max() <= 1.0 and abs(u.mean() - 0.5) < 0.01
+    x = synth.normal(7, 200000)
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1.0) < 0.01
+    c = synth.zernike_coeffs(6, 0)
+    n, l, m = synth.zernike_nlm(6)
+    assert c.shape == (len(n), 2) and np.all(c[m == 0, 1] == 0.0)
+    assert np.array_equal(c, synth.zernike_coeffs(6, 0))
+    s = synth.sh_coeffs(6, 1)
+    ls, ms = synth.sh_lm(6)
+    assert s.shape == (21, 2) and np.all(s[ms == 0, 1] == 0.0)
+
+
+def test_numpy_arrays_must_be_contiguous_float64():
+    import zest_amd
+    with pytest.raises(ValueError):
+        zest_amd._ptr_and_mem(np.zeros(4, dtype=np.float32))
+    with pytest.raises(ValueError):
+        zest_amd._ptr_and_mem(np.zeros((4, 4))[:, 0])

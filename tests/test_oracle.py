@@ -1,0 +1,330 @@
+"""Pins the oracle (oracle/zgl_oracle.c), the CPU restatement of zest's GLQ
+transforms, before it is trusted as the GPU path's checker.
+
+Every function of the oracle is checked against something it did not compute
+itself: the golden fixtures written by the upstream zest library
+(tests/golden/zest_golden.npz), the compiled zest library directly when it is
+built (oracle/_ref), library routines (numpy leggauss, scipy lpmv and
+eval_jacobi, numpy FFT), closed forms, quadrature identities, and the
+known-answer transforms of zest's own tests
+(tests/test_zernike_glq_transformer.cpp:57-516, tests/test_sh_glq_transformer.cpp:50-431).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import special
+
+import oracle
+from zest_amd import synth
+
+GOLDEN = np.load(os.path.join(os.path.dirname(__file__), "golden", "zest_golden.npz"))
+CONV_ZT = [(zn, sn, sp) for zn in (0, 1) for sn in (0, 1) for sp in (-1, 1)]
+CONV_ST = [(sn, sp) for sn in (0, 1) for sp in (-1, 1)]
+needs_ref = pytest.mark.skipif(oracle.REF is None, reason="zest not compiled (oracle/_ref)")
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+# ---------------------------------------------------------------- GL nodes
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 32, 33, 69, 70, 71, 129, 136, 257, 512])
+def test_gl_nodes_match_leggauss(n):
+    z, w = oracle.gl_nodes_weights(n)
+    zr, wr = np.polynomial.legendre.leggauss(n)
+    # leggauss itself is accurate to ~1e-12 in the weights at n = 512
+    assert np.abs(z - zr).max() < 5e-15
+    assert rel(w, wr) < 5e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 32, 33, 69, 70, 71, 129, 136, 512])
+def test_gl_nodes_match_zest_golden(n):
+    z, w = oracle.gl_nodes_weights(n)
+    assert np.abs(z - GOLDEN[f"gl_{n}_nodes"]).max() < 1e-15
+    assert rel(w, GOLDEN[f"gl_{n}_weights"]) < 2e-14
+
+
+@pytest.mark.parametrize("n", [31, 32, 111, 112])
+def test_gl_invariants(n):
+    """tests/test_gauss_legendre.cpp:127-210: ascending, symmetric, sum 2."""
+    z, w = oracle.gl_nodes_weights(n)
+    assert np.all(np.diff(z) > 0)
+    assert np.array_equal(z, -z[::-1])
+    assert np.array_equal(w, w[::-1])
+    assert abs(w.sum() - 2.0) < 1e-14
+    if n & 1:
+        assert z[n // 2] == 0.0
+    zp, wp = oracle.gl_packed(n)
+    assert np.array_equal(zp, z[n // 2:]) and np.array_equal(wp, w[n // 2:])
+
+
+# ---------------------------------------------------------------- P_lm
+def plm_scipy(l, m, z, sh_norm, sh_phase):
+    """Real-normalised associated Legendre function from scipy.special.lpmv
+    (which carries the (-1)^m Condon-Shortley factor); zest's 'none' phase keeps
+    that factor and 'cs' removes it (include/plm_recursion.hpp:197,308)."""
+    norm = math.sqrt((2 - (m == 0)) * (2 * l + 1) * math.factorial(l - m) / math.factorial(l + m))
+    if sh_norm == 1:
+        norm /= math.sqrt(4 * math.pi)
+    sign = 1.0 if sh_phase == -1 else (-1.0) ** m
+    return sign * norm * special.lpmv(m, l, z)
+
+
+@pytest.mark.parametrize("sh_norm,sh_phase", CONV_ST)
+@pytest.mark.parametrize("z", [1.0, -1.0, 0.0, 0.97443, -0.3, 0.5])
+def test_plm_matches_scipy(sh_norm, sh_phase, z):
+    order = 40
+    tab = oracle.plm_table(sh_norm, sh_phase, order, np.array([z]))[:, 0]
+    for l in range(order):
+        for m in range(l + 1):
+            ref = plm_scipy(l, m, z, sh_norm, sh_phase)
+            assert abs(tab[l * (l + 1) // 2 + m] - ref) < 1e-11 * max(1.0, abs(ref)), (l, m)
+
+
+def test_plm_closed_forms():
+    """l <= 2 closed forms (geo, phase none): P11 = -sqrt3 u, P22 = sqrt(15/4) u^2."""
+    z = np.array([0.3, -0.8])
+    u = np.sqrt(1 - z * z)
+    t = oracle.plm_table(0, -1, 3, z)
+    assert np.allclose(t[0], 1.0, rtol=0, atol=1e-15)
+    assert np.allclose(t[1], math.sqrt(3) * z, rtol=1e-14)
+    assert np.allclose(t[2], -math.sqrt(3) * u, rtol=1e-14)
+    assert np.allclose(t[3], math.sqrt(5 / 4) * (3 * z * z - 1), rtol=1e-14)
+    assert np.allclose(t[5], math.sqrt(15 / 4) * u * u, rtol=1e-14)
+
+
+def test_plm_high_order_underflow_seed():
+    """The 1e-280 scaling keeps high-m values finite and accurate near the poles."""
+    z = np.array([0.999999, 0.0, 0.7])
+    t = oracle.plm_table(0, 1, 600, z)
+    assert np.all(np.isfinite(t))
+    zz, ww = oracle.gl_nodes_weights(600)
+    p = oracle.plm_table(0, 1, 600, zz)
+    # orthogonality of P_lm over the quadrature, geo norm: int P^2 dz = 2 (2 - delta) / ...
+    for (l, m) in [(599, 599), (598, 300), (450, 10)]:
+        v = p[l * (l + 1) // 2 + m]
+        assert abs((v * v * ww).sum() - 2.0 * (2 - (m == 0))) < 1e-10 * 2 * (2 - (m == 0))
+
+
+# ---------------------------------------------------------------- R_nl
+def rnl_scipy(n, l, r, normed):
+    v = r ** l * special.eval_jacobi((n - l) // 2, 0.0, l + 0.5, 2 * r * r - 1)
+    return v * math.sqrt(2 * n + 3) if normed else v
+
+
+@pytest.mark.parametrize("zernike_norm", [0, 1])
+def test_radial_matches_jacobi(zernike_norm):
+    order = 48
+    r = np.array([0.0, 0.3, 0.71, 1.0])
+    tab = oracle.radial_table(zernike_norm, order, r)
+    for n in range(order):
+        for l in range(n & 1, n + 1, 2):
+            ref = rnl_scipy(n, l, r, zernike_norm == 0)
+            got = tab[(n + 1) ** 2 // 4 + l // 2]
+            assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max()), (n, l)
+
+
+def test_radial_endpoints_and_orthonormality():
+    """R_nl(1) = 1 unnormed (tests/test_radial_zernike_recursion.cpp:456-525);
+    int_0^1 R_nl R_n'l r^2 dr = delta / (2n+3) by GL quadrature (244-331)."""
+    order = 30
+    tab1 = oracle.radial_table(1, order, np.array([1.0]))[:, 0]
+    assert np.abs(tab1 - 1.0).max() < 1e-13
+    x, v = oracle.gl_nodes_weights(40)
+    r = 0.5 * (1 + x)
+    w = 0.5 * v * r * r
+    t = oracle.radial_table(0, order, r)
+    for l in range(0, order, 3):
+        ns = list(range(l, order, 2))
+        G = np.array([[(t[(n + 1) ** 2 // 4 + l // 2] * t[(n2 + 1) ** 2 // 4 + l // 2] * w).sum()
+                       for n2 in ns] for n in ns])
+        assert np.abs(G - np.eye(len(ns))).max() < 1e-13
+
+
+# ---------------------------------------------------------------- DFT
+@pytest.mark.parametrize("n", [1, 2, 7, 16, 127, 255])
+def test_rdft_matches_numpy(n):
+    x = synth.normal(n, n * 3).reshape(n, 3)
+    nf = n // 2 + 1
+    got = oracle.rdft_forward(x, nf, 0.5)
+    ref = 0.5 * np.fft.rfft(x, axis=0)
+    assert rel(got[..., 0] + 1j * got[..., 1], ref) < 1e-13
+    back = oracle.rdft_backward(got / 0.5, n)
+    assert rel(back, np.fft.irfft(ref / 0.5, n=n, axis=0) * n) < 1e-13
+
+
+# ---------------------------------------------------------------- transforms vs zest
+@pytest.mark.parametrize("order", [1, 2, 5, 6, 9])
+@pytest.mark.parametrize("conv", CONV_ZT)
+def test_zernike_matches_zest_golden(order, conv):
+    zn, sn, sp = conv
+    key = f"zt_{order}_{zn}{sn}{sp + 1}"
+    if key + "_coeffs" not in GOLDEN:
+        pytest.skip("convention not stored at this order")
+    c, g = GOLDEN[key + "_coeffs"], GOLDEN[key + "_grid"]
+    assert rel(oracle.zt_backward(c, order, zernike_norm=zn, sh_norm=sn, sh_phase=sp), GOLDEN[key + "_bwd"]) < 1e-13
+    assert rel(oracle.zt_forward(g, order, zernike_norm=zn, sh_norm=sn, sh_phase=sp), GOLDEN[key + "_fwd"]) < 1e-13
+
+
+def test_zernike_truncation_matches_zest_golden():
+    g = GOLDEN["zt_trunc_grid"]
+    assert rel(oracle.zt_forward(g, 9, 5), GOLDEN["zt_trunc_fwd5"]) < 1e-13
+    out = np.full((oracle.zt_num_coeffs(12), 2), 7.25)
+    assert rel(oracle.zt_forward(g, 9, 12, out=out), GOLDEN["zt_trunc_fwd12"]) < 1e-13
+    assert rel(oracle.zt_backward(GOLDEN["zt_trunc_c12"], 9, 12), GOLDEN["zt_trunc_bwd12"]) < 1e-13
+
+
+def test_zernike_order32_seed0_matches_zest_golden():
+    """The reference's CPU-runnable case: order 32, seed 0."""
+    c = GOLDEN["zt32_coeffs"]
+    assert np.array_equal(c, synth.zernike_coeffs(32, 0))
+    g = oracle.zt_backward(c, 32)
+    idx = GOLDEN["zt32_grid_idx"]
+    assert rel(g.ravel()[idx], GOLDEN["zt32_grid_val"]) < 1e-13
+    sums = GOLDEN["zt32_grid_sums"]
+    assert abs(g.sum() - sums[0]) < 1e-11 * abs(sums[1]) ** 0.5 * g.size ** 0.5
+    assert abs((g * g).sum() / sums[1] - 1) < 1e-13
+    f = oracle.zt_forward(g, 32)
+    assert rel(f, GOLDEN["zt32_fwd"]) < 1e-13
+    assert rel(f, c) < 1e-13  # the quadrature recovers the seeded coefficients
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_zernike_order128_matches_zest_golden(seed):
+    c = synth.zernike_coeffs(128, seed)
+    g = oracle.zt_backward(c, 128)
+    assert rel(g.ravel()[GOLDEN[f"zt128_{seed}_grid_idx"]], GOLDEN[f"zt128_{seed}_grid_val"]) < 1e-12
+    f = oracle.zt_forward(g, 128)
+    assert rel(f[GOLDEN[f"zt128_{seed}_fwd_idx"]], GOLDEN[f"zt128_{seed}_fwd_val"]) < 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2, 5, 7, 12])
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("conv", CONV_ST)
+def test_sh_matches_zest_golden(order, layout, conv):
+    sn, sp = conv
+    key = f"st_{order}_{layout}_{sn}{sp + 1}"
+    if key + "_coeffs" not in GOLDEN:
+        pytest.skip("convention not stored at this order")
+    c, g = GOLDEN[key + "_coeffs"], GOLDEN[key + "_grid"]
+    assert rel(oracle.st_backward(c, order, sh_norm=sn, sh_phase=sp, layout=layout), GOLDEN[key + "_bwd"]) < 1e-13
+    assert rel(oracle.st_forward(g, order, sh_norm=sn, sh_phase=sp, layout=layout), GOLDEN[key + "_fwd"]) < 1e-13
+
+
+def test_sh_order512_matches_zest_golden():
+    c = synth.sh_coeffs(512, 1)
+    g = oracle.st_backward(c, 512)
+    assert rel(g.ravel()[GOLDEN["st512_grid_idx"]], GOLDEN["st512_grid_val"]) < 1e-12
+    f = oracle.st_forward(g, 512)
+    assert rel(f[GOLDEN["st512_fwd_idx"]], GOLDEN["st512_fwd_val"]) < 1e-12
+
+
+@needs_ref
+@pytest.mark.parametrize("conv", CONV_ZT)
+@pytest.mark.parametrize("order,exp_order", [(3, 3), (8, 8), (13, 13), (10, 6), (6, 10)])
+def test_zernike_matches_compiled_zest(conv, order, exp_order):
+    zn, sn, sp = conv
+    z = oracle.ZestZernike(order, zn, sn, sp)
+    g = synth.normal(order, int(np.prod(oracle.zt_shape(order)))).reshape(oracle.zt_shape(order))
+    c = synth.zernike_coeffs(exp_order, order)
+    out1 = np.full((oracle.zt_num_coeffs(exp_order), 2), 3.0)
+    out2 = out1.copy()
+    oracle.zt_forward(g, order, exp_order, zn, sn, sp, out=out1)
+    z.forward(g, exp_order=exp_order, out=out2)
+    assert rel(out1, out2) < 1e-13
+    assert rel(oracle.zt_backward(c, order, exp_order, zn, sn, sp), z.backward(c, exp_order=exp_order)) < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("conv", CONV_ST)
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("order", [3, 8, 11, 64])
+def test_sh_matches_compiled_zest(conv, layout, order):
+    sn, sp = conv
+    s = oracle.ZestSH(order, sn, sp, layout)
+    shape = oracle.st_shape(order, layout)
+    g = synth.normal(order, int(np.prod(shape))).reshape(shape)
+    c = synth.sh_coeffs(order, order)
+    assert rel(oracle.st_forward(g, order, sh_norm=sn, sh_phase=sp, layout=layout), s.forward(g)) < 1e-13
+    assert rel(oracle.st_backward(c, order, sh_norm=sn, sh_phase=sp, layout=layout), s.backward(c)) < 1e-13
+
+
+# ---------------------------------------------------------------- known answers
+def ball_grid_coords(order):
+    nphi, nth, nr = oracle.zt_shape(order)
+    lon = 2 * np.pi * np.arange(nphi) / nphi
+    z, _ = oracle.gl_nodes_weights(nth)
+    x, _ = oracle.gl_nodes_weights(nr)
+    return lon, z, 0.5 * (1 + x)
+
+
+@pytest.mark.parametrize("conv", [(0, 0, -1), (1, 1, 1)])
+@pytest.mark.parametrize("nlm", [(0, 0, 0), (2, 0, 0), (1, 1, 0), (2, 2, 1), (3, 3, -2), (5, 3, 1)])
+def test_zernike_known_answer(conv, nlm):
+    """A grid of one basis function R_nl P_lm cos(m lon) (sin for m < 0)
+    transforms to one unit coefficient (tests/test_zernike_glq_transformer.cpp:57-516,
+    run there for (normed, geo, none) and (unnormed, qm, cs))."""
+    zn, sn, sp = conv
+    n, l, m = nlm
+    order = 6
+    lon, z, r = ball_grid_coords(order)
+    P = plm_scipy(l, abs(m), z, sn, sp)
+    R = rnl_scipy(n, l, r, zn == 0)
+    ang = np.cos(m * lon) if m >= 0 else np.sin(-m * lon)
+    g = ang[:, None, None] * P[None, :, None] * R[None, None, :]
+    c = oracle.zt_forward(g, order, zernike_norm=zn, sh_norm=sn, sh_phase=sp)
+    expect = np.zeros_like(c)
+    idx = (n + 1) * (n + 3) * (2 * n + 1) // 24 + l * l // 4 + abs(m)
+    expect[idx, 0 if m >= 0 else 1] = 1.0
+    assert np.abs(c - expect).max() < 1e-12
+    # and the backward transform reproduces the grid (518-936)
+    assert np.abs(oracle.zt_backward(expect, order, zernike_norm=zn, sh_norm=sn, sh_phase=sp) - g).max() < 1e-12
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("lm", [(0, 0), (1, 0), (2, 1), (3, 1), (4, -3)])
+def test_sh_known_answer(layout, lm):
+    """tests/test_sh_glq_transformer.cpp:50-431, all conventions."""
+    l, m = lm
+    order = 6
+    for sn, sp in CONV_ST:
+        shape = oracle.st_shape(order, layout)
+        nlon = shape[0] if layout == 0 else shape[1]
+        nlat = shape[1] if layout == 0 else shape[0]
+        lon = 2 * np.pi * np.arange(nlon) / nlon
+        z, _ = oracle.gl_nodes_weights(nlat)
+        P = plm_scipy(l, abs(m), z, sn, sp)
+        ang = np.cos(m * lon) if m >= 0 else np.sin(-m * lon)
+        g = ang[:, None] * P[None, :] if layout == 0 else P[:, None] * ang[None, :]
+        c = oracle.st_forward(g, order, sh_norm=sn, sh_phase=sp, layout=layout)
+        expect = np.zeros_like(c)
+        expect[l * (l + 1) // 2 + abs(m), 0 if m >= 0 else 1] = 1.0
+        if layout == 1 and nlat & 1:
+            # zest counts the equator node twice on odd latitude counts
+            # (include/sh_glq_transformer.hpp:857-881); odd l+m vanish there.
+            continue
+        assert np.abs(c - expect).max() < 1e-12
+
+
+def test_zernike_brute_force_quadrature():
+    """Forward = direct triple sum of g * basis * weights (no FFT, no recursion)."""
+    order = 5
+    nphi, nth, nr = oracle.zt_shape(order)
+    lon, z, r = ball_grid_coords(order)
+    _, wz = oracle.gl_nodes_weights(nth)
+    _, wr = oracle.gl_nodes_weights(nr)
+    g = synth.normal(5, nphi * nth * nr).reshape(nphi, nth, nr)
+    c = oracle.zt_forward(g, order, zernike_norm=1, sh_norm=1, sh_phase=1)
+    W = (2 * np.pi / nphi) * wz[None, :, None] * (0.5 * wr * r * r)[None, None, :]
+    for n in range(order):
+        for l in range(n & 1, n + 1, 2):
+            R = rnl_scipy(n, l, r, False)
+            for m in range(l + 1):
+                P = plm_scipy(l, m, z, 1, 1)
+                base = P[None, :, None] * R[None, None, :] * W * (2 * n + 3)
+                cc = (g * np.cos(m * lon)[:, None, None] * base).sum()
+                ss = (g * np.sin(m * lon)[:, None, None] * base).sum()
+                idx = (n + 1) * (n + 3) * (2 * n + 1) // 24 + l * l // 4 + m
+                assert abs(c[idx, 0] - cc) < 1e-12 and abs(c[idx, 1] - ss) < 1e-12
