@@ -1,22 +1,28 @@
-"""Benchmark: Zernike GLQ forward + backward transforms per second at order 128.
+"""Benchmark: Zernike GLQ forward + backward transforms per second at order 128,
+mixed precision (FP32 FFT + FP32-MFMA contractions) against FP64.
 
 Workload: 64 independent Zernike expansions at order N = 128 (grids
 255 x 129 x 136 doubles each), conventions (unnormed, geo, none) as zest's own
 benchmarks use (GLQTransformerGeo). Coefficients are N(0,1)/(1+n) from the
 portable SplitMix64 stream, seeds 0..63. One step = one backward transform of
 all 64 expansions followed by one forward transform of the resulting grids,
-with every input already resident in HBM. value = instances * steps / time.
+with every input already resident in HBM. value = instances * steps / time of
+the mixed form; the FP64 form is timed the same way beside it.
 
-The dominant kernel's roofline is taken from HIP events recorded around each
-kernel launch on its own stream during the timed region (zgl_profile_enable).
-The CPU baseline times the upstream zest library (compiled from its sources
-into oracle/_ref) on the host cores over a bounded sample, or the C oracle
-when that library is absent.
+Accuracy gate of the mixed form (err <= 1e-6): the forward transform of the
+FP64 backward grids, max over instances of max|c - c64| / max|c64| against the
+FP64 forward. The plain FP32 forward is used when it meets the gate, else the
+forward with one FP64-residual refinement step.
+
+The dominant kernel's roofline comes from HIP events recorded around each
+kernel launch on its own stream during the timed region. The CPU baseline
+times the upstream zest library (compiled from its sources into oracle/_ref)
+on the host cores over a bounded sample, or the C oracle when that library is
+absent.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -27,42 +33,50 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 
-def stage_work(order: int):
-    """Algorithmic bytes / flops of one instance per stage (one direction)."""
+def stage_work(order: int, real_bytes: int):
+    """Algorithmic bytes / flops of one instance per stage (one direction).
+    The grid is FP64 in memory; the spectrum holds complex values of the
+    compute type."""
     nphi, nth, nr = 2 * order - 1, order + 1, (order + 1 + 7) // 8 * 8
     F = nphi // 2 + 1
     ncoef = (order + 1) * (order + 3) * (2 * order + 1) // 24
-    fft_bytes = nphi * nth * nr * 8 + F * nth * nr * 16
+    fft_bytes = nphi * nth * nr * 8 + F * nth * nr * 2 * real_bytes
     legendre_flops = 2 * (order * (order + 1) // 2) * nth * nr
     radial_flops = 4 * ncoef * nr
     return {
-        "fft_forward": ("hbm", fft_bytes),
-        "fft_backward": ("hbm", fft_bytes),
-        "legendre_forward": ("mfma", legendre_flops),
-        "legendre_backward": ("mfma", legendre_flops),
-        "radial_forward": ("mfma", radial_flops),
-        "radial_backward": ("mfma", radial_flops),
+        "fft_forward": ("hbm", fft_bytes), "fft_backward": ("hbm", fft_bytes),
+        "legendre_forward": ("mfma", legendre_flops), "legendre_backward": ("mfma", legendre_flops),
+        "radial_forward": ("mfma", radial_flops), "radial_backward": ("mfma", radial_flops),
     }
 
 
-def read_stage_stats(zest_amd, plan):
-    class Stats(ctypes.Structure):
-        _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong),
-                    ("instances", ctypes.c_longlong), ("total_ms", ctypes.c_double)]
-    lib = zest_amd.LIB
-    lib.zgl_profile_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats), ctypes.c_int,
-                                      ctypes.POINTER(ctypes.c_int)]
-    arr = (Stats * 16)()
-    n = ctypes.c_int()
-    zest_amd._check(lib.zgl_profile_stats(plan._h, arr, 16, ctypes.byref(n)))
-    return {arr[i].name.decode(): (arr[i].launches, arr[i].instances, arr[i].total_ms)
-            for i in range(n.value)}
+def stage_table(stats, order, dtype):
+    work = stage_work(order, 8 if dtype == "f64" else 4)
+    stages = {}
+    for name, (launches, inst, ms) in stats.items():
+        if name not in work or launches == 0:
+            continue
+        bound, per_inst = work[name]
+        per_launch = per_inst * inst / launches
+        avg_s = ms / launches / 1e3
+        if bound == "hbm":
+            achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+        else:
+            achieved, peak, unit = per_launch / avg_s / 1e12, PEAK_TFLOPS[dtype], "TFLOP/s"
+        stages[name] = {"ms_per_launch": round(ms / launches, 5), "launches": launches,
+                        "instances_per_launch": round(inst / launches, 3), "bound": bound,
+                        "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                        "frac": round(achieved / peak, 4), "total_ms": ms}
+    total = sum(v["total_ms"] for v in stages.values()) or 1.0
+    for v in stages.values():
+        v["share"] = round(v.pop("total_ms") / total, 4)
+    return stages
 
 
-def cpu_baseline(order: int, seconds: float = 12.0):
+def cpu_baseline(order: int, seconds: float):
     """zest (or the oracle) fwd+bwd pairs/s on the host, one transformer per thread."""
     import numpy as np
     import oracle
@@ -99,13 +113,10 @@ def cpu_baseline(order: int, seconds: float = 12.0):
     elapsed = time.perf_counter() - t0
     total = sum(counts)
     return {
-        "value": total / elapsed,
-        "unit": "transforms/s",
-        "cores": threads,
-        "kind": kind,
+        "value": round(total / elapsed, 3), "unit": "transforms/s", "cores": threads, "kind": kind,
         "sample": (f"{total} Zernike backward+forward pairs at order {order} in {elapsed:.1f} s, "
-                   f"{threads} threads, one transformer each; "
-                   + ("zest compiled from source (-O3 -march=x86-64-v3)" if kind == "reference"
+                   f"{threads} threads, one transformer each, FP64; "
+                   + ("zest compiled from its sources (-O3 -march=x86-64-v3)" if kind == "reference"
                       else "C restatement of zest")),
     }
 
@@ -119,11 +130,13 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="time one form only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the transforms are "
-              "independent per instance, so several GPUs would each run this same workload on "
-              "their own share; no multi-GPU figure is produced.", file=sys.stderr)
+              "independent per instance, so each GPU would run this same workload on its own "
+              "share of instances; no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -138,54 +151,71 @@ def main():
     dc = torch.from_numpy(coeffs).to(dev)
     dg = torch.empty((B,) + zest_amd.zt_shape(N), dtype=torch.float64, device=dev)
     dco = torch.zeros_like(dc)
-    plan = zest_amd.Plan(zest_amd.ZERNIKE, N, zernike_norm=1, sh_norm=0, sh_phase=-1)
     stream = torch.cuda.current_stream()
+    conv = dict(zernike_norm=1, sh_norm=0, sh_phase=-1)
+    p64 = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP64, **conv)
+    p32 = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP32, **conv)
+    p32r = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP32_REFINE, **conv)
 
-    def step():
-        plan.backward(dc, dg, N, batch=B, stream=stream)
-        plan.forward(dg, dco, N, batch=B, stream=stream)
+    def relerr(a, ref):
+        return float(((a - ref).abs().amax(dim=(1, 2)) / ref.abs().amax(dim=(1, 2))).max())
 
-    for _ in range(args.warmup):
-        step()
+    # accuracy of the mixed forms against FP64 on the FP64 backward grids
+    p64.backward(dc, dg, N, batch=B, stream=stream)
+    c64 = torch.zeros_like(dc)
+    p64.forward(dg, c64, N, batch=B, stream=stream)
+    c32 = torch.zeros_like(dc)
+    p32.forward(dg, c32, N, batch=B, stream=stream)
+    c32r = torch.zeros_like(dc)
+    p32r.forward(dg, c32r, N, batch=B, stream=stream)
+    g32 = torch.empty_like(dg)
+    p32.backward(dc, g32, N, batch=B, stream=stream)
     torch.cuda.synchronize()
-    # accuracy of the round trip (forward o backward = identity on band-limited input)
-    err = float(((dco - dc).abs().amax(dim=(1, 2)) / dc.abs().amax(dim=(1, 2))).max())
+    err64_roundtrip = relerr(c64, dc)
+    err32 = relerr(c32, c64)
+    err32r = relerr(c32r, c64)
+    err32_bwd = float(((g32 - dg).abs().amax(dim=(1, 2, 3)) / dg.abs().amax(dim=(1, 2, 3))).max())
+    del g32, c32, c32r
+    mixed_fwd = p32 if err32 <= 1e-6 else p32r
+    mixed_name = "fp32" if err32 <= 1e-6 else "fp32+refine"
 
-    zest_amd.LIB.zgl_profile_enable(plan._h, 1)
-    read_stage_stats(zest_amd, plan)  # discard anything recorded before
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stats = read_stage_stats(zest_amd, plan)
-    zest_amd.LIB.zgl_profile_enable(plan._h, 0)
+    def timed(plan_bwd, plan_fwd):
+        def step():
+            plan_bwd.backward(dc, dg, N, batch=B, stream=stream)
+            plan_fwd.forward(dg, dco, N, batch=B, stream=stream)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        for p in (plan_bwd, plan_fwd):
+            p.profile(True)
+            p.stage_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        stats = {}
+        for p in (plan_bwd, plan_fwd) if plan_bwd is not plan_fwd else (plan_bwd,):
+            for k, v in p.stage_stats().items():
+                a = stats.get(k, (0, 0, 0.0))
+                stats[k] = (a[0] + v[0], a[1] + v[1], a[2] + v[2])
+            p.profile(False)
+        return elapsed, stats
 
+    results = {}
+    if args.only in (None, "fp64"):
+        results["fp64"] = timed(p64, p64)
+    if args.only in (None, "mixed"):
+        results["mixed"] = timed(p32, mixed_fwd)
+
+    head = "mixed" if "mixed" in results else "fp64"
+    elapsed, stats = results[head]
+    dtype = "f32" if head == "mixed" else "f64"
     value = B * args.steps / elapsed
-    work = stage_work(N)
-    stages = {}
-    for name, (launches, inst, ms) in stats.items():
-        bound, per_inst = work[name]
-        per_launch = per_inst * inst / launches
-        avg_s = ms / launches / 1e3
-        if bound == "hbm":
-            achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
-        else:
-            achieved, peak, unit = per_launch / avg_s / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
-        stages[name] = {"ms_per_launch": ms / launches, "launches": launches,
-                        "instances_per_launch": inst / launches, "bound": bound,
-                        "achieved": achieved, "peak": peak, "unit": unit,
-                        "frac": achieved / peak, "share": 0.0}
-    total_ms = sum(v["ms_per_launch"] * v["launches"] for v in stages.values())
-    for v in stages.values():
-        v["share"] = v["ms_per_launch"] * v["launches"] / total_ms
+    stages = stage_table(stats, N, dtype)
     dom_name = max(stages, key=lambda k: stages[k]["share"])
     dom = stages[dom_name]
-    roofline = {"bound": dom["bound"], "achieved": round(dom["achieved"], 3), "peak": dom["peak"],
-                "unit": dom["unit"], "frac": round(dom["frac"], 4), "traffic": None,
-                "kernel": dom_name}
-
     out = {
         "metric": "Zernike fwd+bwd transforms/s @order128: FP32-MFMA vs FP64, err<=1e-6",
         "value": round(value, 2),
@@ -197,16 +227,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": "zernike_glq_backward_forward", "order": N, "instances": B,
                    "grid": list(zest_amd.zt_shape(N)), "conventions": "unnormed,geo,none",
-                   "precision": "fp64"},
-        "max_rel_err_roundtrip": err,
-        "roofline": roofline,
-        "stages": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                   for k, v in stages.items()},
+                   "mixed_form": mixed_name if head == "mixed" else None},
+        "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
+                     "unit": dom["unit"], "frac": dom["frac"], "traffic": None,
+                     "kernel": dom_name},
+        "accuracy": {"mixed_fwd_max_rel_err_vs_fp64": err32, "refined_fwd_max_rel_err_vs_fp64": err32r,
+                     "mixed_bwd_max_rel_err_vs_fp64": err32_bwd,
+                     "fp64_roundtrip_max_rel_err": err64_roundtrip},
+        "stages": stages,
     }
+    if "fp64" in results:
+        e64, s64 = results["fp64"]
+        out["fp64_value"] = round(B * args.steps / e64, 2)
+        out["fp64_ms_per_step"] = round(e64 / args.steps * 1e3, 4)
+        out["fp64_stages"] = stage_table(s64, N, "f64")
+        if head == "mixed":
+            out["mixed_vs_fp64"] = round(value / out["fp64_value"], 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -65,8 +65,15 @@ enum { ZGL_PHASE_NONE = -1, ZGL_PHASE_CS = 1 };
 /* SH grid layouts (include/sh_glq_transformer.hpp:44-195) */
 enum { ZGL_LAYOUT_LONLAT = 0, ZGL_LAYOUT_LATLON = 1 };
 
+/* Arithmetic of the transforms. Grids and coefficients are FP64 in memory in
+ * every mode; the tables are always generated in FP64 (the FP32 modes use
+ * their FP32 roundings). */
 typedef enum zgl_precision {
-    ZGL_FP64 = 0 /* FP64 throughout: the drop-in form */
+    ZGL_FP64 = 0,        /* FP64 throughout: the drop-in form, matches zest */
+    ZGL_FP32 = 1,        /* FP32 FFT and FP32-MFMA contractions, FP32 accumulation */
+    ZGL_FP32_REFINE = 2  /* ZGL_FP32, plus one step of iterative refinement on the
+                            forward transform: c += F32(g - B64(c)), the residual
+                            formed by an FP64 backward transform */
 } zgl_precision;
 
 typedef enum zgl_mem {

@@ -81,11 +81,16 @@ void zo_gl_nodes_weights(size_t n, double *nodes, double *weights) {
         double theta = ZO_PI * ((double)k + 0.75) / ((double)n + 0.5);
         double x = cos(theta);
         double p, dp;
+        /* quadratic convergence: one more step once the update is below 1e-15 */
         for (int it = 0; it < 100; ++it) {
             legendre_p_dp(n, x, &p, &dp);
             double dx = p / dp;
             x -= dx;
-            if (fabs(dx) <= 1e-17 * (fabs(x) + 1e-300)) break;
+            if (fabs(dx) <= 1e-15) {
+                legendre_p_dp(n, x, &p, &dp);
+                x -= p / dp;
+                break;
+            }
         }
         legendre_p_dp(n, x, &p, &dp);
         double w = 2.0 / ((1.0 - x * x) * dp * dp);

@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libzgl.so")
 
 ZERNIKE, SH = 0, 1
-FP64 = 0
+FP64, FP32, FP32_REFINE = 0, 1, 2
 MEM_DEVICE, MEM_HOST = 0, 1
 
 _STATUS = {0: "success", 1: "invalid value", 2: "out of memory", 3: "HIP error", 4: "not supported"}
@@ -35,6 +35,14 @@ class ZglError(RuntimeError):
 
 
 def _load():
+    # torch (ROCm build) bundles its own HIP runtime with the same soname as
+    # the system one libzgl.so links. Importing torch first makes libzgl bind
+    # to that already-loaded runtime; loading libzgl first would pull in a
+    # second HIP runtime, which then finds no device in this process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build the HIP library first "
@@ -59,7 +67,17 @@ def _load():
     lib.zgl_backward.restype = i32
     lib.zgl_last_error_string.argtypes = []
     lib.zgl_last_error_string.restype = ctypes.c_char_p
+    lib.zgl_profile_enable.argtypes = [vp, i32]
+    lib.zgl_profile_enable.restype = i32
+    lib.zgl_profile_stats.argtypes = [vp, ctypes.POINTER(StageStats), i32, ctypes.POINTER(i32)]
+    lib.zgl_profile_stats.restype = i32
     return lib
+
+
+class StageStats(ctypes.Structure):
+    """zgl_stage_stats of include/zgl.h."""
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong),
+                ("instances", ctypes.c_longlong), ("total_ms", ctypes.c_double)]
 
 
 LIB = _load()
@@ -68,7 +86,7 @@ LIB = _load()
 EXPORTS = (
     "zgl_plan_create", "zgl_plan_destroy", "zgl_plan_resize", "zgl_plan_order",
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
-    "zgl_last_error_string",
+    "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
 
@@ -144,6 +162,18 @@ class Plan:
 
     def coeff_size(self, order: int) -> int:
         return LIB.zgl_coeff_size(self._h, order)
+
+    def profile(self, enable: bool = True):
+        """Bracket every kernel launch with HIP events (zgl_profile_enable)."""
+        _check(LIB.zgl_profile_enable(self._h, int(enable)))
+
+    def stage_stats(self) -> dict:
+        """{stage: (launches, instances, total_ms)} since the last call."""
+        arr = (StageStats * 16)()
+        n = ctypes.c_int()
+        _check(LIB.zgl_profile_stats(self._h, arr, 16, ctypes.byref(n)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].instances, arr[i].total_ms)
+                for i in range(min(n.value, 16))}
 
     def grid_shape(self, order: int):
         return zt_shape(order) if self.kind == ZERNIKE else st_shape(order, self.layout)

@@ -9,6 +9,7 @@
 #include <map>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "zgl_internal.hpp"
@@ -51,10 +52,14 @@ struct zgl_plan {
     zgl_precision precision;
     size_t order = 0;
     std::map<size_t, zgl::OrderTables> tables;
-    double2* spec = nullptr;
-    size_t spec_cap = 0;  // double2 elements
-    double2* flm = nullptr;
-    size_t flm_cap = 0;
+    void* spec = nullptr;
+    size_t spec_cap = 0;  // bytes
+    void* flm = nullptr;
+    size_t flm_cap = 0;   // bytes
+    double* rgrid = nullptr;  // refinement residual grids (chunk)
+    size_t rgrid_cap = 0;     // doubles
+    double* rcoef = nullptr;  // refinement corrections (chunk)
+    size_t rcoef_cap = 0;     // doubles
     double* dgrid = nullptr;
     size_t dgrid_cap = 0;  // doubles
     double* dcoef = nullptr;
@@ -87,7 +92,8 @@ const OrderTables& tables_for(zgl_plan* p, size_t order) {
     if (it != p->tables.end()) return it->second;
     OrderTables t;
     try {
-        build_tables(t, p->kind, p->znorm, p->shnorm, p->phase, p->layout, (int)order, nullptr);
+        build_tables(t, p->kind, p->znorm, p->shnorm, p->phase, p->layout, (int)order,
+                     p->precision != ZGL_FP64, nullptr);
     } catch (...) {
         free_tables(t);
         throw;
@@ -101,7 +107,16 @@ void ensure(T*& ptr, size_t& cap, size_t need) {
     if (ptr) check_hip(hipFree(ptr), "free scratch");
     ptr = nullptr;
     cap = 0;
-    check_hip(hipMalloc(&ptr, need * sizeof(T)), "scratch allocation");
+    check_hip(hipMalloc((void**)&ptr, need * sizeof(T)), "scratch allocation");
+    cap = need;
+}
+
+void ensure_bytes(void*& ptr, size_t& cap, size_t need) {
+    if (need <= cap) return;
+    if (ptr) check_hip(hipFree(ptr), "free scratch");
+    ptr = nullptr;
+    cap = 0;
+    check_hip(hipMalloc(&ptr, need), "scratch allocation");
     cap = need;
 }
 
@@ -111,11 +126,11 @@ struct Chunking {
     long flm_stride;   // double2
 };
 
-Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch) {
+Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) {
     Chunking c;
     c.spec_stride = (long)g.nm * 2 * g.jh * g.ncol;
     c.flm_stride = p->kind == ZGL_ZERNIKE ? (long)g.order * (g.order + 1) / 2 * g.ncol : 0;
-    const size_t per = (size_t)(c.spec_stride + c.flm_stride) * sizeof(double2);
+    const size_t per = (size_t)(c.spec_stride + c.flm_stride) * csize;
     size_t budget, cap;
     if (p->kind == ZGL_ZERNIKE) {
         // Keep a chunk's intermediates within the 256 MB Infinity Cache.
@@ -163,13 +178,15 @@ void timed(zgl_plan* p, Stage st, int nb, hipStream_t s, F&& launch) {
 
 double shnorm_value(int sh_norm) { return sh_norm == ZGL_SHNORM_QM ? 1.0 : 1.0 / (4.0 * M_PI); }
 
+template <class T>
 void run_forward(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
                  size_t batch, long gs, long cs, hipStream_t s) {
+    using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
     const Geom& g = t.g;
-    const Chunking ch = chunking(p, g, batch);
-    ensure(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb);
-    if (ch.flm_stride) ensure(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb);
+    const Chunking ch = chunking(p, g, batch, sizeof(C));
+    ensure_bytes(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb * sizeof(C));
+    if (ch.flm_stride) ensure_bytes(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb * sizeof(C));
     FftLaunch L;
     L.g = &g;
     L.t = &t;
@@ -181,26 +198,28 @@ void run_forward(zgl_plan* p, const double* grid, size_t N, double* coeffs, size
     L.center_factor = p->kind == ZGL_ZERNIKE ? 1.0 : 2.0;
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
-        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
+        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T>(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
         if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0, p->spec, ch.spec_stride,
                          p->flm, ch.flm_stride, coeffs + b0 * cs, cs, nb};
-            timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward(a, s); });
-            timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward(a, s); });
+            timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
+            timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward<T>(a, s); });
         } else {
             StGemmArgs a{&t, (int)nt, p->spec, ch.spec_stride, coeffs + b0 * cs, cs, nb};
-            timed(p, ST_LEG_FWD, nb, s, [&] { st_legendre_forward(a, s); });
+            timed(p, ST_LEG_FWD, nb, s, [&] { st_legendre_forward<T>(a, s); });
         }
     }
 }
 
+template <class T>
 void run_backward(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
                   size_t batch, long cs, long gs, hipStream_t s) {
+    using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
     const Geom& g = t.g;
-    const Chunking ch = chunking(p, g, batch);
-    ensure(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb);
-    if (ch.flm_stride) ensure(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb);
+    const Chunking ch = chunking(p, g, batch, sizeof(C));
+    ensure_bytes(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb * sizeof(C));
+    if (ch.flm_stride) ensure_bytes(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb * sizeof(C));
     FftLaunch L;
     L.g = &g;
     L.t = &t;
@@ -213,15 +232,53 @@ void run_backward(zgl_plan* p, const double* coeffs, size_t nt, double* grid, si
         if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, 0, p->spec, ch.spec_stride, p->flm, ch.flm_stride,
                          const_cast<double*>(coeffs) + b0 * cs, cs, nb};
-            timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward(a, s); });
-            timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward(a, s); });
+            timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward<T>(a, s); });
+            timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward<T>(a, s); });
         } else {
             StGemmArgs a{&t, (int)nt, p->spec, ch.spec_stride, const_cast<double*>(coeffs) + b0 * cs,
                          cs, nb};
-            timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward(a, s); });
+            timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward<T>(a, s); });
         }
-        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward(L, p->spec, ch.spec_stride, grid + b0 * gs, gs, nb, s); });
+        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward<T>(L, p->spec, ch.spec_stride, grid + b0 * gs, gs, nb, s); });
     }
+}
+
+// Forward in the plan's precision. With ZGL_FP32_REFINE each chunk of
+// instances is refined once: c += F32(g - B64(c)).
+void forward_any(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
+                 size_t batch, long gs, long cs, hipStream_t s) {
+    if (p->precision == ZGL_FP64) {
+        run_forward<double>(p, grid, N, coeffs, nt, batch, gs, cs, s);
+        return;
+    }
+    if (p->precision == ZGL_FP32) {
+        run_forward<float>(p, grid, N, coeffs, nt, batch, gs, cs, s);
+        return;
+    }
+    const long gsz = (long)grid_size(p, N);
+    const long ncw = 2 * (long)coeff_size(p, nt);
+    const Chunking ch = chunking(p, tables_for(p, N).g, batch, sizeof(float2));
+    const size_t nb_max = (size_t)ch.nb;
+    ensure(p->rgrid, p->rgrid_cap, (size_t)gsz * nb_max);
+    ensure(p->rcoef, p->rcoef_cap, (size_t)ncw * nb_max);
+    for (size_t b0 = 0; b0 < batch; b0 += nb_max) {
+        const int nb = (int)std::min(nb_max, batch - b0);
+        const double* g = grid + b0 * gs;
+        double* c = coeffs + b0 * cs;
+        run_forward<float>(p, g, N, c, nt, nb, gs, cs, s);
+        run_backward<double>(p, c, nt, p->rgrid, N, nb, cs, gsz, s);
+        grid_residual(g, gs, p->rgrid, gsz, gsz, nb, s);
+        run_forward<float>(p, p->rgrid, N, p->rcoef, nt, nb, gsz, ncw, s);
+        coeff_accumulate(c, cs, p->rcoef, ncw, ncw, nb, s);
+    }
+}
+
+void backward_any(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
+                  size_t batch, long cs, long gs, hipStream_t s) {
+    if (p->precision == ZGL_FP64)
+        run_backward<double>(p, coeffs, nt, grid, N, batch, cs, gs, s);
+    else
+        run_backward<float>(p, coeffs, nt, grid, N, batch, cs, gs, s);
 }
 
 template <class F>
@@ -263,7 +320,8 @@ zgl_status zgl_plan_create(zgl_plan** plan, zgl_kind kind, int zernike_norm, int
     *plan = nullptr;
     zgl_status st = check_conventions(kind, zernike_norm, sh_norm, sh_phase, layout);
     if (st != ZGL_SUCCESS) return st;
-    if (precision != ZGL_FP64) return fail(ZGL_ERROR_NOT_SUPPORTED, "unsupported precision");
+    if (precision != ZGL_FP64 && precision != ZGL_FP32 && precision != ZGL_FP32_REFINE)
+        return fail(ZGL_ERROR_NOT_SUPPORTED, "unsupported precision");
     if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
     return guarded([&]() -> zgl_status {
         zgl_plan* p = new zgl_plan;
@@ -294,7 +352,8 @@ zgl_status zgl_plan_destroy(zgl_plan* plan) {
         (void)hipEventDestroy(r.stop);
     }
     for (auto e : plan->pool) (void)hipEventDestroy(e);
-    for (void* q : {(void*)plan->spec, (void*)plan->flm, (void*)plan->dgrid, (void*)plan->dcoef})
+    for (void* q : {(void*)plan->spec, (void*)plan->flm, (void*)plan->dgrid, (void*)plan->dcoef,
+                    (void*)plan->rgrid, (void*)plan->rcoef})
         if (q) (void)hipFree(q);
     delete plan;
     return ZGL_SUCCESS;
@@ -336,7 +395,7 @@ zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, do
         if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            run_forward(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+            forward_any(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
             check_hip(hipGetLastError(), "forward");
             return ZGL_SUCCESS;
         }
@@ -346,7 +405,7 @@ zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, do
         check_hip(hipMemcpy2DAsync(plan->dgrid, gsz * sizeof(double), grid, gs * sizeof(double),
                                    gsz * sizeof(double), batch, hipMemcpyHostToDevice, s),
                   "grid upload");
-        run_forward(plan, plan->dgrid, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+        forward_any(plan, plan->dgrid, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
         check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, ncw * sizeof(double),
                                    ncw * sizeof(double), batch, hipMemcpyDeviceToHost, s),
                   "coefficient download");
@@ -374,7 +433,7 @@ zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, 
         if (!grid || (!coeffs && nt > 0)) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            run_backward(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+            backward_any(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
             check_hip(hipGetLastError(), "backward");
             return ZGL_SUCCESS;
         }
@@ -385,7 +444,7 @@ zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, 
             check_hip(hipMemcpy2DAsync(plan->dcoef, ncr * sizeof(double), coeffs, cs * sizeof(double),
                                        ncr * sizeof(double), batch, hipMemcpyHostToDevice, s),
                       "coefficient upload");
-        run_backward(plan, plan->dcoef, nt, plan->dgrid, grid_order, batch, (long)ncr, gsz, s);
+        backward_any(plan, plan->dcoef, nt, plan->dgrid, grid_order, batch, (long)ncr, gsz, s);
         check_hip(hipMemcpy2DAsync(grid, gs * sizeof(double), plan->dgrid, gsz * sizeof(double),
                                    gsz * sizeof(double), batch, hipMemcpyDeviceToHost, s),
                   "grid download");

@@ -2,7 +2,7 @@
 // length (odd lengths 2N-1 are often prime), fused with the quadrature
 // weights, the conjugation and the north/south parity split (forward), or
 // with the parity merge, the m > 0 halving and the Hermitian extension
-// (backward).
+// (backward). FP64 or FP32 arithmetic; grids are always FP64 in memory.
 //
 // Replaces pocketfft::r2c / c2r at the call sites
 //   include/zernike_glq_transformer.hpp:710-776 (r2c + apply_weights)
@@ -10,17 +10,20 @@
 //   include/sh_glq_transformer.hpp:757-884, 1136-1201 (r2c + fft_to_symm_asymm,
 //                                                     symm_asymm_to_fft + c2r)
 //
-// Design. A workgroup owns T "column pairs": the columns of a latitude j
-// and of its mirror nth-1-j at the same radial node. The two real columns are
-// packed as one complex sequence z = a + i b (the classic two-for-one real
-// FFT), so one complex FFT yields both spectra, and the pair is exactly what
-// the parity split needs. The T sequences live in LDS as [n][T] complex
-// doubles (lanes of a wave touch consecutive columns: conflict-free), and are
-// transformed in place by a mixed-radix decimation-in-time FFT whose input is
-// stored in digit-reversed order during the load. Each pass of radix p runs
-// p-point DFTs in registers (the symmetric cos/sin form for odd p) and folds
-// the next pass's twiddle factors into its stores. Prime radices above 17 use
-// an LDS-staged variant that computes two outputs per work item.
+// Design. A workgroup owns T "column pairs": the columns of a latitude j and
+// of its mirror nth-1-j at the same radial node. The two real columns are
+// packed as one complex sequence z = a + i b (the two-for-one real FFT), so
+// one complex FFT yields both spectra, and the pair is exactly what the parity
+// split needs. For Zernike grids a tile holds T consecutive radial nodes of one
+// latitude pair, so its loads are contiguous runs of T doubles per longitude
+// row. The T sequences live in LDS as [n][T] complex values (lanes of a wave
+// touch consecutive columns: conflict-free) and are transformed in place by a
+// mixed-radix decimation-in-time FFT: the input is stored in digit-reversed
+// order during the load, and each pass of radix p multiplies its inputs by a
+// precomputed per-slot twiddle table (kept in LDS, so no index arithmetic
+// beyond the butterfly base) and runs p-point DFTs in registers (the
+// symmetric cos/sin form for odd p). Prime radices above 17 use an LDS-staged
+// variant that computes two outputs per work item.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,37 +40,30 @@ constexpr int kThreads = 256;
 constexpr int kMaxBigItems = 9;
 constexpr int kMaxRegRadix = 17;  // larger primes use the LDS-staged pass
 
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <class R> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+
+template <class C> __device__ inline C mk(decltype(C{}.x) x, decltype(C{}.x) y) { C c; c.x = x; c.y = y; return c; }
+template <class C> __device__ inline C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <class C> __device__ inline C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <class C> __device__ inline C cmul(C a, C b) {
+    return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Twiddle of the next pass for LDS slot `pos`: in a pass of radix pn whose
-// sub-transforms have length Ln, slot pos = B*pn*Ln + r*Ln + q takes
-// w^(r q) with w = e^{-/+ 2 pi i / (pn Ln)}.
-template <int DIR>
-__device__ inline double2 next_twiddle(const double2* twl, int n, int pos, int Ln, int pn) {
-    const int q = pos % Ln;
-    const int r = (pos / Ln) % pn;
-    const int idx = (r * q * (n / (Ln * pn))) % n;
-    double2 w = twl[idx];
-    if (DIR > 0) w.y = -w.y;
-    return w;
-}
-
-// In-register p-point DFT, y_s = sum_r x_r e^{DIR 2 pi i r s / p}.
-template <int P, int DIR>
-__device__ inline void dft_small(double2 (&x)[P], const double2* twl, int n) {
+// In-register p-point DFT, y_s = sum_r x_r e^{DIR 2 pi i r s / p}; `twl` holds
+// e^{-2 pi i t / n} and p divides n.
+template <int P, int DIR, class C>
+__device__ inline void dft_small(C (&x)[P], const C* twl, int n) {
+    using R = decltype(C{}.x);
     if constexpr (P == 2) {
-        const double2 a = x[0], b = x[1];
+        const C a = x[0], b = x[1];
         x[0] = cadd(a, b);
         x[1] = csub(a, b);
     } else if constexpr (P == 4) {
-        const double2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
-        const double2 c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
-        // forward: y1 = b - i d, y3 = b + i d
-        const double2 mid = make_double2(d.y, -d.x);  // -i d
+        const C a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+        const C c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+        const C mid = mk<C>(d.y, -d.x);  // -i d
         x[0] = cadd(a, c);
         x[2] = csub(a, c);
         if (DIR < 0) {
@@ -79,39 +75,39 @@ __device__ inline void dft_small(double2 (&x)[P], const double2* twl, int n) {
         }
     } else {
         constexpr int H = (P - 1) / 2;
-        double cs[H + 1], sn[H + 1];
+        R cs[H + 1], sn[H + 1];
 #pragma unroll
         for (int t = 1; t <= H; ++t) {
-            const double2 w = twl[t * (n / P)];  // e^{-2 pi i t / P}
+            const C w = twl[t * (n / P)];
             cs[t] = w.x;
             sn[t] = -w.y;
         }
-        const double2 x0 = x[0];
-        double2 y0 = x0;
+        const C x0 = x[0];
+        C y0 = x0;
 #pragma unroll
         for (int r = 1; r <= H; ++r) {
-            const double2 u = cadd(x[r], x[P - r]);
-            const double2 v = csub(x[r], x[P - r]);
+            const C u = cadd(x[r], x[P - r]);
+            const C v = csub(x[r], x[P - r]);
             x[r] = u;
             x[P - r] = v;
             y0 = cadd(y0, u);
         }
-        double2 ys[H + 1], yc[H + 1];
+        C ys[H + 1], yc[H + 1];
 #pragma unroll
         for (int s = 1; s <= H; ++s) {
-            double2 a = x0, b = make_double2(0.0, 0.0);
+            C a = x0, b = mk<C>(0, 0);
 #pragma unroll
             for (int r = 1; r <= H; ++r) {
                 const int t = (r * s) % P;
-                const double c = (t <= H) ? cs[t] : cs[P - t];
-                const double si = (t <= H) ? sn[t] : -sn[P - t];
+                const R c = (t <= H) ? cs[t] : cs[P - t];
+                const R si = (t <= H) ? sn[t] : -sn[P - t];
                 a.x += x[r].x * c;
                 a.y += x[r].y * c;
                 b.x += x[P - r].x * si;
                 b.y += x[P - r].y * si;
             }
             // forward: y_s = a - i b, y_{P-s} = a + i b
-            const double2 mib = make_double2(b.y, -b.x);
+            const C mib = mk<C>(b.y, -b.x);
             if (DIR < 0) {
                 ys[s] = cadd(a, mib);
                 yc[s] = csub(a, mib);
@@ -129,8 +125,17 @@ __device__ inline void dft_small(double2 (&x)[P], const double2* twl, int n) {
     }
 }
 
-template <int P, int T, int DIR>
-__device__ void pass_reg(double2* data, const double2* twl, int n, int L, int pn, int Ln) {
+// Input twiddle of slot `pos` in a pass; `tw` is that pass's table (or null).
+template <int DIR, class C>
+__device__ inline C twiddled(C v, const C* tw, int pos) {
+    if (!tw) return v;
+    C w = tw[pos];
+    if (DIR > 0) w.y = -w.y;
+    return cmul(v, w);
+}
+
+template <int P, int T, int DIR, class C>
+__device__ void pass_reg(C* data, const C* twl, const C* tw, int n, int L) {
     const int nbf = n / P;
     for (int it = threadIdx.x; it < nbf * T; it += kThreads) {
         const int t = it % T;
@@ -138,30 +143,29 @@ __device__ void pass_reg(double2* data, const double2* twl, int n, int L, int pn
         const int bq = bf / L;
         const int q = bf - bq * L;
         const int base = bq * P * L + q;
-        double2 x[P];
+        C x[P];
 #pragma unroll
-        for (int r = 0; r < P; ++r) x[r] = data[(base + r * L) * T + t];
+        for (int r = 0; r < P; ++r) {
+            const int pos = base + r * L;
+            x[r] = r ? twiddled<DIR>(data[pos * T + t], tw, pos) : data[pos * T + t];
+        }
         dft_small<P, DIR>(x, twl, n);
 #pragma unroll
-        for (int s = 0; s < P; ++s) {
-            const int pos = base + s * L;
-            double2 y = x[s];
-            if (pn) y = cmul(y, next_twiddle<DIR>(twl, n, pos, Ln, pn));
-            data[pos * T + t] = y;
-        }
+        for (int s = 0; s < P; ++s) data[(base + s * L) * T + t] = x[s];
     }
 }
 
 // Prime radix P > 17: work item = (butterfly, transform, s in [0, H]) producing
-// y_s and y_{P-s}; results are held in registers across a barrier because
-// the DFT reads every input slot of its butterfly.
-template <int T, int DIR>
-__device__ void pass_big(double2* data, const double2* twl, int n, int P, int L, int pn, int Ln) {
+// y_s and y_{P-s}; results are held in registers across a barrier because the
+// DFT reads every input slot of its butterfly.
+template <int T, int DIR, class C>
+__device__ void pass_big(C* data, const C* twl, const C* tw, int n, int P, int L) {
+    using R = decltype(C{}.x);
     const int H = (P - 1) / 2;
     const int nbf = n / P;
     const int items = nbf * T * (H + 1);
     const int stride = n / P;
-    double2 ya[kMaxBigItems], yb[kMaxBigItems];
+    C ya[kMaxBigItems], yb[kMaxBigItems];
 #pragma unroll
     for (int c = 0; c < kMaxBigItems; ++c) {
         const int it = threadIdx.x + c * kThreads;
@@ -173,28 +177,32 @@ __device__ void pass_big(double2* data, const double2* twl, int n, int P, int L,
         const int bq = bf / L;
         const int q = bf - bq * L;
         const int base = bq * P * L + q;
-        const double2 x0 = data[base * T + t];
+        const C x0 = data[base * T + t];
         if (s == 0) {
-            double2 y = x0;
-            for (int r = 1; r < P; ++r) y = cadd(y, data[(base + r * L) * T + t]);
+            C y = x0;
+            for (int r = 1; r < P; ++r) {
+                const int pos = base + r * L;
+                y = cadd(y, twiddled<DIR>(data[pos * T + t], tw, pos));
+            }
             ya[c] = y;
             yb[c] = y;
         } else {
-            double2 a = x0, b = make_double2(0.0, 0.0);
+            C a = x0, b = mk<C>(0, 0);
             int idx = 0;
             for (int r = 1; r <= H; ++r) {
-                const double2 xr = data[(base + r * L) * T + t];
-                const double2 xm = data[(base + (P - r) * L) * T + t];
+                const int pr = base + r * L, pm = base + (P - r) * L;
+                const C xr = twiddled<DIR>(data[pr * T + t], tw, pr);
+                const C xm = twiddled<DIR>(data[pm * T + t], tw, pm);
                 idx += s;
                 if (idx >= P) idx -= P;
-                const double2 w = twl[idx * stride];
-                const double c0 = w.x, s0 = -w.y;
+                const C w = twl[idx * stride];
+                const R c0 = w.x, s0 = -w.y;
                 a.x += (xr.x + xm.x) * c0;
                 a.y += (xr.y + xm.y) * c0;
                 b.x += (xr.x - xm.x) * s0;
                 b.y += (xr.y - xm.y) * s0;
             }
-            const double2 mib = make_double2(b.y, -b.x);
+            const C mib = mk<C>(b.y, -b.x);
             if (DIR < 0) {
                 ya[c] = cadd(a, mib);
                 yb[c] = csub(a, mib);
@@ -216,112 +224,137 @@ __device__ void pass_big(double2* data, const double2* twl, int n, int P, int L,
         const int bq = bf / L;
         const int q = bf - bq * L;
         const int base = bq * P * L + q;
-        {
-            const int pos = base + s * L;
-            double2 y = ya[c];
-            if (pn) y = cmul(y, next_twiddle<DIR>(twl, n, pos, Ln, pn));
-            data[pos * T + t] = y;
-        }
-        if (s > 0) {
-            const int pos = base + (P - s) * L;
-            double2 y = yb[c];
-            if (pn) y = cmul(y, next_twiddle<DIR>(twl, n, pos, Ln, pn));
-            data[pos * T + t] = y;
-        }
+        data[(base + s * L) * T + t] = ya[c];
+        if (s > 0) data[(base + (P - s) * L) * T + t] = yb[c];
     }
 }
 
-template <int T, int DIR>
-__device__ void run_passes(double2* data, const double2* twl, const FftDesc& d) {
+template <int T, int DIR, class C>
+__device__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d) {
     const int n = d.n;
     for (int ps = 0; ps < d.npass; ++ps) {
         __syncthreads();
         const int P = d.radix[ps], L = d.L[ps];
-        const int pn = ps + 1 < d.npass ? d.radix[ps + 1] : 0;
-        const int Ln = L * P;
+        const C* tw = ps ? twp + (ps - 1) * n : nullptr;
         switch (P) {
-            case 2: pass_reg<2, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 3: pass_reg<3, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 4: pass_reg<4, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 5: pass_reg<5, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 7: pass_reg<7, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 11: pass_reg<11, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 13: pass_reg<13, T, DIR>(data, twl, n, L, pn, Ln); break;
-            case 17: pass_reg<17, T, DIR>(data, twl, n, L, pn, Ln); break;
-            default: pass_big<T, DIR>(data, twl, n, P, L, pn, Ln); break;
+            case 2: pass_reg<2, T, DIR>(data, twl, tw, n, L); break;
+            case 3: pass_reg<3, T, DIR>(data, twl, tw, n, L); break;
+            case 4: pass_reg<4, T, DIR>(data, twl, tw, n, L); break;
+            case 5: pass_reg<5, T, DIR>(data, twl, tw, n, L); break;
+            case 7: pass_reg<7, T, DIR>(data, twl, tw, n, L); break;
+            case 11: pass_reg<11, T, DIR>(data, twl, tw, n, L); break;
+            case 13: pass_reg<13, T, DIR>(data, twl, tw, n, L); break;
+            case 17: pass_reg<17, T, DIR>(data, twl, tw, n, L); break;
+            default: pass_big<T, DIR>(data, twl, tw, n, P, L); break;
         }
     }
     __syncthreads();
 }
 
 struct PairMap {
-    int npairs, ncol, nth, jh, lat_stride, lon_stride;
+    int ncol, nth, jh, lat_stride, lon_stride;
+    int tiles_per_h;  // Zernike: tiles per latitude pair (ncol > 1)
 };
 
-template <int T, int LK>
+// (h, first column) of a tile: Zernike tiles run over the radial nodes of one
+// latitude pair; SH tiles (ncol = 1) run over T consecutive latitude pairs.
+template <int T>
+__device__ inline void tile_origin(const PairMap& pm, int tile, int& h0, int& c0) {
+    if (pm.ncol > 1) {
+        h0 = tile / pm.tiles_per_h;
+        c0 = (tile - h0 * pm.tiles_per_h) * T;
+    } else {
+        h0 = tile * T;
+        c0 = 0;
+    }
+}
+
+// Shared-memory layout: data [n*T], e^{-2 pi i t/n} [n], pass twiddles
+// [(npass-1)*n], then the DIT permutation [n] as unsigned shorts.
+template <int T, class C>
+__device__ inline void stage_tables(C*& data, C*& twl, C*& twp, unsigned short*& perm,
+                                    const FftDesc& d, const C* twg, const C* twpg,
+                                    const unsigned short* permg) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = d.n;
+    data = reinterpret_cast<C*>(smem_raw);
+    twl = data + n * T;
+    twp = twl + n;
+    perm = reinterpret_cast<unsigned short*>(twp + (d.npass > 0 ? d.npass - 1 : 0) * n);
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        twl[i] = twg[i];
+        perm[i] = permg[i];
+    }
+    for (int i = threadIdx.x; i < (d.npass - 1) * n; i += kThreads) twp[i] = twpg[i];
+    __syncthreads();
+}
+
+template <int T, int LK, class R>
 __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
-    FftDesc d, const double2* __restrict__ twg, const unsigned short* __restrict__ permg,
-    const double* __restrict__ grid, long grid_stride, double2* __restrict__ spec,
+    FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
+    const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
+    const double* __restrict__ grid, long grid_stride, typename Cplx<R>::type* __restrict__ spec,
     long spec_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
-    extern __shared__ double2 smem[];
+    using C = typename Cplx<R>::type;
+    C *data, *twl, *twp;
+    unsigned short* perm;
+    stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
-    double2* data = smem;
-    double2* twl = smem + n * T;
-    const int tile = blockIdx.x;
+    int h0, c0;
+    tile_origin<T>(pm, blockIdx.x, h0, c0);
     const double* g = grid + (long)blockIdx.y * grid_stride;
-    double2* sp = spec + (long)blockIdx.y * spec_stride;
-    for (int i = threadIdx.x; i < n; i += kThreads) twl[i] = twg[i];
+    C* sp = spec + (long)blockIdx.y * spec_stride;
 
     if (LK == GRID_COLS) {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
             const int i = e / T, t = e % T;
-            const int p = tile * T + t;
-            double a = 0.0, b = 0.0;
-            if (p < pm.npairs) {
-                const int h = p / pm.ncol, c = p - h * pm.ncol;
+            const int h = pm.ncol > 1 ? h0 : h0 + t;
+            const int c = pm.ncol > 1 ? c0 + t : 0;
+            R a = 0, b = 0;
+            if (h < pm.jh && c < pm.ncol) {
                 const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
                 const double* row = g + (long)i * pm.lon_stride + c;
-                a = row[(long)jn * pm.lat_stride];
-                if (js != jn) b = row[(long)js * pm.lat_stride];
+                a = (R)row[(long)jn * pm.lat_stride];
+                if (js != jn) b = (R)row[(long)js * pm.lat_stride];
             }
-            data[permg[i] * T + t] = make_double2(a, b);
+            data[perm[i] * T + t] = mk<C>(a, b);
         }
     } else {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
             const int t = e / n, i = e - t * n;
-            const int p = tile * T + t;
-            double a = 0.0, b = 0.0;
-            if (p < pm.npairs) {
-                const int jn = pm.nth / 2 + p, js = pm.jh - 1 - p;
-                a = g[(long)jn * pm.lat_stride + i];
-                if (js != jn) b = g[(long)js * pm.lat_stride + i];
+            const int h = h0 + t;
+            R a = 0, b = 0;
+            if (h < pm.jh) {
+                const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+                a = (R)g[(long)jn * pm.lat_stride + i];
+                if (js != jn) b = (R)g[(long)js * pm.lat_stride + i];
             }
-            data[permg[i] * T + t] = make_double2(a, b);
+            data[perm[i] * T + t] = mk<C>(a, b);
         }
     }
 
-    run_passes<T, -1>(data, twl, d);
+    run_passes<T, -1>(data, twl, twp, d);
 
     for (int e = threadIdx.x; e < nm * T; e += kThreads) {
         const int m = e / T, t = e % T;
-        const int p = tile * T + t;
-        if (p >= pm.npairs) continue;
-        const double2 zm = data[m * T + t];
-        const double2 zc = data[((n - m) % n) * T + t];
+        const int h = pm.ncol > 1 ? h0 : h0 + t;
+        const int c = pm.ncol > 1 ? c0 + t : 0;
+        if (h >= pm.jh || c >= pm.ncol) continue;
+        const C zm = data[m * T + t];
+        const C zc = data[(m ? n - m : 0) * T + t];
         // two-for-one split: A = (Z_m + conj Z_{n-m}) / 2, B = (Z_m - conj Z_{n-m}) / (2i)
-        const double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y - zc.y));
-        const double2 B = make_double2(0.5 * (zm.y + zc.y), -0.5 * (zm.x - zc.x));
-        const int h = p / pm.ncol, c = p - h * pm.ncol;
-        const double W = wlat[h] * (wrad ? wrad[c] : 1.0) * scale;
+        const C A = mk<C>(R(0.5) * (zm.x + zc.x), R(0.5) * (zm.y - zc.y));
+        const C B = mk<C>(R(0.5) * (zm.y + zc.y), R(-0.5) * (zm.x - zc.x));
+        const R W = (R)(wlat[h] * (wrad ? wrad[c] : 1.0) * scale);
         // H = weight * conj(scale * X)
-        const double2 Ha = make_double2(W * A.x, -W * A.y);
-        const double2 Hb = make_double2(W * B.x, -W * B.y);
-        double2 E, O;
-        const bool center = (pm.nth / 2 + h) == (pm.jh - 1 - h);
-        if (center) {
-            E = make_double2(center_factor * Ha.x, center_factor * Ha.y);
-            O = make_double2(0.0, 0.0);
+        const C Ha = mk<C>(W * A.x, -W * A.y);
+        const C Hb = mk<C>(W * B.x, -W * B.y);
+        C E, O;
+        if (pm.nth / 2 + h == pm.jh - 1 - h) {
+            const R cf = (R)center_factor;
+            E = mk<C>(cf * Ha.x, cf * Ha.y);
+            O = mk<C>(0, 0);
         } else {
             E = cadd(Ha, Hb);
             O = csub(Ha, Hb);
@@ -331,73 +364,74 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     }
 }
 
-template <int T, int LK>
+template <int T, int LK, class R>
 __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
-    FftDesc d, const double2* __restrict__ twg, const unsigned short* __restrict__ permg,
-    const double2* __restrict__ spec, long spec_stride, double* __restrict__ grid,
+    FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
+    const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
+    const typename Cplx<R>::type* __restrict__ spec, long spec_stride, double* __restrict__ grid,
     long grid_stride, PairMap pm, int nm) {
-    extern __shared__ double2 smem[];
+    using C = typename Cplx<R>::type;
+    C *data, *twl, *twp;
+    unsigned short* perm;
+    stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
-    double2* data = smem;
-    double2* twl = smem + n * T;
-    const int tile = blockIdx.x;
+    int h0, c0;
+    tile_origin<T>(pm, blockIdx.x, h0, c0);
     double* g = grid + (long)blockIdx.y * grid_stride;
-    const double2* sp = spec + (long)blockIdx.y * spec_stride;
-    for (int i = threadIdx.x; i < n; i += kThreads) twl[i] = twg[i];
+    const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
     for (int e = threadIdx.x; e < F * T; e += kThreads) {
         const int m = e / T, t = e % T;
-        const int p = tile * T + t;
-        double2 zm = make_double2(0.0, 0.0), zp = make_double2(0.0, 0.0);
-        if (p < pm.npairs && m < nm) {
-            const int h = p / pm.ncol, c = p - h * pm.ncol;
-            const double2 Ae = sp[((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c];
-            const double2 Ao = sp[((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c];
-            const double mu = m > 0 ? 0.5 : 1.0;
-            const bool center = (pm.nth / 2 + h) == (pm.jh - 1 - h);
-            double2 Sa, Sb;
-            if (center) {
-                Sa = make_double2(mu * Ae.x, -mu * Ae.y);
-                Sb = make_double2(0.0, 0.0);
+        const int h = pm.ncol > 1 ? h0 : h0 + t;
+        const int c = pm.ncol > 1 ? c0 + t : 0;
+        C zm = mk<C>(0, 0), zp = mk<C>(0, 0);
+        if (h < pm.jh && c < pm.ncol && m < nm) {
+            const C Ae = sp[((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c];
+            const C Ao = sp[((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c];
+            const R mu = m > 0 ? R(0.5) : R(1);
+            C Sa, Sb;
+            if (pm.nth / 2 + h == pm.jh - 1 - h) {
+                Sa = mk<C>(mu * Ae.x, -mu * Ae.y);
+                Sb = mk<C>(0, 0);
             } else {
-                const double2 Ev = cadd(Ae, Ao), Dv = csub(Ae, Ao);
-                Sa = make_double2(mu * Ev.x, -mu * Ev.y);
-                Sb = make_double2(mu * Dv.x, -mu * Dv.y);
+                const C Ev = cadd(Ae, Ao), Dv = csub(Ae, Ao);
+                Sa = mk<C>(mu * Ev.x, -mu * Ev.y);
+                Sb = mk<C>(mu * Dv.x, -mu * Dv.y);
             }
             if (m == 0 || 2 * m == n) {
-                zm = make_double2(Sa.x, Sb.x);  // imaginary parts of X_0 (and Nyquist) ignored
+                zm = mk<C>(Sa.x, Sb.x);  // imaginary parts of X_0 (and Nyquist) ignored
             } else {
-                zm = make_double2(Sa.x - Sb.y, Sa.y + Sb.x);   // Sa + i Sb
-                zp = make_double2(Sa.x + Sb.y, -Sa.y + Sb.x);  // conj(Sa) + i conj(Sb)
+                zm = mk<C>(Sa.x - Sb.y, Sa.y + Sb.x);   // Sa + i Sb
+                zp = mk<C>(Sa.x + Sb.y, -Sa.y + Sb.x);  // conj(Sa) + i conj(Sb)
             }
         }
-        data[permg[m] * T + t] = zm;
-        if (m > 0 && 2 * m != n) data[permg[n - m] * T + t] = zp;
+        data[perm[m] * T + t] = zm;
+        if (m > 0 && 2 * m != n) data[perm[n - m] * T + t] = zp;
     }
 
-    run_passes<T, +1>(data, twl, d);
+    run_passes<T, +1>(data, twl, twp, d);
 
     if (LK == GRID_COLS) {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
             const int i = e / T, t = e % T;
-            const int p = tile * T + t;
-            if (p >= pm.npairs) continue;
-            const double2 z = data[i * T + t];
-            const int h = p / pm.ncol, c = p - h * pm.ncol;
+            const int h = pm.ncol > 1 ? h0 : h0 + t;
+            const int c = pm.ncol > 1 ? c0 + t : 0;
+            if (h >= pm.jh || c >= pm.ncol) continue;
+            const C z = data[i * T + t];
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
             double* row = g + (long)i * pm.lon_stride + c;
-            row[(long)jn * pm.lat_stride] = z.x;
-            if (js != jn) row[(long)js * pm.lat_stride] = z.y;
+            row[(long)jn * pm.lat_stride] = (double)z.x;
+            if (js != jn) row[(long)js * pm.lat_stride] = (double)z.y;
         }
     } else {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
             const int t = e / n, i = e - t * n;
-            const int p = tile * T + t;
-            if (p >= pm.npairs) continue;
-            const double2 z = data[i * T + t];
-            const int jn = pm.nth / 2 + p, js = pm.jh - 1 - p;
-            g[(long)jn * pm.lat_stride + i] = z.x;
-            if (js != jn) g[(long)js * pm.lat_stride + i] = z.y;
+            const int h = h0 + t;
+            if (h >= pm.jh) continue;
+            const C z = data[i * T + t];
+            const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+            g[(long)jn * pm.lat_stride + i] = (double)z.x;
+            if (js != jn) g[(long)js * pm.lat_stride + i] = (double)z.y;
         }
     }
 }
@@ -416,57 +450,117 @@ int lds_limit() {
     return limit;
 }
 
-size_t lds_bytes(int n, int T) { return (size_t)(n * T + n) * sizeof(double2); }
-
-template <int T>
-void launch_forward(const FftLaunch& L, const PairMap& pm, const double* grid, long gs,
-                    double2* spec, long ss, int nb, hipStream_t s) {
-    const FftPlan& f = L.t->fft;
-    const int ntiles = (pm.npairs + T - 1) / T;
-    const size_t lds = lds_bytes(f.desc.n, T);
-    dim3 grd(ntiles, nb);
-    if (L.layout_kind == GRID_COLS) {
-        auto k = fft_forward_kernel<T, GRID_COLS>;
-        if (lds > 65536) check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "lds attribute");
-        k<<<grd, kThreads, lds, s>>>(f.desc, f.tw, f.perm, grid, gs, spec, ss, pm, L.nm_used,
-                                     L.scale, L.t->wlat, L.t->wrad, L.center_factor);
-    } else {
-        auto k = fft_forward_kernel<T, GRID_ROWS>;
-        if (lds > 65536) check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "lds attribute");
-        k<<<grd, kThreads, lds, s>>>(f.desc, f.tw, f.perm, grid, gs, spec, ss, pm, L.nm_used,
-                                     L.scale, L.t->wlat, L.t->wrad, L.center_factor);
-    }
-    check_hip(hipGetLastError(), "fft_forward launch");
+size_t lds_bytes(int n, int npass, int T, size_t csize) {
+    return (size_t)(n * T + n + (npass > 0 ? npass - 1 : 0) * n) * csize + 2 * (size_t)n;
 }
 
-template <int T>
-void launch_backward(const FftLaunch& L, const PairMap& pm, const double2* spec, long ss,
-                     double* grid, long gs, int nb, hipStream_t s) {
-    const FftPlan& f = L.t->fft;
-    const int ntiles = (pm.npairs + T - 1) / T;
-    const size_t lds = lds_bytes(f.desc.n, T);
-    dim3 grd(ntiles, nb);
-    if (L.layout_kind == GRID_COLS) {
-        auto k = fft_backward_kernel<T, GRID_COLS>;
-        if (lds > 65536) check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "lds attribute");
-        k<<<grd, kThreads, lds, s>>>(f.desc, f.tw, f.perm, spec, ss, grid, gs, pm, L.nm_used);
-    } else {
-        auto k = fft_backward_kernel<T, GRID_ROWS>;
-        if (lds > 65536) check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "lds attribute");
-        k<<<grd, kThreads, lds, s>>>(f.desc, f.tw, f.perm, spec, ss, grid, gs, pm, L.nm_used);
-    }
-    check_hip(hipGetLastError(), "fft_backward launch");
-}
-
-PairMap pair_map(const Geom& g) {
+PairMap pair_map(const Geom& g, int T) {
     PairMap pm;
     pm.ncol = g.ncol;
     pm.nth = g.nth;
     pm.jh = g.jh;
-    pm.npairs = g.jh * g.ncol;
     pm.lat_stride = g.lat_stride;
     pm.lon_stride = g.lon_stride;
+    pm.tiles_per_h = (g.ncol + T - 1) / T;
     return pm;
+}
+
+int num_tiles(const PairMap& pm, int T) {
+    return pm.ncol > 1 ? pm.jh * pm.tiles_per_h : (pm.jh + T - 1) / T;
+}
+
+template <class R> const typename Cplx<R>::type* tw_of(const FftPlan& f);
+template <> const double2* tw_of<double>(const FftPlan& f) { return f.tw; }
+template <> const float2* tw_of<float>(const FftPlan& f) { return f.tw32; }
+template <class R> const typename Cplx<R>::type* twp_of(const FftPlan& f);
+template <> const double2* twp_of<double>(const FftPlan& f) { return f.twp; }
+template <> const float2* twp_of<float>(const FftPlan& f) { return f.twp32; }
+
+template <int T, class R>
+void launch_forward(const FftLaunch& L, const double* grid, long gs, void* spec, long ss, int nb,
+                    hipStream_t s) {
+    using C = typename Cplx<R>::type;
+    const FftPlan& f = L.t->fft;
+    const PairMap pm = pair_map(*L.g, T);
+    const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
+    dim3 grd(num_tiles(pm, T), nb);
+    auto k = L.layout_kind == GRID_COLS ? fft_forward_kernel<T, GRID_COLS, R>
+                                        : fft_forward_kernel<T, GRID_ROWS, R>;
+    if (lds > 65536)
+        check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                  "lds attribute");
+    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, gs, (C*)spec, ss, pm,
+                                 L.nm_used, L.scale, L.t->wlat, L.t->wrad, L.center_factor);
+    check_hip(hipGetLastError(), "fft_forward launch");
+}
+
+template <int T, class R>
+void launch_backward(const FftLaunch& L, const void* spec, long ss, double* grid, long gs, int nb,
+                     hipStream_t s) {
+    using C = typename Cplx<R>::type;
+    const FftPlan& f = L.t->fft;
+    const PairMap pm = pair_map(*L.g, T);
+    const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
+    dim3 grd(num_tiles(pm, T), nb);
+    auto k = L.layout_kind == GRID_COLS ? fft_backward_kernel<T, GRID_COLS, R>
+                                        : fft_backward_kernel<T, GRID_ROWS, R>;
+    if (lds > 65536)
+        check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                  "lds attribute");
+    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid, gs,
+                                 pm, L.nm_used);
+    check_hip(hipGetLastError(), "fft_backward launch");
+}
+
+template <class R>
+void dispatch_forward(const FftLaunch& L, const double* grid, long gs, void* spec, long ss, int nb,
+                      hipStream_t s) {
+    switch (L.t->fft.tile[sizeof(R) == 4]) {
+        case 16: launch_forward<16, R>(L, grid, gs, spec, ss, nb, s); break;
+        case 8: launch_forward<8, R>(L, grid, gs, spec, ss, nb, s); break;
+        case 4: launch_forward<4, R>(L, grid, gs, spec, ss, nb, s); break;
+        case 2: launch_forward<2, R>(L, grid, gs, spec, ss, nb, s); break;
+        default: launch_forward<1, R>(L, grid, gs, spec, ss, nb, s); break;
+    }
+}
+
+template <class R>
+void dispatch_backward(const FftLaunch& L, const void* spec, long ss, double* grid, long gs, int nb,
+                       hipStream_t s) {
+    switch (L.t->fft.tile[sizeof(R) == 4]) {
+        case 16: launch_backward<16, R>(L, spec, ss, grid, gs, nb, s); break;
+        case 8: launch_backward<8, R>(L, spec, ss, grid, gs, nb, s); break;
+        case 4: launch_backward<4, R>(L, spec, ss, grid, gs, nb, s); break;
+        case 2: launch_backward<2, R>(L, spec, ss, grid, gs, nb, s); break;
+        default: launch_backward<1, R>(L, spec, ss, grid, gs, nb, s); break;
+    }
+}
+
+int choose_tile(int n, const std::vector<int>& rad, size_t csize) {
+    const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
+    const int npass = (int)rad.size();
+    int T = 16;
+    for (; T > 1; T /= 2) {
+        if (lds_bytes(n, npass, T, csize) > limit) continue;
+        bool ok = true;
+        for (int r : rad)
+            if (r > kMaxRegRadix) {
+                const long items = (long)(n / r) * T * ((r - 1) / 2 + 1);
+                if ((items + kThreads - 1) / kThreads > kMaxBigItems) ok = false;
+            }
+        if (ok) break;
+    }
+    if (lds_bytes(n, npass, T, csize) > limit)
+        throw Error{ZGL_ERROR_NOT_SUPPORTED, "longitude count too large for the LDS"};
+    return T;
+}
+
+template <class C>
+C* upload(const std::vector<C>& v, hipStream_t s, const char* what) {
+    C* d = nullptr;
+    check_hip(hipMalloc(&d, std::max<size_t>(v.size(), 1) * sizeof(C)), what);
+    if (!v.empty()) check_hip(hipMemcpyAsync(d, v.data(), v.size() * sizeof(C), hipMemcpyHostToDevice, s), what);
+    return d;
 }
 
 }  // namespace
@@ -494,23 +588,8 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
         p.desc.L[i] = L;
         L *= rad[i];
     }
-    // Tile width: the most column pairs whose LDS fits, and whose large-prime
-    // passes fit the per-thread register budget.
-    const int limit = std::min(lds_limit(), 160 * 1024);
-    int T = 16;
-    for (; T > 1; T /= 2) {
-        if (lds_bytes(n, T) > (size_t)limit) continue;
-        bool ok = true;
-        for (int r : rad)
-            if (r > kMaxRegRadix) {
-                const long items = (long)(n / r) * T * ((r - 1) / 2 + 1);
-                if ((items + kThreads - 1) / kThreads > kMaxBigItems) ok = false;
-            }
-        if (ok) break;
-    }
-    if (lds_bytes(n, T) > (size_t)limit)
-        throw Error{ZGL_ERROR_NOT_SUPPORTED, "longitude count too large for the LDS"};
-    p.tile = T;
+    p.tile[0] = choose_tile(n, rad, sizeof(double2));
+    p.tile[1] = choose_tile(n, rad, sizeof(float2));
 
     // Input slot of sample i for the DIT order: the last pass combines
     // sub-transforms of samples i = r (mod p_last), stored in block r.
@@ -525,46 +604,60 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
         }
         perm[i] = (unsigned short)pos;
     }
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    auto root = [&](long t) {  // e^{-2 pi i t / n}
+        const long double ang = two_pi * (long double)(t % n) / (long double)n;
+        return std::pair<double, double>((double)cosl(ang), (double)(-sinl(ang)));
+    };
     std::vector<double2> tw(n);
+    std::vector<float2> tw32(n);
     for (int t = 0; t < n; ++t) {
-        const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
-        tw[t] = make_double2((double)cosl(ang), (double)(-sinl(ang)));
+        auto [c, sn] = root(t);
+        tw[t] = make_double2(c, sn);
+        tw32[t] = make_float2((float)c, (float)sn);
     }
-    check_hip(hipMalloc(&p.tw, n * sizeof(double2)), "fft twiddles");
-    check_hip(hipMalloc(&p.perm, n * sizeof(unsigned short)), "fft permutation");
-    check_hip(hipMemcpyAsync(p.tw, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice, s), "upload twiddles");
-    check_hip(hipMemcpyAsync(p.perm, perm.data(), n * sizeof(unsigned short), hipMemcpyHostToDevice, s), "upload permutation");
+    // Pass ps >= 1 of radix P over sub-transforms of length Lp: slot
+    // pos = B P Lp + r Lp + q takes e^{-2 pi i r q / (P Lp)}.
+    std::vector<double2> twp;
+    std::vector<float2> twp32;
+    for (int ps = 1; ps < p.desc.npass; ++ps) {
+        const int P = rad[ps], Lp = p.desc.L[ps];
+        for (int pos = 0; pos < n; ++pos) {
+            const int q = pos % Lp, r = (pos / Lp) % P;
+            auto [c, sn] = root((long)r * q * (n / (P * Lp)));
+            twp.push_back(make_double2(c, sn));
+            twp32.push_back(make_float2((float)c, (float)sn));
+        }
+    }
+    p.tw = upload(tw, s, "fft twiddles");
+    p.tw32 = upload(tw32, s, "fft twiddles");
+    p.twp = upload(twp, s, "fft pass twiddles");
+    p.twp32 = upload(twp32, s, "fft pass twiddles");
+    p.perm = upload(perm, s, "fft permutation");
     check_hip(hipStreamSynchronize(s), "fft plan");
 }
 
 void fft_plan_destroy(FftPlan& p) {
-    if (p.tw) (void)hipFree(p.tw);
-    if (p.perm) (void)hipFree(p.perm);
+    for (void* q : {(void*)p.tw, (void*)p.tw32, (void*)p.twp, (void*)p.twp32, (void*)p.perm})
+        if (q) (void)hipFree(q);
     p = FftPlan{};
 }
 
-void fft_forward(const FftLaunch& L, const double* grid, long grid_stride, double2* spec,
+template <class R>
+void fft_forward(const FftLaunch& L, const double* grid, long grid_stride, void* spec,
                  long spec_stride, int nb, hipStream_t s) {
-    const PairMap pm = pair_map(*L.g);
-    switch (L.t->fft.tile) {
-        case 16: launch_forward<16>(L, pm, grid, grid_stride, spec, spec_stride, nb, s); break;
-        case 8: launch_forward<8>(L, pm, grid, grid_stride, spec, spec_stride, nb, s); break;
-        case 4: launch_forward<4>(L, pm, grid, grid_stride, spec, spec_stride, nb, s); break;
-        case 2: launch_forward<2>(L, pm, grid, grid_stride, spec, spec_stride, nb, s); break;
-        default: launch_forward<1>(L, pm, grid, grid_stride, spec, spec_stride, nb, s); break;
-    }
+    dispatch_forward<R>(L, grid, grid_stride, spec, spec_stride, nb, s);
 }
 
-void fft_backward(const FftLaunch& L, const double2* spec, long spec_stride, double* grid,
+template <class R>
+void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, double* grid,
                   long grid_stride, int nb, hipStream_t s) {
-    const PairMap pm = pair_map(*L.g);
-    switch (L.t->fft.tile) {
-        case 16: launch_backward<16>(L, pm, spec, spec_stride, grid, grid_stride, nb, s); break;
-        case 8: launch_backward<8>(L, pm, spec, spec_stride, grid, grid_stride, nb, s); break;
-        case 4: launch_backward<4>(L, pm, spec, spec_stride, grid, grid_stride, nb, s); break;
-        case 2: launch_backward<2>(L, pm, spec, spec_stride, grid, grid_stride, nb, s); break;
-        default: launch_backward<1>(L, pm, spec, spec_stride, grid, grid_stride, nb, s); break;
-    }
+    dispatch_backward<R>(L, spec, spec_stride, grid, grid_stride, nb, s);
 }
+
+template void fft_forward<double>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
+template void fft_forward<float>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
+template void fft_backward<double>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
+template void fft_backward<float>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
 
 }  // namespace zgl

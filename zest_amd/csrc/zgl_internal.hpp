@@ -32,9 +32,12 @@ struct FftDesc {
 
 struct FftPlan {
     FftDesc desc{};
-    double2* tw = nullptr;          // e^{-2 pi i t / n}, t < n
-    unsigned short* perm = nullptr; // LDS slot of input sample i
-    int tile = 16;                  // transforms (column pairs) per workgroup
+    double2* tw = nullptr;           // e^{-2 pi i t / n}, t < n
+    float2* tw32 = nullptr;
+    double2* twp = nullptr;          // input twiddle of every slot, passes 1..npass-1
+    float2* twp32 = nullptr;
+    unsigned short* perm = nullptr;  // LDS slot of input sample i
+    int tile[2] = {16, 16};          // column pairs per workgroup: FP64, FP32
 };
 
 // Geometry of one order for a plan, both transform kinds.
@@ -56,12 +59,20 @@ struct OrderTables {
     double* wrad = nullptr;  // [ncol] r^2 v of the radial nodes (Zernike)
     double* ptab = nullptr;  // P_lm(z_h), parity-split [m][par][l'][h]
     double* rtab = nullptr;  // R_nl(r_k), zest layout [((n+1)^2>>2)+(l>>1)][nr]
+    float* ptab32 = nullptr; // FP32 roundings of the two tables (FP32 modes)
+    float* rtab32 = nullptr;
     FftPlan fft;
     int4* work_lf = nullptr; int n_lf = 0;  // (problem, m0) tiles, forward Legendre
     int4* work_lb = nullptr; int n_lb = 0;  // backward Legendre
     int4* work_rf = nullptr; int n_rf = 0;  // forward radial
     int4* work_rb = nullptr; int n_rb = 0;  // backward radial
 };
+
+// Rows per GEMM tile are 16 * NS; the work lists step by that many rows.
+constexpr int kNsLegendreFwd = 4;
+constexpr int kNsLegendreBwdZt = 5;  // 80 rows: one tile for 65 half latitudes
+constexpr int kNsLegendreBwdSt = 4;
+constexpr int kNsRadial = 4;
 
 // Offsets of the parity-split P table: rows l = m + par + 2 l' < order.
 __host__ __device__ inline int plm_count(int order, int m, int par) {
@@ -81,7 +92,8 @@ __host__ __device__ inline long tet_idx(long n, long l, long m) {
 
 // ---- table generation (zgl_tables.hip) ----
 void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm,
-                  int sh_phase, int layout, int order, hipStream_t s);
+                  int sh_phase, int layout, int order, bool want_fp32, hipStream_t s);
+void ensure_fp32_tables(OrderTables& t, hipStream_t s);
 void free_tables(OrderTables& t);
 
 // ---- FFT (zgl_fft.hip) ----
@@ -102,9 +114,13 @@ struct FftLaunch {
     double center_factor;   // weight factor of an equator node (zest's SH quirk: 2)
 };
 
+// R = double or float: the arithmetic; spec holds complex R values and its
+// stride counts complex elements.
+template <class R>
 void fft_forward(const FftLaunch& L, const double* grid, long grid_stride,
-                 double2* spec, long spec_stride, int nb, hipStream_t s);
-void fft_backward(const FftLaunch& L, const double2* spec, long spec_stride,
+                 void* spec, long spec_stride, int nb, hipStream_t s);
+template <class R>
+void fft_backward(const FftLaunch& L, const void* spec, long spec_stride,
                   double* grid, long grid_stride, int nb, hipStream_t s);
 
 // ---- contractions (zgl_gemm.hip) ----
@@ -112,24 +128,30 @@ struct ZtGemmArgs {
     const OrderTables* t;
     int nt;                 // truncated order min(exp_order, grid_order)
     int zernike_unnormed;   // scale coefficients by 2n+3
-    double2* spec; long spec_stride;
-    double2* flm; long flm_stride;
+    void* spec; long spec_stride;   // complex elements of the compute type
+    void* flm; long flm_stride;
     double* coeffs; long coeff_stride;  // doubles
     int nb;
 };
-void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
-void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
-void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);
-void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s);
+template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
+template <class T> void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
+template <class T> void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);
+template <class T> void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s);
 
 struct StGemmArgs {
     const OrderTables* t;
     int nt;
-    double2* spec; long spec_stride;
+    void* spec; long spec_stride;
     double* coeffs; long coeff_stride;  // doubles
     int nb;
 };
-void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
-void st_legendre_backward(const StGemmArgs& a, hipStream_t s);
+template <class T> void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
+template <class T> void st_legendre_backward(const StGemmArgs& a, hipStream_t s);
+
+// ---- elementwise helpers (zgl_tables.hip) ----
+// r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in doubles.
+void grid_residual(const double* g, long gs, double* r, long rs, long n, int nb, hipStream_t s);
+// c[b][i] += d[b][i] for i < n.
+void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s);
 
 }  // namespace zgl

@@ -38,11 +38,17 @@ __global__ void gl_kernel(int n, double* nodes, double* weights) {
     if (k >= (n + 1) / 2) return;
     double x = cos(M_PI * ((double)k + 0.75) / ((double)n + 0.5));
     double p, dp;
+    // Quadratic convergence: stop one step after the update falls below
+    // 1e-15 (the last step moves the root by at most an ulp or two).
     for (int it = 0; it < 100; ++it) {
         legendre_p_dp(n, x, p, dp);
         const double dx = p / dp;
         x -= dx;
-        if (fabs(dx) <= 1e-17 * (fabs(x) + 1e-300)) break;
+        if (fabs(dx) <= 1e-15) {
+            legendre_p_dp(n, x, p, dp);
+            x -= p / dp;
+            break;
+        }
     }
     legendre_p_dp(n, x, p, dp);
     const double w = 2.0 / ((1.0 - x * x) * dp * dp);
@@ -179,13 +185,32 @@ __global__ void radial_kernel(int order, int nr, int normed, const double* rk, d
         }
 }
 
+__global__ void to_fp32_kernel(const double* in, float* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+__global__ void residual_kernel(const double* g, long gs, double* r, long rs, long n) {
+    const double* gb = g + blockIdx.y * gs;
+    double* rb = r + blockIdx.y * rs;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        rb[i] = gb[i] - rb[i];
+}
+
+__global__ void accumulate_kernel(double* c, long cs, const double* d, long ds, long n) {
+    double* cb = c + blockIdx.y * cs;
+    const double* db = d + blockIdx.y * ds;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        cb[i] += db[i];
+}
+
+unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 4096); }
+
 template <class T>
 T* dev_alloc(size_t count) {
     void* p = nullptr;
     if (count == 0) count = 1;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess)
-        throw Error{ZGL_ERROR_OUT_OF_MEMORY, "hipMalloc of a table failed"};
+    check_hip(hipMalloc(&p, count * sizeof(T)), "table allocation");
     return static_cast<T*>(p);
 }
 
@@ -199,8 +224,39 @@ int4* upload_work(const std::vector<int4>& w, hipStream_t s) {
 
 }  // namespace
 
+void grid_residual(const double* g, long gs, double* r, long rs, long n, int nb, hipStream_t s) {
+    if (n <= 0 || nb <= 0) return;
+    residual_kernel<<<dim3(grid_for(n), nb), 256, 0, s>>>(g, gs, r, rs, n);
+    check_hip(hipGetLastError(), "residual launch");
+}
+
+void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s) {
+    if (n <= 0 || nb <= 0) return;
+    accumulate_kernel<<<dim3(grid_for(n), nb), 256, 0, s>>>(c, cs, d, ds, n);
+    check_hip(hipGetLastError(), "accumulate launch");
+}
+
+static long ptab_size(const Geom& g) {
+    const int order = g.order;
+    return plm_offset(order, order - 1, 1, g.jh) + (long)plm_count(order, order - 1, 1) * g.jh;
+}
+
+void ensure_fp32_tables(OrderTables& t, hipStream_t s) {
+    if (t.g.order == 0 || t.ptab32) return;
+    const long np = ptab_size(t.g);
+    t.ptab32 = dev_alloc<float>(np);
+    to_fp32_kernel<<<grid_for(np), 256, 0, s>>>(t.ptab, t.ptab32, np);
+    if (t.rtab) {
+        const long nr = (long)((t.g.order + 1) * (t.g.order + 1) / 4) * t.g.ncol;
+        t.rtab32 = dev_alloc<float>(nr);
+        to_fp32_kernel<<<grid_for(nr), 256, 0, s>>>(t.rtab, t.rtab32, nr);
+    }
+    check_hip(hipGetLastError(), "fp32 tables");
+    check_hip(hipStreamSynchronize(s), "fp32 tables");
+}
+
 void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, int sh_phase,
-                  int layout, int order, hipStream_t s) {
+                  int layout, int order, bool want_fp32, hipStream_t s) {
     Geom& g = t.g;
     g.order = order;
     const int lon_min = 2 * order - (order > 0 ? 1 : 0);
@@ -236,8 +292,7 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
     half_nodes_kernel<<<g.jh / 64 + 1, 64, 0, s>>>(g.nth, z, w, zh, t.wlat);
 
     // P table.
-    const long ptab_size = plm_offset(order, order - 1, 1, g.jh) + (long)plm_count(order, order - 1, 1) * g.jh;
-    t.ptab = dev_alloc<double>(ptab_size);
+    t.ptab = dev_alloc<double>(ptab_size(g));
     {
         const long threads = (long)order * g.jh;
         plm_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(
@@ -268,20 +323,23 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
 
     fft_plan_create(t.fft, g.nphi, s);
 
-    // Work lists: (problem, first row of the 32-row tile).
+    // Work lists: (problem, first row of its tile); tiles have 16 * NS rows.
     std::vector<int4> lf, lb, rf, rb;
+    const int tm_lf = 16 * kNsLegendreFwd;
+    const int tm_lb = 16 * (kind == ZGL_ZERNIKE ? kNsLegendreBwdZt : kNsLegendreBwdSt);
+    const int tm_r = 16 * kNsRadial;
     for (int m = 0; m < order; ++m)
         for (int par = 0; par < 2; ++par) {
             const int prob = 2 * m + par;
             const int cnt = plm_count(order, m, par);
-            for (int m0 = 0; m0 < cnt; m0 += 32) lf.push_back(make_int4(prob, m0, 0, 0));
-            for (int m0 = 0; m0 < g.jh; m0 += 32) lb.push_back(make_int4(prob, m0, 0, 0));
+            for (int m0 = 0; m0 < cnt; m0 += tm_lf) lf.push_back(make_int4(prob, m0, 0, 0));
+            for (int m0 = 0; m0 < g.jh; m0 += tm_lb) lb.push_back(make_int4(prob, m0, 0, 0));
         }
     if (kind == ZGL_ZERNIKE) {
         for (int l = 0; l < order; ++l) {
             const int ncount = (order - l + 1) / 2;
-            for (int m0 = 0; m0 < ncount; m0 += 32) rf.push_back(make_int4(l, m0, 0, 0));
-            for (int m0 = 0; m0 < 2 * (l + 1); m0 += 32) rb.push_back(make_int4(l, m0, 0, 0));
+            for (int m0 = 0; m0 < ncount; m0 += tm_r) rf.push_back(make_int4(l, m0, 0, 0));
+            for (int m0 = 0; m0 < 2 * (l + 1); m0 += tm_r) rb.push_back(make_int4(l, m0, 0, 0));
         }
     }
     t.work_lf = upload_work(lf, s); t.n_lf = (int)lf.size();
@@ -289,11 +347,13 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
     t.work_rf = upload_work(rf, s); t.n_rf = (int)rf.size();
     t.work_rb = upload_work(rb, s); t.n_rb = (int)rb.size();
     check_hip(hipStreamSynchronize(s), "work lists");
+    if (want_fp32) ensure_fp32_tables(t, s);
 }
 
 void free_tables(OrderTables& t) {
-    for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.rtab, (void*)t.work_lf,
-                    (void*)t.work_lb, (void*)t.work_rf, (void*)t.work_rb})
+    for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.rtab, (void*)t.ptab32,
+                    (void*)t.rtab32, (void*)t.work_lf, (void*)t.work_lb, (void*)t.work_rf,
+                    (void*)t.work_rb})
         if (p) (void)hipFree(p);
     fft_plan_destroy(t.fft);
     t = OrderTables{};
