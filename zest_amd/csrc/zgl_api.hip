@@ -133,8 +133,9 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
     const size_t per = (size_t)(c.spec_stride + c.flm_stride) * csize;
     size_t budget, cap;
     if (p->kind == ZGL_ZERNIKE) {
-        // Keep a chunk's intermediates within the 256 MB Infinity Cache.
-        budget = env_mb("ZGL_SCRATCH_MB", 192) << 20;
+        // Large chunks keep every CU busy through each stage; measured faster
+        // than chunks sized to stay in the 256 MB Infinity Cache.
+        budget = env_mb("ZGL_SCRATCH_MB", 1024) << 20;
         cap = 64;
     } else {
         // Instances form the GEMM column axis: larger chunks reuse the P table more.

@@ -39,6 +39,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxBigItems = 9;
 constexpr int kMaxRegRadix = 17;  // larger primes use the LDS-staged pass
+constexpr int kBatch = 8;         // global loads in flight per thread
 
 template <class R> struct Cplx;
 template <> struct Cplx<double> { using type = double2; };
@@ -306,31 +307,56 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     const double* g = grid + (long)blockIdx.y * grid_stride;
     C* sp = spec + (long)blockIdx.y * spec_stride;
 
+    // Loads are issued in batches of kBatch per thread before any LDS store,
+    // so each thread keeps that many independent global reads in flight.
     if (LK == GRID_COLS) {
-        for (int e = threadIdx.x; e < n * T; e += kThreads) {
-            const int i = e / T, t = e % T;
-            const int h = pm.ncol > 1 ? h0 : h0 + t;
-            const int c = pm.ncol > 1 ? c0 + t : 0;
-            R a = 0, b = 0;
-            if (h < pm.jh && c < pm.ncol) {
-                const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
-                const double* row = g + (long)i * pm.lon_stride + c;
-                a = (R)row[(long)jn * pm.lat_stride];
-                if (js != jn) b = (R)row[(long)js * pm.lat_stride];
+        for (int e0 = threadIdx.x; e0 < n * T; e0 += kThreads * kBatch) {
+            R av[kBatch], bv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * kThreads;
+                const int i = e / T, t = e % T;
+                const int h = pm.ncol > 1 ? h0 : h0 + t;
+                const int c = pm.ncol > 1 ? c0 + t : 0;
+                av[u] = 0;
+                bv[u] = 0;
+                if (e < n * T && h < pm.jh && c < pm.ncol) {
+                    const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+                    const double* row = g + (long)i * pm.lon_stride + c;
+                    av[u] = (R)row[(long)jn * pm.lat_stride];
+                    if (js != jn) bv[u] = (R)row[(long)js * pm.lat_stride];
+                }
             }
-            data[perm[i] * T + t] = mk<C>(a, b);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * kThreads;
+                if (e < n * T) data[perm[e / T] * T + e % T] = mk<C>(av[u], bv[u]);
+            }
         }
     } else {
-        for (int e = threadIdx.x; e < n * T; e += kThreads) {
-            const int t = e / n, i = e - t * n;
-            const int h = h0 + t;
-            R a = 0, b = 0;
-            if (h < pm.jh) {
-                const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
-                a = (R)g[(long)jn * pm.lat_stride + i];
-                if (js != jn) b = (R)g[(long)js * pm.lat_stride + i];
+        for (int e0 = threadIdx.x; e0 < n * T; e0 += kThreads * kBatch) {
+            R av[kBatch], bv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * kThreads;
+                const int t = e / n, i = e - t * n;
+                const int h = h0 + t;
+                av[u] = 0;
+                bv[u] = 0;
+                if (e < n * T && h < pm.jh) {
+                    const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+                    av[u] = (R)g[(long)jn * pm.lat_stride + i];
+                    if (js != jn) bv[u] = (R)g[(long)js * pm.lat_stride + i];
+                }
             }
-            data[perm[i] * T + t] = mk<C>(a, b);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * kThreads;
+                if (e < n * T) {
+                    const int t = e / n, i = e - t * n;
+                    data[perm[i] * T + t] = mk<C>(av[u], bv[u]);
+                }
+            }
         }
     }
 
@@ -380,33 +406,47 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     double* g = grid + (long)blockIdx.y * grid_stride;
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
-    for (int e = threadIdx.x; e < F * T; e += kThreads) {
-        const int m = e / T, t = e % T;
-        const int h = pm.ncol > 1 ? h0 : h0 + t;
-        const int c = pm.ncol > 1 ? c0 + t : 0;
-        C zm = mk<C>(0, 0), zp = mk<C>(0, 0);
-        if (h < pm.jh && c < pm.ncol && m < nm) {
-            const C Ae = sp[((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c];
-            const C Ao = sp[((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c];
+    for (int e0 = threadIdx.x; e0 < F * T; e0 += kThreads * kBatch) {
+        C ae[kBatch], ao[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = e0 + u * kThreads;
+            const int m = e / T, t = e % T;
+            const int h = pm.ncol > 1 ? h0 : h0 + t;
+            const int c = pm.ncol > 1 ? c0 + t : 0;
+            ae[u] = mk<C>(0, 0);
+            ao[u] = mk<C>(0, 0);
+            if (e < F * T && h < pm.jh && c < pm.ncol && m < nm) {
+                ae[u] = sp[((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c];
+                ao[u] = sp[((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = e0 + u * kThreads;
+            if (e >= F * T) continue;
+            const int m = e / T, t = e % T;
+            const int h = pm.ncol > 1 ? h0 : h0 + t;
             const R mu = m > 0 ? R(0.5) : R(1);
             C Sa, Sb;
             if (pm.nth / 2 + h == pm.jh - 1 - h) {
-                Sa = mk<C>(mu * Ae.x, -mu * Ae.y);
+                Sa = mk<C>(mu * ae[u].x, -mu * ae[u].y);
                 Sb = mk<C>(0, 0);
             } else {
-                const C Ev = cadd(Ae, Ao), Dv = csub(Ae, Ao);
+                const C Ev = cadd(ae[u], ao[u]), Dv = csub(ae[u], ao[u]);
                 Sa = mk<C>(mu * Ev.x, -mu * Ev.y);
                 Sb = mk<C>(mu * Dv.x, -mu * Dv.y);
             }
+            C zm, zp = mk<C>(0, 0);
             if (m == 0 || 2 * m == n) {
                 zm = mk<C>(Sa.x, Sb.x);  // imaginary parts of X_0 (and Nyquist) ignored
             } else {
                 zm = mk<C>(Sa.x - Sb.y, Sa.y + Sb.x);   // Sa + i Sb
                 zp = mk<C>(Sa.x + Sb.y, -Sa.y + Sb.x);  // conj(Sa) + i conj(Sb)
             }
+            data[perm[m] * T + t] = zm;
+            if (m > 0 && 2 * m != n) data[perm[n - m] * T + t] = zp;
         }
-        data[perm[m] * T + t] = zm;
-        if (m > 0 && 2 * m != n) data[perm[n - m] * T + t] = zp;
     }
 
     run_passes<T, +1>(data, twl, twp, d);

@@ -15,10 +15,14 @@
 // the radial stages. A workgroup of four waves owns a tile of up to 16*NS rows
 // and 64 columns of one problem. The tile is cut into 16x16 MFMA blocks
 // ((strip, column block) pairs) dealt round-robin to the waves, so a short
-// problem pays for whole 16-row strips only, and the contraction axis is
-// staged through LDS in chunks of 32 whose last chunk runs only the 4-deep
-// MFMA steps it needs. Work lists of (problem, first row) are built per
-// order on the host; grid.y walks the column tiles and grid.z the instances.
+// problem pays for whole 16-row strips only.
+//
+// Operands are described by row base pointers: every operand is a set of
+// contiguous rows, either one row per output index (contiguous along the
+// contraction axis, "kind 0") or one row per contraction index (contiguous
+// along the output axis, "kind 1"). Each thread loads fixed row segments, so
+// the loaders need no division, and the next chunk of the contraction axis is
+// fetched into registers while the current chunk runs on the MFMA units.
 #include <hip/hip_runtime.h>
 
 #include "zgl_internal.hpp"
@@ -48,12 +52,15 @@ template <> struct Mfma<float> {
     __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int TN = 64, TK = 32;
+constexpr int TN = 64;     // columns per tile
+constexpr int TK = 32;     // contraction chunk
+constexpr int kThreads = 256;
 
-enum LoadOrder { K_FAST = 0, MN_FAST = 1, PAIR_K_FAST = 2 };
+// Operand kinds (see the header comment).
+enum { ROWS_OUT = 0, ROWS_K = 1 };
 
 template <class S, int NS>
-__global__ __launch_bounds__(256) void gemm_kernel(S st, const int4* __restrict__ work) {
+__global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __restrict__ work) {
     using T = typename S::T;
     using M = Mfma<T>;
     constexpr int TM = 16 * NS;
@@ -69,28 +76,110 @@ __global__ __launch_bounds__(256) void gemm_kernel(S st, const int4* __restrict_
     const int rows = min(TM, P.M - m0), cols = min(TN, P.N - n0);
     const int strips = (rows + 15) >> 4, blocks = (cols + 15) >> 4;
     const int npairs = strips * blocks;
-    const int arows = strips * 16, bcols = blocks * 16;
+
+    // A tile TM x TK. ROWS_OUT: thread -> (i = tid / 32 + 8q, k = tid % 32);
+    // ROWS_K: thread -> (k = tid / 32 + 8 (q % 4), i = tid % 32 + 32 (q / 4)).
+    // Consecutive threads always read consecutive elements of one row.
+    constexpr int A_NQ = (S::A_KIND == ROWS_OUT) ? (TM + 7) / 8 : (TK / 8) * ((TM + 31) / 32);
+    constexpr int B_NQ = (S::B_KIND == ROWS_OUT) ? (TN / 8) : (TK / 4);
+    T ra[A_NQ], rb[B_NQ];
+
+    auto load_a = [&](int k0) {
+        if constexpr (S::A_KIND == ROWS_OUT) {
+            const int k = tid & (TK - 1);
+#pragma unroll
+            for (int q = 0; q < A_NQ; ++q) {
+                const int i = (tid >> 5) + 8 * q;
+                ra[q] = (i < rows && k0 + k < P.K) ? (T)st.a_row(P, m0 + i)[k0 + k] : T(0);
+            }
+        } else {
+            const int il = tid & 31;
+#pragma unroll
+            for (int q = 0; q < A_NQ; ++q) {
+                const int k = (tid >> 5) + 8 * (q % (TK / 8));
+                const int i = il + 32 * (q / (TK / 8));
+                ra[q] = (i < rows && k0 + k < P.K) ? (T)st.a_row(P, k0 + k)[m0 + i] : T(0);
+            }
+        }
+    };
+    auto store_a = [&]() {
+        if constexpr (S::A_KIND == ROWS_OUT) {
+            const int k = tid & (TK - 1);
+#pragma unroll
+            for (int q = 0; q < A_NQ; ++q) {
+                const int i = (tid >> 5) + 8 * q;
+                if (i < TM) As[i][k] = ra[q];
+            }
+        } else {
+            const int il = tid & 31;
+#pragma unroll
+            for (int q = 0; q < A_NQ; ++q) {
+                const int k = (tid >> 5) + 8 * (q % (TK / 8));
+                const int i = il + 32 * (q / (TK / 8));
+                if (i < TM) As[i][k] = ra[q];
+            }
+        }
+    };
+    // B: ROWS_K -> thread (k = tid / 64 + 4q, j = tid % 64); ROWS_OUT -> rows j
+    // hold the contraction axis: thread (j = tid / 32 + 8q, k = tid % 32); with
+    // B_PAIR the rows are indexed by j / 2 and hold (k, re/im) pairs.
+    auto load_b = [&](int k0) {
+        if constexpr (S::B_KIND == ROWS_K) {
+            const int j = tid & 63;
+#pragma unroll
+            for (int q = 0; q < B_NQ; ++q) {
+                const int k = (tid >> 6) + 4 * q;
+                rb[q] = (k0 + k < P.K && j < cols) ? (T)st.b_row(P, k0 + k)[n0 + j] : T(0);
+            }
+        } else if constexpr (!S::B_PAIR) {
+            const int k = tid & (TK - 1);
+#pragma unroll
+            for (int q = 0; q < B_NQ; ++q) {
+                const int j = (tid >> 5) + 8 * q;
+                rb[q] = (k0 + k < P.K && j < cols) ? (T)st.b_row(P, n0 + j)[k0 + k] : T(0);
+            }
+        } else {
+            // pairs: thread -> (jj = tid / 64 + 4q pair row, e = tid % 64 over (k, ri))
+            const int e = tid & 63, k = e >> 1, ri = e & 1;
+#pragma unroll
+            for (int q = 0; q < B_NQ; ++q) {
+                const int jj = (tid >> 6) + 4 * q;
+                const int j = 2 * jj + ri;
+                rb[q] = (k0 + k < P.K && j < cols) ? (T)st.b_row(P, (n0 >> 1) + jj)[2 * (k0 + k) + ri] : T(0);
+            }
+        }
+    };
+    auto store_b = [&]() {
+        if constexpr (S::B_KIND == ROWS_K) {
+            const int j = tid & 63;
+#pragma unroll
+            for (int q = 0; q < B_NQ; ++q) Bs[(tid >> 6) + 4 * q][j] = rb[q];
+        } else if constexpr (!S::B_PAIR) {
+            const int k = tid & (TK - 1);
+#pragma unroll
+            for (int q = 0; q < B_NQ; ++q) Bs[k][(tid >> 5) + 8 * q] = rb[q];
+        } else {
+            const int e = tid & 63, k = e >> 1, ri = e & 1;
+#pragma unroll
+            for (int q = 0; q < B_NQ; ++q) Bs[k][2 * ((tid >> 6) + 4 * q) + ri] = rb[q];
+        }
+    };
+
     typename M::acc_t acc[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc[q] = typename M::acc_t{0, 0, 0, 0};
 
+    load_a(0);
+    load_b(0);
     for (int k0 = 0; k0 < P.K; k0 += TK) {
-        const int kc = min(TK, P.K - k0);
-        const int kpad = (kc + 3) & ~3;
-        for (int e = tid; e < arows * kpad; e += 256) {
-            int i, k;
-            if (S::A_ORDER == K_FAST) { k = e % kpad; i = e / kpad; }
-            else { i = e % arows; k = e / arows; }
-            As[i][k] = (i < rows && k < kc) ? st.a(P, m0 + i, k0 + k) : T(0);
-        }
-        for (int e = tid; e < kpad * bcols; e += 256) {
-            int k, j;
-            if (S::B_ORDER == MN_FAST) { j = e % bcols; k = e / bcols; }
-            else if (S::B_ORDER == K_FAST) { k = e % kpad; j = e / kpad; }
-            else { const int ri = e & 1; k = (e >> 1) % kpad; j = 2 * ((e >> 1) / kpad) + ri; }
-            Bs[k][j] = (k < kc && j < cols) ? st.b(P, k0 + k, n0 + j) : T(0);
-        }
+        store_a();
+        store_b();
         __syncthreads();
+        if (k0 + TK < P.K) {  // prefetch the next chunk into registers
+            load_a(k0 + TK);
+            load_b(k0 + TK);
+        }
+        const int kpad = (min(TK, P.K - k0) + 3) & ~3;
         for (int kk = 0; kk < kpad; kk += 4) {
             const int kr = kk + (lane >> 4);
 #pragma unroll
@@ -98,9 +187,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(S st, const int4* __restrict_
                 const int p = wave + 4 * q;
                 if (p < npairs) {
                     const int s = p / blocks, b = p - s * blocks;
-                    const T av = As[s * 16 + (lane & 15)][kr];
-                    const T bv = Bs[kr][b * 16 + (lane & 15)];
-                    acc[q] = M::mma(av, bv, acc[q]);
+                    acc[q] = M::mma(As[s * 16 + (lane & 15)][kr], Bs[kr][b * 16 + (lane & 15)], acc[q]);
                 }
             }
         }
@@ -127,7 +214,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(S st, const int4* __restrict_
 template <class TT>
 struct ZtLegendreFwd {
     using T = TT;
-    static constexpr int A_ORDER = K_FAST, B_ORDER = MN_FAST;
+    static constexpr int A_KIND = ROWS_OUT, B_KIND = ROWS_K;
+    static constexpr bool B_PAIR = false;
     const T* ptab; const T* spec; T* flm;
     long spec_stride, flm_stride;  // elements of T
     int order, nt, jh, ncol;
@@ -142,8 +230,8 @@ struct ZtLegendreFwd {
         p.C = flm + inst * flm_stride;
         return p;
     }
-    __device__ T a(const Prob& p, int i, int k) const { return p.A[(long)i * jh + k]; }
-    __device__ T b(const Prob& p, int k, int j) const { return p.B[(long)k * 2 * ncol + j]; }
+    __device__ const T* a_row(const Prob& p, int i) const { return p.A + (long)i * jh; }
+    __device__ const T* b_row(const Prob& p, int k) const { return p.B + (long)k * 2 * ncol; }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         const int l = p.m + p.par + 2 * i;
         p.C[tri_idx(l, p.m) * 2 * ncol + j] = v;
@@ -154,7 +242,8 @@ struct ZtLegendreFwd {
 template <class TT>
 struct ZtLegendreBwd {
     using T = TT;
-    static constexpr int A_ORDER = MN_FAST, B_ORDER = MN_FAST;
+    static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_K;
+    static constexpr bool B_PAIR = false;
     const T* ptab; const T* flm; T* spec;
     long spec_stride, flm_stride;
     int order, nt, jh, ncol;
@@ -169,10 +258,9 @@ struct ZtLegendreBwd {
         p.C = spec + inst * spec_stride + (long)(p.m * 2 + p.par) * jh * 2 * ncol;
         return p;
     }
-    __device__ T a(const Prob& p, int i, int k) const { return p.A[(long)k * jh + i]; }
-    __device__ T b(const Prob& p, int k, int j) const {
-        const int l = p.m + p.par + 2 * k;
-        return p.B[tri_idx(l, p.m) * 2 * ncol + j];
+    __device__ const T* a_row(const Prob& p, int k) const { return p.A + (long)k * jh; }
+    __device__ const T* b_row(const Prob& p, int k) const {
+        return p.B + tri_idx(p.m + p.par + 2 * k, p.m) * 2 * ncol;
     }
     __device__ void c(const Prob& p, int i, int j, T v) const { p.C[(long)i * 2 * ncol + j] = v; }
 };
@@ -181,7 +269,8 @@ struct ZtLegendreBwd {
 template <class TT>
 struct ZtRadialFwd {
     using T = TT;
-    static constexpr int A_ORDER = K_FAST, B_ORDER = PAIR_K_FAST;
+    static constexpr int A_KIND = ROWS_OUT, B_KIND = ROWS_OUT;
+    static constexpr bool B_PAIR = true;
     const T* rtab; const T* flm; double* coeffs;
     long flm_stride, coeff_stride;  // elements of T, doubles
     int nt, nr, unnormed;
@@ -195,12 +284,10 @@ struct ZtRadialFwd {
         p.C = coeffs + inst * coeff_stride;
         return p;
     }
-    __device__ T a(const Prob& p, int i, int k) const {
-        return rtab[rad_idx(p.l + 2 * i, p.l) * nr + k];
+    __device__ const T* a_row(const Prob& p, int i) const {
+        return rtab + rad_idx(p.l + 2 * i, p.l) * nr;
     }
-    __device__ T b(const Prob& p, int k, int j) const {
-        return p.B[(long)(j >> 1) * 2 * nr + 2 * k + (j & 1)];
-    }
+    __device__ const T* b_row(const Prob& p, int m) const { return p.B + (long)m * 2 * nr; }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         const int n = p.l + 2 * i;
         const double kappa = unnormed ? (double)(2 * n + 3) : 1.0;
@@ -213,7 +300,8 @@ struct ZtRadialFwd {
 template <class TT>
 struct ZtRadialBwd {
     using T = TT;
-    static constexpr int A_ORDER = MN_FAST, B_ORDER = MN_FAST;
+    static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_K;
+    static constexpr bool B_PAIR = false;
     const T* rtab; const double* coeffs; T* flm;
     long flm_stride, coeff_stride;
     int nt, nr;
@@ -227,11 +315,11 @@ struct ZtRadialBwd {
         p.C = flm + inst * flm_stride + tri_idx(p.l, 0) * 2 * nr;
         return p;
     }
-    __device__ T a(const Prob& p, int i, int k) const {
-        return (T)p.A[tet_idx(p.l + 2 * k, p.l, 0) * 2 + i];
+    __device__ const double* a_row(const Prob& p, int k) const {
+        return p.A + tet_idx(p.l + 2 * k, p.l, 0) * 2;
     }
-    __device__ T b(const Prob& p, int k, int j) const {
-        return rtab[rad_idx(p.l + 2 * k, p.l) * nr + j];
+    __device__ const T* b_row(const Prob& p, int k) const {
+        return rtab + rad_idx(p.l + 2 * k, p.l) * nr;
     }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         p.C[(long)(i >> 1) * 2 * nr + 2 * j + (i & 1)] = v;
@@ -243,7 +331,8 @@ struct ZtRadialBwd {
 template <class TT>
 struct StLegendreFwd {
     using T = TT;
-    static constexpr int A_ORDER = K_FAST, B_ORDER = PAIR_K_FAST;
+    static constexpr int A_KIND = ROWS_OUT, B_KIND = ROWS_OUT;
+    static constexpr bool B_PAIR = true;
     const T* ptab; const T* spec; double* coeffs;
     long spec_stride, coeff_stride;  // elements of T, doubles
     int order, nt, jh, nb;
@@ -257,20 +346,21 @@ struct StLegendreFwd {
         p.boff = (long)(p.m * 2 + p.par) * jh * 2;
         return p;
     }
-    __device__ T a(const Prob& p, int i, int k) const { return p.A[(long)i * jh + k]; }
-    __device__ T b(const Prob& p, int k, int j) const {
-        return spec[(long)(j >> 1) * spec_stride + p.boff + 2 * k + (j & 1)];
-    }
+    __device__ const T* a_row(const Prob& p, int i) const { return p.A + (long)i * jh; }
+    __device__ const T* b_row(const Prob& p, int b) const { return spec + (long)b * spec_stride + p.boff; }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         const int l = p.m + p.par + 2 * i;
         coeffs[(long)(j >> 1) * coeff_stride + tri_idx(l, p.m) * 2 + (j & 1)] = (double)v;
     }
 };
 
+// Backward: the coefficient operand is gathered per (instance, re/im) column,
+// so it is staged as rows over the contraction axis with a strided column.
 template <class TT>
 struct StLegendreBwd {
     using T = TT;
-    static constexpr int A_ORDER = MN_FAST, B_ORDER = MN_FAST;
+    static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_OUT;
+    static constexpr bool B_PAIR = true;
     const T* ptab; const double* coeffs; T* spec;
     long spec_stride, coeff_stride;
     int order, nt, jh, nb;
@@ -284,10 +374,19 @@ struct StLegendreBwd {
         p.soff = (long)(p.m * 2 + p.par) * jh * 2;
         return p;
     }
-    __device__ T a(const Prob& p, int i, int k) const { return p.A[(long)k * jh + i]; }
-    __device__ T b(const Prob& p, int k, int j) const {
-        const int l = p.m + p.par + 2 * k;
-        return (T)coeffs[(long)(j >> 1) * coeff_stride + tri_idx(l, p.m) * 2 + (j & 1)];
+    __device__ const T* a_row(const Prob& p, int k) const { return p.A + (long)k * jh; }
+    // Row b of the B operand: element 2k + ri is c[b][l(k), m][ri]. The rows
+    // are not contiguous in memory, so b_row returns a proxy.
+    struct Gather {
+        const double* base;
+        int m, par;
+        __device__ double operator[](int e) const {
+            const int k = e >> 1;
+            return base[tri_idx(m + par + 2 * k, m) * 2 + (e & 1)];
+        }
+    };
+    __device__ Gather b_row(const Prob& p, int b) const {
+        return Gather{coeffs + (long)b * coeff_stride, p.m, p.par};
     }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         spec[(long)(j >> 1) * spec_stride + p.soff + 2 * i + (j & 1)] = v;
@@ -298,7 +397,7 @@ template <int NS, class S>
 void launch(const S& st, const int4* work, int nwork, int ncols, int nz, hipStream_t s) {
     if (nwork == 0 || ncols <= 0 || nz <= 0) return;
     dim3 grid(nwork, (ncols + TN - 1) / TN, nz);
-    gemm_kernel<S, NS><<<grid, 256, 0, s>>>(st, work);
+    gemm_kernel<S, NS><<<grid, kThreads, 0, s>>>(st, work);
     check_hip(hipGetLastError(), "gemm launch");
 }
 
