@@ -60,7 +60,47 @@ static size_t tet_idx(size_t n, size_t l, size_t m) {
 
 /* ---- Gauss-Legendre nodes and weights --------------------------------------
  * Unpacked, ascending z (nodes[0] ~ -1), cos style, as
- * gl::gl_nodes_and_weights<UnpackedLayout, GLNodeStyle::cos> returns them. */
+ * gl::gl_nodes_and_weights<UnpackedLayout, GLNodeStyle::cos> returns them.
+ * Newton's method on the Legendre three-term recurrence in double precision,
+ * then two Newton steps and the weight 2 / ((1 - x^2) P_n'(x)^2) in
+ * double-double arithmetic (error-free sums and FMA products), so the rounded
+ * nodes and weights are within about an ulp. */
+typedef struct { double hi, lo; } dd_t;
+
+static dd_t dd_two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    dd_t r = {s, (a - (s - bb)) + (b - bb)};
+    return r;
+}
+static dd_t dd_fast_two_sum(double a, double b) {
+    const double s = a + b;
+    dd_t r = {s, b - (s - a)};
+    return r;
+}
+static dd_t dd_add(dd_t x, dd_t y) {
+    dd_t s = dd_two_sum(x.hi, y.hi);
+    return dd_fast_two_sum(s.hi, s.lo + x.lo + y.lo);
+}
+static dd_t dd_neg(dd_t x) { dd_t r = {-x.hi, -x.lo}; return r; }
+static dd_t dd_mul(dd_t x, dd_t y) {
+    const double p = x.hi * y.hi, e = fma(x.hi, y.hi, -p);
+    return dd_fast_two_sum(p, e + (x.hi * y.lo + x.lo * y.hi));
+}
+static dd_t dd_mul_d(dd_t x, double y) {
+    const double p = x.hi * y, e = fma(x.hi, y, -p);
+    return dd_fast_two_sum(p, e + x.lo * y);
+}
+static dd_t dd_div(dd_t x, dd_t y) {
+    const double q1 = x.hi / y.hi;
+    dd_t r = dd_add(x, dd_neg(dd_mul_d(y, q1)));
+    const double q2 = r.hi / y.hi;
+    r = dd_add(r, dd_neg(dd_mul_d(y, q2)));
+    const double q3 = r.hi / y.hi;
+    dd_t q = dd_fast_two_sum(q1, q2);
+    dd_t c = {q3, 0.0};
+    return dd_add(q, c);
+}
+
 static void legendre_p_dp(size_t n, double x, double *p, double *dp) {
     double p0 = 1.0, p1 = x;
     if (n == 0) { *p = 1.0; *dp = 0.0; return; }
@@ -73,32 +113,48 @@ static void legendre_p_dp(size_t n, double x, double *p, double *dp) {
     *dp = (double)n * (x * p1 - p0) / (x * x - 1.0);
 }
 
+static void legendre_p_dp_dd(size_t n, dd_t x, dd_t *p, dd_t *dp) {
+    dd_t p0 = {1.0, 0.0}, p1 = x;
+    for (size_t k = 1; k < n; ++k) {
+        dd_t t = dd_add(dd_mul_d(dd_mul(x, p1), (double)(2 * k + 1)), dd_neg(dd_mul_d(p0, (double)k)));
+        dd_t kk = {(double)(k + 1), 0.0};
+        p0 = p1;
+        p1 = dd_div(t, kk);
+    }
+    *p = p1;
+    dd_t one = {1.0, 0.0};
+    dd_t num = dd_mul_d(dd_add(dd_mul(x, p1), dd_neg(p0)), (double)n);
+    *dp = dd_div(num, dd_add(dd_mul(x, x), dd_neg(one)));
+}
+
 void zo_gl_nodes_weights(size_t n, double *nodes, double *weights) {
     if (n == 0) return;
+    if (n == 1) { nodes[0] = 0.0; weights[0] = 2.0; return; }
     const size_t half = n / 2;
     for (size_t k = 0; k < (n + 1) / 2; ++k) {
         /* k-th largest root; Tricomi's initial guess */
-        double theta = ZO_PI * ((double)k + 0.75) / ((double)n + 0.5);
-        double x = cos(theta);
+        double x = cos(ZO_PI * ((double)k + 0.75) / ((double)n + 0.5));
         double p, dp;
-        /* quadratic convergence: one more step once the update is below 1e-15 */
         for (int it = 0; it < 100; ++it) {
             legendre_p_dp(n, x, &p, &dp);
-            double dx = p / dp;
+            const double dx = p / dp;
             x -= dx;
-            if (fabs(dx) <= 1e-15) {
-                legendre_p_dp(n, x, &p, &dp);
-                x -= p / dp;
-                break;
-            }
+            if (fabs(dx) <= 1e-15) break;
         }
-        legendre_p_dp(n, x, &p, &dp);
-        double w = 2.0 / ((1.0 - x * x) * dp * dp);
-        if ((n & 1) && k == half) { x = 0.0; }
-        nodes[n - 1 - k] = x;
-        nodes[k] = -x;
-        weights[n - 1 - k] = w;
-        weights[k] = w;
+        dd_t xd = {x, 0.0}, pd, dpd;
+        if ((n & 1) && k == half) xd.hi = 0.0;
+        else
+            for (int it = 0; it < 2; ++it) {
+                legendre_p_dp_dd(n, xd, &pd, &dpd);
+                xd = dd_add(xd, dd_neg(dd_div(pd, dpd)));
+            }
+        legendre_p_dp_dd(n, xd, &pd, &dpd);
+        dd_t one = {1.0, 0.0}, two = {2.0, 0.0};
+        dd_t w = dd_div(two, dd_mul(dd_add(one, dd_neg(dd_mul(xd, xd))), dd_mul(dpd, dpd)));
+        nodes[n - 1 - k] = xd.hi;
+        nodes[k] = -xd.hi;
+        weights[n - 1 - k] = w.hi;
+        weights[k] = w.hi;
     }
 }
 

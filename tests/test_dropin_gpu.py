@@ -1,0 +1,28 @@
+"""zest's own transformer test programs, compiled unchanged against the drop-in
+headers (include/zest_amd/) and libzgl.so, run on the GPU.
+
+The programs are zest's tests/test_zernike_glq_transformer.cpp (known-answer
+forward and backward transforms of single basis functions at order 6 and an
+order-100 round trip at 1e-13, all for (normed, geo, none) and
+(unnormed, qm, cs)) and tests/test_sh_glq_transformer.cpp (the same for
+spherical harmonics over both grid layouts and all four conventions). They are
+built by tests/dropin/Makefile where the zest sources are present; the test
+skips when the binaries were not built.
+"""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "_bin")
+
+
+@pytest.mark.parametrize("name", ["zest_test_zernike_glq_transformer", "zest_test_sh_glq_transformer"])
+def test_zest_transformer_tests_pass_on_gpu(name):
+    exe = os.path.join(BIN, name)
+    if not os.path.exists(exe):
+        pytest.skip("drop-in test programs not built (zest sources absent at build time)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, f"{name} failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"

@@ -42,8 +42,9 @@ def test_gl_nodes_match_leggauss(n):
 @pytest.mark.parametrize("n", [1, 2, 7, 32, 33, 69, 70, 71, 129, 136, 512])
 def test_gl_nodes_match_zest_golden(n):
     z, w = oracle.gl_nodes_weights(n)
-    assert np.abs(z - GOLDEN[f"gl_{n}_nodes"]).max() < 1e-15
-    assert rel(w, GOLDEN[f"gl_{n}_weights"]) < 2e-14
+    # zest tabulates below 70 nodes and uses Bogaert's ~1 ulp formulas above
+    assert np.abs(z - GOLDEN[f"gl_{n}_nodes"]).max() < 5e-16
+    assert rel(w, GOLDEN[f"gl_{n}_weights"]) < (1e-16 if n < 70 else 1e-15)
 
 
 @pytest.mark.parametrize("n", [31, 32, 111, 112])

@@ -28,35 +28,88 @@ __device__ void legendre_p_dp(int n, double x, double& p, double& dp) {
     dp = (double)n * (x * p1 - p0) / (x * x - 1.0);
 }
 
-// Gauss-Legendre nodes (ascending, cos style) and weights of n points:
-// Newton's method on P_n from Tricomi's initial guess, one thread per node
-// pair. Same node set as gl::gl_nodes_and_weights<UnpackedLayout, cos>
-// (include/gauss_legendre.hpp:1549-1562), which uses tables below 70 nodes
-// and Bogaert's formulas above; all agree to a few ulp.
+// Double-double arithmetic (error-free sums, FMA products) for the final
+// Newton steps and the weights.
+struct dd { double hi, lo; };
+__device__ inline dd dd_two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline dd dd_fast_two_sum(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ inline dd dd_add(dd x, dd y) {
+    const dd s = dd_two_sum(x.hi, y.hi);
+    return dd_fast_two_sum(s.hi, s.lo + x.lo + y.lo);
+}
+__device__ inline dd dd_neg(dd x) { return {-x.hi, -x.lo}; }
+__device__ inline dd dd_mul(dd x, dd y) {
+    const double p = x.hi * y.hi, e = fma(x.hi, y.hi, -p);
+    return dd_fast_two_sum(p, e + (x.hi * y.lo + x.lo * y.hi));
+}
+__device__ inline dd dd_mul_d(dd x, double y) {
+    const double p = x.hi * y, e = fma(x.hi, y, -p);
+    return dd_fast_two_sum(p, e + x.lo * y);
+}
+__device__ inline dd dd_div(dd x, dd y) {
+    const double q1 = x.hi / y.hi;
+    dd r = dd_add(x, dd_neg(dd_mul_d(y, q1)));
+    const double q2 = r.hi / y.hi;
+    r = dd_add(r, dd_neg(dd_mul_d(y, q2)));
+    const double q3 = r.hi / y.hi;
+    return dd_add(dd_fast_two_sum(q1, q2), dd{q3, 0.0});
+}
+
+__device__ void legendre_p_dp_dd(int n, dd x, dd& p, dd& dp) {
+    dd p0{1.0, 0.0}, p1 = x;
+    for (int k = 1; k < n; ++k) {
+        const dd t = dd_add(dd_mul_d(dd_mul(x, p1), (double)(2 * k + 1)), dd_neg(dd_mul_d(p0, (double)k)));
+        p0 = p1;
+        p1 = dd_div(t, dd{(double)(k + 1), 0.0});
+    }
+    p = p1;
+    const dd num = dd_mul_d(dd_add(dd_mul(x, p1), dd_neg(p0)), (double)n);
+    dp = dd_div(num, dd_add(dd_mul(x, x), dd{-1.0, 0.0}));
+}
+
+// Gauss-Legendre nodes (ascending, cos style) and weights of n points, one
+// thread per node pair: Newton's method on P_n from Tricomi's initial guess,
+// then two Newton steps and the weight 2 / ((1 - x^2) P_n'(x)^2) in
+// double-double, so both round to within about an ulp. Same node set as
+// gl::gl_nodes_and_weights<UnpackedLayout, cos> (include/gauss_legendre.hpp:1549-1562),
+// which uses tables below 70 nodes and Bogaert's formulas above.
 __global__ void gl_kernel(int n, double* nodes, double* weights) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (n + 1) / 2) return;
+    if (n == 1) {
+        nodes[0] = 0.0;
+        weights[0] = 2.0;
+        return;
+    }
     double x = cos(M_PI * ((double)k + 0.75) / ((double)n + 0.5));
     double p, dp;
-    // Quadratic convergence: stop one step after the update falls below
-    // 1e-15 (the last step moves the root by at most an ulp or two).
     for (int it = 0; it < 100; ++it) {
         legendre_p_dp(n, x, p, dp);
         const double dx = p / dp;
         x -= dx;
-        if (fabs(dx) <= 1e-15) {
-            legendre_p_dp(n, x, p, dp);
-            x -= p / dp;
-            break;
+        if (fabs(dx) <= 1e-15) break;
+    }
+    dd xd{x, 0.0}, pd, dpd;
+    if ((n & 1) && k == n / 2) {
+        xd.hi = 0.0;
+    } else {
+        for (int it = 0; it < 2; ++it) {
+            legendre_p_dp_dd(n, xd, pd, dpd);
+            xd = dd_add(xd, dd_neg(dd_div(pd, dpd)));
         }
     }
-    legendre_p_dp(n, x, p, dp);
-    const double w = 2.0 / ((1.0 - x * x) * dp * dp);
-    if ((n & 1) && k == n / 2) x = 0.0;
-    nodes[n - 1 - k] = x;
-    nodes[k] = -x;
-    weights[n - 1 - k] = w;
-    weights[k] = w;
+    legendre_p_dp_dd(n, xd, pd, dpd);
+    const dd w = dd_div(dd{2.0, 0.0}, dd_mul(dd_add(dd{1.0, 0.0}, dd_neg(dd_mul(xd, xd))), dd_mul(dpd, dpd)));
+    nodes[n - 1 - k] = xd.hi;
+    nodes[k] = -xd.hi;
+    weights[n - 1 - k] = w.hi;
+    weights[k] = w.hi;
 }
 
 // North-half weights w[nth/2 + h] and nodes z[nth/2 + h]; for Zernike also
