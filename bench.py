@@ -9,10 +9,16 @@ all 64 expansions followed by one forward transform of the resulting grids,
 with every input already resident in HBM. value = instances * steps / time of
 the mixed form; the FP64 form is timed the same way beside it.
 
-Accuracy gate of the mixed form (err <= 1e-6): the forward transform of the
-FP64 backward grids, max over instances of max|c - c64| / max|c64| against the
-FP64 forward. The plain FP32 forward is used when it meets the gate, else the
-forward with one FP64-residual refinement step.
+Two mixed forms are timed: with the grids stored in FP64 (zgl_forward /
+zgl_backward of an FP32 plan) and with the grids stored in FP32
+(zgl_forward_grid32 / zgl_backward_grid32); tables and coefficients are FP64
+in both. Accuracy gate (err <= 1e-6): max over instances of
+max|c - c64| / max|c64|, where c64 is the FP64 forward of the FP64 backward
+grids and c the mixed forward of the same grids as that form stores them (so
+the FP32-grid form's error includes rounding the grids to FP32). The plain
+FP32 forward is used when it meets the gate, else the forward with one
+FP64-residual refinement step. The headline is the faster form that meets
+the gate; the other is reported beside it.
 
 The dominant kernel's roofline comes from HIP events recorded around each
 kernel launch on its own stream during the timed region. The CPU baseline
@@ -36,14 +42,14 @@ HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 
-def stage_work(order: int, real_bytes: int):
+def stage_work(order: int, real_bytes: int, grid_bytes: int = 8):
     """Algorithmic bytes / flops of one instance per stage (one direction).
-    The grid is FP64 in memory; the spectrum holds complex values of the
-    compute type."""
+    The grid holds `grid_bytes` reals per point; the spectrum holds complex
+    values of the compute type."""
     nphi, nth, nr = 2 * order - 1, order + 1, (order + 1 + 7) // 8 * 8
     F = nphi // 2 + 1
     ncoef = (order + 1) * (order + 3) * (2 * order + 1) // 24
-    fft_bytes = nphi * nth * nr * 8 + F * nth * nr * 2 * real_bytes
+    fft_bytes = nphi * nth * nr * grid_bytes + F * nth * nr * 2 * real_bytes
     legendre_flops = 2 * (order * (order + 1) // 2) * nth * nr
     radial_flops = 4 * ncoef * nr
     return {
@@ -53,8 +59,8 @@ def stage_work(order: int, real_bytes: int):
     }
 
 
-def stage_table(stats, order, dtype):
-    work = stage_work(order, 8 if dtype == "f64" else 4)
+def stage_table(stats, order, dtype, grid_bytes=8):
+    work = stage_work(order, 8 if dtype == "f64" else 4, grid_bytes)
     stages = {}
     for name, (launches, inst, ms) in stats.items():
         if name not in work or launches == 0:
@@ -130,7 +136,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+    ap.add_argument("--only", choices=["fp64", "mixed", "mixed_grid32"], default=None,
                     help="time one form only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -158,31 +164,38 @@ def main():
     p32r = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP32_REFINE, **conv)
 
     def relerr(a, ref):
-        return float(((a - ref).abs().amax(dim=(1, 2)) / ref.abs().amax(dim=(1, 2))).max())
+        dims = tuple(range(1, ref.dim()))
+        return float(((a.double() - ref).abs().amax(dim=dims) / ref.abs().amax(dim=dims)).max())
 
     # accuracy of the mixed forms against FP64 on the FP64 backward grids
     p64.backward(dc, dg, N, batch=B, stream=stream)
     c64 = torch.zeros_like(dc)
     p64.forward(dg, c64, N, batch=B, stream=stream)
-    c32 = torch.zeros_like(dc)
-    p32.forward(dg, c32, N, batch=B, stream=stream)
-    c32r = torch.zeros_like(dc)
-    p32r.forward(dg, c32r, N, batch=B, stream=stream)
-    g32 = torch.empty_like(dg)
-    p32.backward(dc, g32, N, batch=B, stream=stream)
-    torch.cuda.synchronize()
-    err64_roundtrip = relerr(c64, dc)
-    err32 = relerr(c32, c64)
-    err32r = relerr(c32r, c64)
-    err32_bwd = float(((g32 - dg).abs().amax(dim=(1, 2, 3)) / dg.abs().amax(dim=(1, 2, 3))).max())
-    del g32, c32, c32r
-    mixed_fwd = p32 if err32 <= 1e-6 else p32r
-    mixed_name = "fp32" if err32 <= 1e-6 else "fp32+refine"
+    acc = {"fp64_roundtrip_max_rel_err": None}
+    forms = {}
+    for form, gdtype in (("mixed", torch.float64), ("mixed_grid32", torch.float32)):
+        gin = dg.to(gdtype)
+        c32 = torch.zeros_like(dc)
+        p32.forward(gin, c32, N, batch=B, stream=stream)
+        c32r = torch.zeros_like(dc)
+        p32r.forward(gin, c32r, N, batch=B, stream=stream)
+        gout = torch.empty_like(gin)
+        p32.backward(dc, gout, N, batch=B, stream=stream)
+        torch.cuda.synchronize()
+        e32, e32r, ebwd = relerr(c32, c64), relerr(c32r, c64), relerr(gout, dg)
+        acc[form] = {"fwd_max_rel_err_vs_fp64": e32, "refined_fwd_max_rel_err_vs_fp64": e32r,
+                     "bwd_max_rel_err_vs_fp64": ebwd}
+        fwd_plan, fwd_name = (p32, "fp32") if e32 <= 1e-6 else (p32r, "fp32+refine")
+        gate = min(e32, e32r) <= 1e-6
+        forms[form] = (gdtype, fwd_plan, fwd_name, gate)
+        del gin, c32, c32r, gout
+    acc["fp64_roundtrip_max_rel_err"] = relerr(c64, dc)
+    dg32 = torch.empty(dg.shape, dtype=torch.float32, device=dev)
 
-    def timed(plan_bwd, plan_fwd):
+    def timed(plan_bwd, plan_fwd, grid):
         def step():
-            plan_bwd.backward(dc, dg, N, batch=B, stream=stream)
-            plan_fwd.forward(dg, dco, N, batch=B, stream=stream)
+            plan_bwd.backward(dc, grid, N, batch=B, stream=stream)
+            plan_fwd.forward(grid, dco, N, batch=B, stream=stream)
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
@@ -205,15 +218,18 @@ def main():
 
     results = {}
     if args.only in (None, "fp64"):
-        results["fp64"] = timed(p64, p64)
-    if args.only in (None, "mixed"):
-        results["mixed"] = timed(p32, mixed_fwd)
+        results["fp64"] = timed(p64, p64, dg)
+    for form, (gdtype, fwd_plan, _, gate) in forms.items():
+        if gate and args.only in (None, form):
+            results[form] = timed(p32, fwd_plan, dg if gdtype == torch.float64 else dg32)
 
-    head = "mixed" if "mixed" in results else "fp64"
+    mixed = [f for f in ("mixed", "mixed_grid32") if f in results]
+    head = max(mixed, key=lambda f: -results[f][0]) if mixed else "fp64"
     elapsed, stats = results[head]
-    dtype = "f32" if head == "mixed" else "f64"
+    dtype = "f32" if head != "fp64" else "f64"
+    gbytes = 4 if head == "mixed_grid32" else 8
     value = B * args.steps / elapsed
-    stages = stage_table(stats, N, dtype)
+    stages = stage_table(stats, N, dtype, gbytes)
     dom_name = max(stages, key=lambda k: stages[k]["share"])
     dom = stages[dom_name]
     out = {
@@ -231,21 +247,28 @@ def main():
         "data": "synthetic",
         "config": {"workload": "zernike_glq_backward_forward", "order": N, "instances": B,
                    "grid": list(zest_amd.zt_shape(N)), "conventions": "unnormed,geo,none",
-                   "mixed_form": mixed_name if head == "mixed" else None},
+                   "form": head,
+                   "grid_storage": "f32" if head == "mixed_grid32" else "f64",
+                   "forward": forms[head][2] if head in forms else "fp64"},
         "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
                      "unit": dom["unit"], "frac": dom["frac"], "traffic": None,
                      "kernel": dom_name},
-        "accuracy": {"mixed_fwd_max_rel_err_vs_fp64": err32, "refined_fwd_max_rel_err_vs_fp64": err32r,
-                     "mixed_bwd_max_rel_err_vs_fp64": err32_bwd,
-                     "fp64_roundtrip_max_rel_err": err64_roundtrip},
+        "accuracy": acc,
         "stages": stages,
     }
+    for form in mixed:
+        if form == head:
+            continue
+        e, st = results[form]
+        out[form] = {"value": round(B * args.steps / e, 2),
+                     "ms_per_step": round(e / args.steps * 1e3, 4), "forward": forms[form][2],
+                     "stages": stage_table(st, N, "f32", 4 if form == "mixed_grid32" else 8)}
     if "fp64" in results:
         e64, s64 = results["fp64"]
         out["fp64_value"] = round(B * args.steps / e64, 2)
         out["fp64_ms_per_step"] = round(e64 / args.steps * 1e3, 4)
         out["fp64_stages"] = stage_table(s64, N, "f64")
-        if head == "mixed":
+        if head != "fp64":
             out["mixed_vs_fp64"] = round(value / out["fp64_value"], 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds)

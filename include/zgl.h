@@ -65,9 +65,10 @@ enum { ZGL_PHASE_NONE = -1, ZGL_PHASE_CS = 1 };
 /* SH grid layouts (include/sh_glq_transformer.hpp:44-195) */
 enum { ZGL_LAYOUT_LONLAT = 0, ZGL_LAYOUT_LATLON = 1 };
 
-/* Arithmetic of the transforms. Grids and coefficients are FP64 in memory in
- * every mode; the tables are always generated in FP64 (the FP32 modes use
- * their FP32 roundings). */
+/* Arithmetic of the transforms. Coefficients are FP64 in memory in every
+ * mode; grids are FP64 except through the *_grid32 entry points of the FP32
+ * modes; the tables are always generated in FP64 (the FP32 modes use their
+ * FP32 roundings). */
 typedef enum zgl_precision {
     ZGL_FP64 = 0,        /* FP64 throughout: the drop-in form, matches zest */
     ZGL_FP32 = 1,        /* FP32 FFT and FP32-MFMA contractions, FP32 accumulation */
@@ -127,6 +128,25 @@ zgl_status zgl_backward(zgl_plan *plan, const double *coeffs, size_t exp_order,
                         double *grid, size_t grid_order, size_t batch,
                         ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
                         zgl_mem mem, void *stream);
+
+/* zgl_forward / zgl_backward with the grids stored in FP32 (same layouts,
+ * floats instead of doubles; strides in floats). Only for plans of precision
+ * ZGL_FP32 or ZGL_FP32_REFINE (ZGL_ERROR_NOT_SUPPORTED otherwise); the
+ * coefficients stay FP64. With ZGL_FP32_REFINE the forward residual
+ * g - B64(c) is formed in FP64 against the FP32 grid values, so the result
+ * is the FP64-accurate transform of the grid as stored. Replaces the same
+ * zest members as zgl_forward / zgl_backward
+ * (include/zernike_glq_transformer.hpp:640-679,
+ *  include/sh_glq_transformer.hpp:632-668) for callers that hold their
+ * grids in single precision. */
+zgl_status zgl_forward_grid32(zgl_plan *plan, const float *grid, size_t grid_order,
+                              double *coeffs, size_t exp_order, size_t batch,
+                              ptrdiff_t grid_stride, ptrdiff_t coeff_stride,
+                              zgl_mem mem, void *stream);
+zgl_status zgl_backward_grid32(zgl_plan *plan, const double *coeffs, size_t exp_order,
+                               float *grid, size_t grid_order, size_t batch,
+                               ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
+                               zgl_mem mem, void *stream);
 
 /* Stage timing. No counterpart in the reference: it serves benchmarks and
  * profilers. While enabled, every kernel launch of the plan is bracketed by

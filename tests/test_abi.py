@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "zgl.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(zgl_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(zgl_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -85,3 +85,6 @@ def test_numpy_arrays_must_be_contiguous_float64():
         zest_amd._ptr_and_mem(np.zeros(4, dtype=np.float32))
     with pytest.raises(ValueError):
         zest_amd._ptr_and_mem(np.zeros((4, 4))[:, 0])
+    # grids may be float32 (the *_grid32 entry points); coefficients may not
+    assert zest_amd._ptr_and_mem(np.zeros(4, dtype=np.float32), allow_f32=True)[2]
+    assert not zest_amd._ptr_and_mem(np.zeros(4), allow_f32=True)[2]

@@ -65,6 +65,10 @@ def _load():
     lib.zgl_forward.restype = i32
     lib.zgl_backward.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_backward.restype = i32
+    lib.zgl_forward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_forward_grid32.restype = i32
+    lib.zgl_backward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_backward_grid32.restype = i32
     lib.zgl_last_error_string.argtypes = []
     lib.zgl_last_error_string.restype = ctypes.c_char_p
     lib.zgl_profile_enable.argtypes = [vp, i32]
@@ -86,6 +90,7 @@ LIB = _load()
 EXPORTS = (
     "zgl_plan_create", "zgl_plan_destroy", "zgl_plan_resize", "zgl_plan_order",
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
+    "zgl_forward_grid32", "zgl_backward_grid32",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -117,18 +122,24 @@ def st_num_coeffs(order: int) -> int:
     return order * (order + 1) // 2
 
 
-def _ptr_and_mem(x):
-    """(pointer, memory kind) of a torch tensor or numpy array."""
+def _ptr_and_mem(x, allow_f32: bool = False):
+    """(pointer, memory kind, is_float32) of a torch tensor or numpy array."""
     if isinstance(x, np.ndarray):
-        if not x.flags.c_contiguous or x.dtype != np.float64:
-            raise ValueError("arrays must be C-contiguous float64")
-        return x.ctypes.data, MEM_HOST
+        if not x.flags.c_contiguous:
+            raise ValueError("arrays must be C-contiguous")
+        f32 = x.dtype == np.float32
+        if not (x.dtype == np.float64 or (allow_f32 and f32)):
+            raise ValueError("arrays must be float64" + (" or float32" if allow_f32 else ""))
+        return x.ctypes.data, MEM_HOST, f32
     import torch
     if not isinstance(x, torch.Tensor):
         raise TypeError("expected a numpy array or a torch tensor")
-    if x.dtype != torch.float64 or not x.is_contiguous():
-        raise ValueError("tensors must be contiguous float64")
-    return x.data_ptr(), (MEM_DEVICE if x.is_cuda else MEM_HOST)
+    if not x.is_contiguous():
+        raise ValueError("tensors must be contiguous")
+    f32 = x.dtype == torch.float32
+    if not (x.dtype == torch.float64 or (allow_f32 and f32)):
+        raise ValueError("tensors must be float64" + (" or float32" if allow_f32 else ""))
+    return x.data_ptr(), (MEM_DEVICE if x.is_cuda else MEM_HOST), f32
 
 
 class Plan:
@@ -180,25 +191,29 @@ class Plan:
 
     def forward(self, grid, coeffs, grid_order: int, exp_order: int | None = None,
                 batch: int = 1, grid_stride: int = 0, coeff_stride: int = 0, stream=None):
-        """Enqueue forward transforms of `batch` instances (zgl_forward)."""
+        """Enqueue forward transforms of `batch` instances (zgl_forward, or
+        zgl_forward_grid32 for a float32 grid)."""
         exp_order = grid_order if exp_order is None else exp_order
-        gp, gm = _ptr_and_mem(grid)
-        cp, cm = _ptr_and_mem(coeffs)
+        gp, gm, g32 = _ptr_and_mem(grid, allow_f32=True)
+        cp, cm, _ = _ptr_and_mem(coeffs)
         if gm != cm:
             raise ValueError("grid and coefficients must live in the same memory")
-        _check(LIB.zgl_forward(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
-                               coeff_stride, gm, _stream_ptr(stream, gm)))
+        fn = LIB.zgl_forward_grid32 if g32 else LIB.zgl_forward
+        _check(fn(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
+                  coeff_stride, gm, _stream_ptr(stream, gm)))
 
     def backward(self, coeffs, grid, grid_order: int, exp_order: int | None = None,
                  batch: int = 1, coeff_stride: int = 0, grid_stride: int = 0, stream=None):
-        """Enqueue backward transforms of `batch` instances (zgl_backward)."""
+        """Enqueue backward transforms of `batch` instances (zgl_backward, or
+        zgl_backward_grid32 for a float32 grid)."""
         exp_order = grid_order if exp_order is None else exp_order
-        cp, cm = _ptr_and_mem(coeffs)
-        gp, gm = _ptr_and_mem(grid)
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        gp, gm, g32 = _ptr_and_mem(grid, allow_f32=True)
         if gm != cm:
             raise ValueError("grid and coefficients must live in the same memory")
-        _check(LIB.zgl_backward(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
-                                grid_stride, gm, _stream_ptr(stream, gm)))
+        fn = LIB.zgl_backward_grid32 if g32 else LIB.zgl_backward
+        _check(fn(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
+                  grid_stride, gm, _stream_ptr(stream, gm)))
 
 
 def _stream_ptr(stream, mem):

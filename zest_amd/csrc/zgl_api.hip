@@ -179,8 +179,9 @@ void timed(zgl_plan* p, Stage st, int nb, hipStream_t s, F&& launch) {
 
 double shnorm_value(int sh_norm) { return sh_norm == ZGL_SHNORM_QM ? 1.0 : 1.0 / (4.0 * M_PI); }
 
-template <class T>
-void run_forward(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
+// T: arithmetic (double or float); G: grid element type in memory.
+template <class T, class G>
+void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt,
                  size_t batch, long gs, long cs, hipStream_t s) {
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
@@ -199,7 +200,7 @@ void run_forward(zgl_plan* p, const double* grid, size_t N, double* coeffs, size
     L.center_factor = p->kind == ZGL_ZERNIKE ? 1.0 : 2.0;
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
-        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T>(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
+        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T, G>(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
         if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0, p->spec, ch.spec_stride,
                          p->flm, ch.flm_stride, coeffs + b0 * cs, cs, nb};
@@ -212,8 +213,8 @@ void run_forward(zgl_plan* p, const double* grid, size_t N, double* coeffs, size
     }
 }
 
-template <class T>
-void run_backward(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
+template <class T, class G>
+void run_backward(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t N,
                   size_t batch, long cs, long gs, hipStream_t s) {
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
@@ -240,20 +241,21 @@ void run_backward(zgl_plan* p, const double* coeffs, size_t nt, double* grid, si
                          cs, nb};
             timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward<T>(a, s); });
         }
-        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward<T>(L, p->spec, ch.spec_stride, grid + b0 * gs, gs, nb, s); });
+        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward<T, G>(L, p->spec, ch.spec_stride, grid + b0 * gs, gs, nb, s); });
     }
 }
 
 // Forward in the plan's precision. With ZGL_FP32_REFINE each chunk of
-// instances is refined once: c += F32(g - B64(c)).
-void forward_any(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
+// instances is refined once: c += F32(g - B64(c)), the residual in FP64.
+template <class G>
+void forward_any(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt,
                  size_t batch, long gs, long cs, hipStream_t s) {
     if (p->precision == ZGL_FP64) {
-        run_forward<double>(p, grid, N, coeffs, nt, batch, gs, cs, s);
+        if constexpr (std::is_same_v<G, double>) run_forward<double, G>(p, grid, N, coeffs, nt, batch, gs, cs, s);
         return;
     }
     if (p->precision == ZGL_FP32) {
-        run_forward<float>(p, grid, N, coeffs, nt, batch, gs, cs, s);
+        run_forward<float, G>(p, grid, N, coeffs, nt, batch, gs, cs, s);
         return;
     }
     const long gsz = (long)grid_size(p, N);
@@ -264,22 +266,24 @@ void forward_any(zgl_plan* p, const double* grid, size_t N, double* coeffs, size
     ensure(p->rcoef, p->rcoef_cap, (size_t)ncw * nb_max);
     for (size_t b0 = 0; b0 < batch; b0 += nb_max) {
         const int nb = (int)std::min(nb_max, batch - b0);
-        const double* g = grid + b0 * gs;
+        const G* g = grid + b0 * gs;
         double* c = coeffs + b0 * cs;
-        run_forward<float>(p, g, N, c, nt, nb, gs, cs, s);
-        run_backward<double>(p, c, nt, p->rgrid, N, nb, cs, gsz, s);
-        grid_residual(g, gs, p->rgrid, gsz, gsz, nb, s);
-        run_forward<float>(p, p->rgrid, N, p->rcoef, nt, nb, gsz, ncw, s);
+        run_forward<float, G>(p, g, N, c, nt, nb, gs, cs, s);
+        run_backward<double, double>(p, c, nt, p->rgrid, N, nb, cs, gsz, s);
+        grid_residual<G>(g, gs, p->rgrid, gsz, gsz, nb, s);
+        run_forward<float, double>(p, p->rgrid, N, p->rcoef, nt, nb, gsz, ncw, s);
         coeff_accumulate(c, cs, p->rcoef, ncw, ncw, nb, s);
     }
 }
 
-void backward_any(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
+template <class G>
+void backward_any(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t N,
                   size_t batch, long cs, long gs, hipStream_t s) {
-    if (p->precision == ZGL_FP64)
-        run_backward<double>(p, coeffs, nt, grid, N, batch, cs, gs, s);
-    else
-        run_backward<float>(p, coeffs, nt, grid, N, batch, cs, gs, s);
+    if (p->precision == ZGL_FP64) {
+        if constexpr (std::is_same_v<G, double>) run_backward<double, G>(p, coeffs, nt, grid, N, batch, cs, gs, s);
+    } else {
+        run_backward<float, G>(p, coeffs, nt, grid, N, batch, cs, gs, s);
+    }
 }
 
 template <class F>
@@ -376,12 +380,20 @@ size_t zgl_grid_size(const zgl_plan* plan, size_t order) { return plan ? grid_si
 
 size_t zgl_coeff_size(const zgl_plan* plan, size_t order) { return plan ? coeff_size(plan, order) : 0; }
 
-zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, double* coeffs,
-                       size_t exp_order, size_t batch, ptrdiff_t grid_stride,
-                       ptrdiff_t coeff_stride, zgl_mem mem, void* stream) {
+}  // extern "C"
+
+namespace zgl {
+namespace {
+
+template <class G>
+zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, double* coeffs,
+                         size_t exp_order, size_t batch, ptrdiff_t grid_stride,
+                         ptrdiff_t coeff_stride, zgl_mem mem, void* stream) {
     if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
     if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
     if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    if (std::is_same_v<G, float> && plan->precision == ZGL_FP64)
+        return fail(ZGL_ERROR_NOT_SUPPORTED, "FP32 grids need an FP32 plan");
     const size_t nt = std::min(exp_order, grid_order);
     const long gsz = (long)grid_size(plan, grid_order);
     const long csz = 2 * (long)coeff_size(plan, exp_order);
@@ -396,17 +408,18 @@ zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, do
         if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            forward_any(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+            forward_any<G>(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
             check_hip(hipGetLastError(), "forward");
             return ZGL_SUCCESS;
         }
         const size_t ncw = 2 * coeff_size(plan, nt);  // doubles written per instance
-        ensure(plan->dgrid, plan->dgrid_cap, (size_t)gsz * batch);
+        ensure(plan->dgrid, plan->dgrid_cap, ((size_t)gsz * batch * sizeof(G) + 7) / 8);
         ensure(plan->dcoef, plan->dcoef_cap, ncw * batch);
-        check_hip(hipMemcpy2DAsync(plan->dgrid, gsz * sizeof(double), grid, gs * sizeof(double),
-                                   gsz * sizeof(double), batch, hipMemcpyHostToDevice, s),
+        G* dg = reinterpret_cast<G*>(plan->dgrid);
+        check_hip(hipMemcpy2DAsync(dg, gsz * sizeof(G), grid, gs * sizeof(G), gsz * sizeof(G), batch,
+                                   hipMemcpyHostToDevice, s),
                   "grid upload");
-        forward_any(plan, plan->dgrid, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+        forward_any<G>(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
         check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, ncw * sizeof(double),
                                    ncw * sizeof(double), batch, hipMemcpyDeviceToHost, s),
                   "coefficient download");
@@ -415,12 +428,15 @@ zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, do
     });
 }
 
-zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, double* grid,
-                        size_t grid_order, size_t batch, ptrdiff_t coeff_stride,
-                        ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
+template <class G>
+zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order, G* grid,
+                          size_t grid_order, size_t batch, ptrdiff_t coeff_stride,
+                          ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
     if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
     if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
     if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    if (std::is_same_v<G, float> && plan->precision == ZGL_FP64)
+        return fail(ZGL_ERROR_NOT_SUPPORTED, "FP32 grids need an FP32 plan");
     const size_t nt = std::min(exp_order, grid_order);
     const long gsz = (long)grid_size(plan, grid_order);
     const long csz = 2 * (long)coeff_size(plan, exp_order);
@@ -434,24 +450,58 @@ zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, 
         if (!grid || (!coeffs && nt > 0)) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            backward_any(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+            backward_any<G>(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
             check_hip(hipGetLastError(), "backward");
             return ZGL_SUCCESS;
         }
         const size_t ncr = 2 * coeff_size(plan, nt);  // doubles read per instance
-        ensure(plan->dgrid, plan->dgrid_cap, (size_t)gsz * batch);
+        ensure(plan->dgrid, plan->dgrid_cap, ((size_t)gsz * batch * sizeof(G) + 7) / 8);
         ensure(plan->dcoef, plan->dcoef_cap, std::max<size_t>(ncr, 1) * batch);
+        G* dg = reinterpret_cast<G*>(plan->dgrid);
         if (ncr)
             check_hip(hipMemcpy2DAsync(plan->dcoef, ncr * sizeof(double), coeffs, cs * sizeof(double),
                                        ncr * sizeof(double), batch, hipMemcpyHostToDevice, s),
                       "coefficient upload");
-        backward_any(plan, plan->dcoef, nt, plan->dgrid, grid_order, batch, (long)ncr, gsz, s);
-        check_hip(hipMemcpy2DAsync(grid, gs * sizeof(double), plan->dgrid, gsz * sizeof(double),
-                                   gsz * sizeof(double), batch, hipMemcpyDeviceToHost, s),
+        backward_any<G>(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+        check_hip(hipMemcpy2DAsync(grid, gs * sizeof(G), dg, gsz * sizeof(G), gsz * sizeof(G), batch,
+                                   hipMemcpyDeviceToHost, s),
                   "grid download");
         check_hip(hipStreamSynchronize(s), "backward");
         return ZGL_SUCCESS;
     });
+}
+
+}  // namespace
+}  // namespace zgl
+
+extern "C" {
+
+zgl_status zgl_forward(zgl_plan* plan, const double* grid, size_t grid_order, double* coeffs,
+                       size_t exp_order, size_t batch, ptrdiff_t grid_stride,
+                       ptrdiff_t coeff_stride, zgl_mem mem, void* stream) {
+    return forward_entry<double>(plan, grid, grid_order, coeffs, exp_order, batch, grid_stride,
+                                 coeff_stride, mem, stream);
+}
+
+zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, double* grid,
+                        size_t grid_order, size_t batch, ptrdiff_t coeff_stride,
+                        ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
+    return backward_entry<double>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
+                                  grid_stride, mem, stream);
+}
+
+zgl_status zgl_forward_grid32(zgl_plan* plan, const float* grid, size_t grid_order, double* coeffs,
+                              size_t exp_order, size_t batch, ptrdiff_t grid_stride,
+                              ptrdiff_t coeff_stride, zgl_mem mem, void* stream) {
+    return forward_entry<float>(plan, grid, grid_order, coeffs, exp_order, batch, grid_stride,
+                                coeff_stride, mem, stream);
+}
+
+zgl_status zgl_backward_grid32(zgl_plan* plan, const double* coeffs, size_t exp_order, float* grid,
+                               size_t grid_order, size_t batch, ptrdiff_t coeff_stride,
+                               ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
+    return backward_entry<float>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
+                                 grid_stride, mem, stream);
 }
 
 zgl_status zgl_profile_enable(zgl_plan* plan, int enable) {

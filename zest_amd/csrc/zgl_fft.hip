@@ -2,7 +2,8 @@
 // length (odd lengths 2N-1 are often prime), fused with the quadrature
 // weights, the conjugation and the north/south parity split (forward), or
 // with the parity merge, the m > 0 halving and the Hermitian extension
-// (backward). FP64 or FP32 arithmetic; grids are always FP64 in memory.
+// (backward). FP64 or FP32 arithmetic; FP64 grids, or FP32 grids with FP32
+// arithmetic.
 //
 // Replaces pocketfft::r2c / c2r at the call sites
 //   include/zernike_glq_transformer.hpp:710-776 (r2c + apply_weights)
@@ -290,11 +291,11 @@ __device__ inline void stage_tables(C*& data, C*& twl, C*& twp, unsigned short*&
     __syncthreads();
 }
 
-template <int T, int LK, class R>
+template <int T, int LK, class R, class G>
 __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
-    const double* __restrict__ grid, long grid_stride, typename Cplx<R>::type* __restrict__ spec,
+    const G* __restrict__ grid, long grid_stride, typename Cplx<R>::type* __restrict__ spec,
     long spec_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     const int n = d.n;
     int h0, c0;
     tile_origin<T>(pm, blockIdx.x, h0, c0);
-    const double* g = grid + (long)blockIdx.y * grid_stride;
+    const G* g = grid + (long)blockIdx.y * grid_stride;
     C* sp = spec + (long)blockIdx.y * spec_stride;
 
     // Loads are issued in batches of kBatch per thread before any LDS store,
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
                 bv[u] = 0;
                 if (e < n * T && h < pm.jh && c < pm.ncol) {
                     const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
-                    const double* row = g + (long)i * pm.lon_stride + c;
+                    const G* row = g + (long)i * pm.lon_stride + c;
                     av[u] = (R)row[(long)jn * pm.lat_stride];
                     if (js != jn) bv[u] = (R)row[(long)js * pm.lat_stride];
                 }
@@ -390,11 +391,11 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     }
 }
 
-template <int T, int LK, class R>
+template <int T, int LK, class R, class G>
 __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
-    const typename Cplx<R>::type* __restrict__ spec, long spec_stride, double* __restrict__ grid,
+    const typename Cplx<R>::type* __restrict__ spec, long spec_stride, G* __restrict__ grid,
     long grid_stride, PairMap pm, int nm) {
     using C = typename Cplx<R>::type;
     C *data, *twl, *twp;
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     const int n = d.n;
     int h0, c0;
     tile_origin<T>(pm, blockIdx.x, h0, c0);
-    double* g = grid + (long)blockIdx.y * grid_stride;
+    G* g = grid + (long)blockIdx.y * grid_stride;
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
     for (int e0 = threadIdx.x; e0 < F * T; e0 += kThreads * kBatch) {
@@ -459,9 +460,9 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
             if (h >= pm.jh || c >= pm.ncol) continue;
             const C z = data[i * T + t];
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
-            double* row = g + (long)i * pm.lon_stride + c;
-            row[(long)jn * pm.lat_stride] = (double)z.x;
-            if (js != jn) row[(long)js * pm.lat_stride] = (double)z.y;
+            G* row = g + (long)i * pm.lon_stride + c;
+            row[(long)jn * pm.lat_stride] = (G)z.x;
+            if (js != jn) row[(long)js * pm.lat_stride] = (G)z.y;
         }
     } else {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
@@ -470,8 +471,8 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
             if (h >= pm.jh) continue;
             const C z = data[i * T + t];
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
-            g[(long)jn * pm.lat_stride + i] = (double)z.x;
-            if (js != jn) g[(long)js * pm.lat_stride + i] = (double)z.y;
+            g[(long)jn * pm.lat_stride + i] = (G)z.x;
+            if (js != jn) g[(long)js * pm.lat_stride + i] = (G)z.y;
         }
     }
 }
@@ -516,16 +517,16 @@ template <class R> const typename Cplx<R>::type* twp_of(const FftPlan& f);
 template <> const double2* twp_of<double>(const FftPlan& f) { return f.twp; }
 template <> const float2* twp_of<float>(const FftPlan& f) { return f.twp32; }
 
-template <int T, class R>
-void launch_forward(const FftLaunch& L, const double* grid, long gs, void* spec, long ss, int nb,
+template <int T, class R, class G>
+void launch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long ss, int nb,
                     hipStream_t s) {
     using C = typename Cplx<R>::type;
     const FftPlan& f = L.t->fft;
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(num_tiles(pm, T), nb);
-    auto k = L.layout_kind == GRID_COLS ? fft_forward_kernel<T, GRID_COLS, R>
-                                        : fft_forward_kernel<T, GRID_ROWS, R>;
+    auto k = L.layout_kind == GRID_COLS ? fft_forward_kernel<T, GRID_COLS, R, G>
+                                        : fft_forward_kernel<T, GRID_ROWS, R, G>;
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -534,16 +535,16 @@ void launch_forward(const FftLaunch& L, const double* grid, long gs, void* spec,
     check_hip(hipGetLastError(), "fft_forward launch");
 }
 
-template <int T, class R>
-void launch_backward(const FftLaunch& L, const void* spec, long ss, double* grid, long gs, int nb,
+template <int T, class R, class G>
+void launch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, long gs, int nb,
                      hipStream_t s) {
     using C = typename Cplx<R>::type;
     const FftPlan& f = L.t->fft;
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(num_tiles(pm, T), nb);
-    auto k = L.layout_kind == GRID_COLS ? fft_backward_kernel<T, GRID_COLS, R>
-                                        : fft_backward_kernel<T, GRID_ROWS, R>;
+    auto k = L.layout_kind == GRID_COLS ? fft_backward_kernel<T, GRID_COLS, R, G>
+                                        : fft_backward_kernel<T, GRID_ROWS, R, G>;
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -552,8 +553,8 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, double* grid
     check_hip(hipGetLastError(), "fft_backward launch");
 }
 
-template <class R>
-void dispatch_forward(const FftLaunch& L, const double* grid, long gs, void* spec, long ss, int nb,
+template <class R, class G>
+void dispatch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long ss, int nb,
                       hipStream_t s) {
     switch (L.t->fft.tile[sizeof(R) == 4]) {
         case 16: launch_forward<16, R>(L, grid, gs, spec, ss, nb, s); break;
@@ -564,8 +565,8 @@ void dispatch_forward(const FftLaunch& L, const double* grid, long gs, void* spe
     }
 }
 
-template <class R>
-void dispatch_backward(const FftLaunch& L, const void* spec, long ss, double* grid, long gs, int nb,
+template <class R, class G>
+void dispatch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, long gs, int nb,
                        hipStream_t s) {
     switch (L.t->fft.tile[sizeof(R) == 4]) {
         case 16: launch_backward<16, R>(L, spec, ss, grid, gs, nb, s); break;
@@ -683,21 +684,24 @@ void fft_plan_destroy(FftPlan& p) {
     p = FftPlan{};
 }
 
-template <class R>
-void fft_forward(const FftLaunch& L, const double* grid, long grid_stride, void* spec,
+template <class R, class G>
+void fft_forward(const FftLaunch& L, const G* grid, long grid_stride, void* spec,
                  long spec_stride, int nb, hipStream_t s) {
     dispatch_forward<R>(L, grid, grid_stride, spec, spec_stride, nb, s);
 }
 
-template <class R>
-void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, double* grid,
+template <class R, class G>
+void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, G* grid,
                   long grid_stride, int nb, hipStream_t s) {
     dispatch_backward<R>(L, spec, spec_stride, grid, grid_stride, nb, s);
 }
 
-template void fft_forward<double>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
-template void fft_forward<float>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
-template void fft_backward<double>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
-template void fft_backward<float>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
+// FP64 grids with FP64 or FP32 arithmetic; FP32 grids with FP32 arithmetic.
+template void fft_forward<double, double>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
+template void fft_forward<float, double>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
+template void fft_forward<float, float>(const FftLaunch&, const float*, long, void*, long, int, hipStream_t);
+template void fft_backward<double, double>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
+template void fft_backward<float, double>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
+template void fft_backward<float, float>(const FftLaunch&, const void*, long, float*, long, int, hipStream_t);
 
 }  // namespace zgl

@@ -114,14 +114,15 @@ struct FftLaunch {
     double center_factor;   // weight factor of an equator node (zest's SH quirk: 2)
 };
 
-// R = double or float: the arithmetic; spec holds complex R values and its
-// stride counts complex elements.
-template <class R>
-void fft_forward(const FftLaunch& L, const double* grid, long grid_stride,
+// R = double or float: the arithmetic; G: the grid element type (double, or
+// float with R = float). spec holds complex R values and its stride counts
+// complex elements.
+template <class R, class G>
+void fft_forward(const FftLaunch& L, const G* grid, long grid_stride,
                  void* spec, long spec_stride, int nb, hipStream_t s);
-template <class R>
+template <class R, class G>
 void fft_backward(const FftLaunch& L, const void* spec, long spec_stride,
-                  double* grid, long grid_stride, int nb, hipStream_t s);
+                  G* grid, long grid_stride, int nb, hipStream_t s);
 
 // ---- contractions (zgl_gemm.hip) ----
 struct ZtGemmArgs {
@@ -149,8 +150,9 @@ template <class T> void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
 template <class T> void st_legendre_backward(const StGemmArgs& a, hipStream_t s);
 
 // ---- elementwise helpers (zgl_tables.hip) ----
-// r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in doubles.
-void grid_residual(const double* g, long gs, double* r, long rs, long n, int nb, hipStream_t s);
+// r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in elements.
+template <class G>
+void grid_residual(const G* g, long gs, double* r, long rs, long n, int nb, hipStream_t s);
 // c[b][i] += d[b][i] for i < n.
 void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s);
 

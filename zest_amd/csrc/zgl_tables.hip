@@ -243,11 +243,12 @@ __global__ void to_fp32_kernel(const double* in, float* out, long n) {
         out[i] = (float)in[i];
 }
 
-__global__ void residual_kernel(const double* g, long gs, double* r, long rs, long n) {
-    const double* gb = g + blockIdx.y * gs;
+template <class G>
+__global__ void residual_kernel(const G* g, long gs, double* r, long rs, long n) {
+    const G* gb = g + blockIdx.y * gs;
     double* rb = r + blockIdx.y * rs;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        rb[i] = gb[i] - rb[i];
+        rb[i] = (double)gb[i] - rb[i];
 }
 
 __global__ void accumulate_kernel(double* c, long cs, const double* d, long ds, long n) {
@@ -277,11 +278,14 @@ int4* upload_work(const std::vector<int4>& w, hipStream_t s) {
 
 }  // namespace
 
-void grid_residual(const double* g, long gs, double* r, long rs, long n, int nb, hipStream_t s) {
+template <class G>
+void grid_residual(const G* g, long gs, double* r, long rs, long n, int nb, hipStream_t s) {
     if (n <= 0 || nb <= 0) return;
-    residual_kernel<<<dim3(grid_for(n), nb), 256, 0, s>>>(g, gs, r, rs, n);
+    residual_kernel<G><<<dim3(grid_for(n), nb), 256, 0, s>>>(g, gs, r, rs, n);
     check_hip(hipGetLastError(), "residual launch");
 }
+template void grid_residual<double>(const double*, long, double*, long, long, int, hipStream_t);
+template void grid_residual<float>(const float*, long, double*, long, long, int, hipStream_t);
 
 void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s) {
     if (n <= 0 || nb <= 0) return;
