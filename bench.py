@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["fp64", "mixed", "mixed_grid32"], default=None,
                     help="time one form only (profiling runs)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the accuracy phase (profiling runs): the mixed forms then use the "
+                         "plain FP32 forward and no accuracy is reported")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the transforms are "
@@ -168,12 +171,17 @@ def main():
         return float(((a.double() - ref).abs().amax(dim=dims) / ref.abs().amax(dim=dims)).max())
 
     # accuracy of the mixed forms against FP64 on the FP64 backward grids
-    p64.backward(dc, dg, N, batch=B, stream=stream)
-    c64 = torch.zeros_like(dc)
-    p64.forward(dg, c64, N, batch=B, stream=stream)
+    if not args.no_accuracy:
+        p64.backward(dc, dg, N, batch=B, stream=stream)
+        c64 = torch.zeros_like(dc)
+        p64.forward(dg, c64, N, batch=B, stream=stream)
     acc = {"fp64_roundtrip_max_rel_err": None}
     forms = {}
-    for form, gdtype in (("mixed", torch.float64), ("mixed_grid32", torch.float32)):
+    if args.no_accuracy:
+        forms = {"mixed": (torch.float64, p32, "fp32", True),
+                 "mixed_grid32": (torch.float32, p32, "fp32", True)}
+    for form, gdtype in (() if args.no_accuracy else
+                         (("mixed", torch.float64), ("mixed_grid32", torch.float32))):
         gin = dg.to(gdtype)
         c32 = torch.zeros_like(dc)
         p32.forward(gin, c32, N, batch=B, stream=stream)
@@ -189,7 +197,10 @@ def main():
         gate = min(e32, e32r) <= 1e-6
         forms[form] = (gdtype, fwd_plan, fwd_name, gate)
         del gin, c32, c32r, gout
-    acc["fp64_roundtrip_max_rel_err"] = relerr(c64, dc)
+    if not args.no_accuracy:
+        acc["fp64_roundtrip_max_rel_err"] = relerr(c64, dc)
+    else:
+        acc = None
     dg32 = torch.empty(dg.shape, dtype=torch.float32, device=dev)
 
     def timed(plan_bwd, plan_fwd, grid):

@@ -25,6 +25,9 @@
 // fetched into registers while the current chunk runs on the MFMA units.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "zgl_internal.hpp"
 
 namespace zgl {
@@ -412,8 +415,14 @@ template <> const float* rtab_of<float>(const OrderTables& t) { return t.rtab32;
 
 }  // namespace
 
+bool legendre_stream_enabled() {
+    const char* e = std::getenv("ZGL_LEGENDRE_ENGINE");
+    return !(e && std::strcmp(e, "tiled") == 0);
+}
+
 template <class T>
 void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s) {
+    if (legendre_stream_enabled() && zt_legendre_forward_stream<T>(a, s)) return;
     const Geom& g = a.t->g;
     ZtLegendreFwd<T> st{ptab_of<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
                         a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol};
@@ -422,6 +431,7 @@ void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s) {
 
 template <class T>
 void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s) {
+    if (legendre_stream_enabled() && zt_legendre_backward_stream<T>(a, s)) return;
     const Geom& g = a.t->g;
     ZtLegendreBwd<T> st{ptab_of<T>(*a.t), (const T*)a.flm, (T*)a.spec, a.spec_stride * 2,
                         a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol};

@@ -66,6 +66,7 @@ struct OrderTables {
     int4* work_lb = nullptr; int n_lb = 0;  // backward Legendre
     int4* work_rf = nullptr; int n_rf = 0;  // forward radial
     int4* work_rb = nullptr; int n_rb = 0;  // backward radial
+    int4* work_sb = nullptr; int n_sb = 0;  // backward Legendre, streaming kernel (Zernike)
 };
 
 // Rows per GEMM tile are 16 * NS; the work lists step by that many rows.
@@ -73,6 +74,9 @@ constexpr int kNsLegendreFwd = 4;
 constexpr int kNsLegendreBwdZt = 5;  // 80 rows: one tile for 65 half latitudes
 constexpr int kNsLegendreBwdSt = 4;
 constexpr int kNsRadial = 4;
+// Streaming backward Legendre: 16 * kNsStreamBwd rows of the nth/2 latitudes
+// off the equator per tile; the equator row (odd nth) is computed beside them.
+constexpr int kNsStreamBwd = 4;
 
 // Offsets of the parity-split P table: rows l = m + par + 2 l' < order.
 __host__ __device__ inline int plm_count(int order, int m, int par) {
@@ -138,6 +142,13 @@ template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s);
+// Streaming forms of the two Legendre stages (zgl_stream.hip): false when the
+// P block of a workgroup does not fit the LDS budget (large orders), in which
+// case nothing was launched.
+template <class T> bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s);
+template <class T> bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s);
+// ZGL_LEGENDRE_ENGINE=tiled selects the tiled LDS engine for every order.
+bool legendre_stream_enabled();
 
 struct StGemmArgs {
     const OrderTables* t;

@@ -1,0 +1,359 @@
+// Streaming MFMA kernels for the Zernike colatitude (Legendre) stages.
+//
+// Replaces the hot loops
+//   integrate_latitudinal  include/zernike_glq_transformer.hpp:778-817
+//   sum_l                  include/zernike_glq_transformer.hpp:901-941
+//
+// Each (m, parity) problem multiplies a small table block A (the P_lm values,
+// at most 16*NS rows by jh half latitudes) with a long, contiguous operand B
+// (the spectrum of one instance: jh rows of 2*ncol reals, or the flm rows of
+// one instance). The roles are fixed by HBM traffic: A is staged once per
+// workgroup in LDS, B and the output stream through registers only.
+//
+// Column permutation. An MFMA 16x16x4 takes one B value per lane, lane l
+// holding B[k = l/16][j = l%16]. Any permutation of the output columns is
+// allowed as long as the same permutation is used when storing, so a lane
+// loads VW consecutive columns with one 16-byte load,
+//   B[k][c0 + VW*j + v], v = 0..VW-1,
+// and feeds value v to MFMA "virtual block" v. One load instruction then
+// covers 4 rows x 256 contiguous bytes, and the results come back as VW
+// consecutive columns per lane, stored with one 16-byte store. Lanes past the
+// end of the column axis (a partial last tile) load zeros and store nothing.
+// The number of active 16-row strips is a template parameter of the tile
+// loop, dispatched once per workgroup, so the MFMA sequence is branch-free.
+//
+// Pipelining: each wave walks its tiles in chunks of KC k-steps, holding two
+// register buffers of B values; the loads of the next chunk are in flight
+// while the MFMAs of the current one run.
+#include <hip/hip_runtime.h>
+
+#include "zgl_internal.hpp"
+
+namespace zgl {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct SM;
+template <> struct SM<float> {
+    using V = f4;
+    using acc_t = f4;
+    static constexpr int VW = 4;
+    __device__ static acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // lane holds D[4(lane/16) + r][lane%16]
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <> struct SM<double> {
+    using V = d2;
+    using acc_t = d4;
+    static constexpr int VW = 2;
+    __device__ static acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // lane holds D[lane/16 + 4r][lane%16]
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+constexpr int kThreads = 256;
+constexpr int kKC = 6;  // k-steps of 4 per register chunk (K = 65 -> 3 chunks)
+
+// Legendre forward: flm[l,m][c] = sum_h P_lm(z_h) X^par_m[h][c], l = m+par+2i.
+template <class TT>
+struct LegFwd {
+    using T = TT;
+    const T* ptab; const T* spec; T* flm;
+    long spec_stride, flm_stride;  // elements of T
+    int order, nt, jh, ncol, G, as;
+    static constexpr bool kCenter = false;
+    __device__ int center_lds_row() const { return 16 * kNsLegendreFwd; }
+    struct Prob {
+        int M, K, N, m, par; const T* A; long a_si, a_sk; long boff;
+        bool center; const T* Ac; int hoff;
+    };
+    __device__ Prob problem(int prob, int m0) const {
+        Prob p;
+        p.m = prob >> 1; p.par = prob & 1;
+        p.M = plm_count(nt, p.m, p.par); p.K = jh; p.N = 2 * ncol;
+        p.center = false; p.Ac = nullptr; p.hoff = 0;
+        p.A = ptab + plm_offset(order, p.m, p.par, jh) + (long)m0 * jh;
+        p.a_si = jh; p.a_sk = 1;
+        p.boff = (long)(2 * p.m + p.par) * jh * 2 * ncol;
+        return p;
+    }
+    // operand rows as a per-instance base (wave-uniform) plus a 32-bit offset
+    __device__ const T* b_base(int inst) const { return spec + inst * spec_stride; }
+    __device__ unsigned b_off(const Prob& p, int k) const { return (unsigned)(p.boff + k * 2 * ncol); }
+    __device__ T* c_base(int inst) const { return flm + inst * flm_stride; }
+    __device__ unsigned c_off(const Prob& p, int i) const {
+        return (unsigned)(tri_idx(p.m + p.par + 2 * i, p.m) * 2 * ncol);
+    }
+};
+
+// Legendre backward: X^par_m[h][c] = sum_i P_lm(z_h) flm[l,m][c], l = m+par+2i.
+// Half latitude h pairs latitudes nth/2 + h and jh - 1 - h; with odd nth,
+// h = 0 is the equator, paired with itself. The MFMA rows are the nth/2 half
+// latitudes off the equator, h = hoff + r with hoff = nth & 1; the equator row
+// is computed beside them on the VALU by the first row block of parity 0.
+// Parity 1 has no equator term (the backward FFT reads only the even part
+// there).
+template <class TT>
+struct LegBwd {
+    using T = TT;
+    const T* ptab; const T* flm; T* spec;
+    long spec_stride, flm_stride;
+    int order, nt, jh, ncol, G, as, nth;
+    static constexpr bool kCenter = true;
+    __device__ int center_lds_row() const { return 16 * kNsStreamBwd; }
+    struct Prob {
+        int M, K, N, m, par; const T* A; long a_si, a_sk; long boff;
+        bool center; const T* Ac; int hoff;
+    };
+    __device__ Prob problem(int prob, int m0) const {
+        Prob p;
+        p.m = prob >> 1; p.par = prob & 1;
+        p.M = p.m < nt ? nth / 2 : 0;
+        p.K = plm_count(nt, p.m, p.par); p.N = 2 * ncol;
+        p.hoff = nth & 1;
+        const T* a0 = ptab + plm_offset(order, p.m, p.par, jh);  // [i][h]
+        p.A = a0 + p.hoff + m0;
+        p.a_si = 1; p.a_sk = jh;
+        p.boff = (long)(2 * p.m + p.par) * jh * 2 * ncol;
+        p.center = p.hoff && p.par == 0 && m0 == 0;
+        p.Ac = a0;  // the equator column h = 0
+        return p;
+    }
+    __device__ const T* b_base(int inst) const { return flm + inst * flm_stride; }
+    __device__ unsigned b_off(const Prob& p, int k) const {
+        return (unsigned)(tri_idx(p.m + p.par + 2 * k, p.m) * 2 * ncol);
+    }
+    __device__ T* c_base(int inst) const { return spec + inst * spec_stride; }
+    // output row r is half latitude h = r + hoff (r = -hoff: the equator)
+    __device__ unsigned c_off(const Prob& p, int r) const {
+        return (unsigned)(p.boff + (r + p.hoff) * 2 * ncol);
+    }
+};
+
+// The tile loop of one wave, for NSA active 16-row strips (a compile-time
+// count, so the MFMA sequence has no per-strip branches).
+template <int NSA, bool CENTER, class S>
+__device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob& P,
+                                             const typename S::T* As, int AS, int m0, int rows,
+                                             int g0, int gn, int wave, int lane) {
+    using T = typename S::T;
+    using M = SM<T>;
+    using V = typename M::V;
+    constexpr int VW = M::VW;
+    constexpr int TNW = 16 * VW;
+    const int j = lane & 15, g = lane >> 4;
+    const int K = P.K, steps = (K + 3) >> 2;
+    const int N = P.N;
+    const int ntile = (N + TNW - 1) / TNW;
+    const int tiles = gn * ntile;
+    const int ntw = tiles > wave ? (tiles - wave + 3) >> 2 : 0;  // tiles of this wave
+    const int nchunks = max(1, (steps + kKC - 1) / kKC);
+    const int U = ntw * nchunks;
+
+    typename M::acc_t acc[NSA][VW];
+#pragma unroll
+    for (int q = 0; q < NSA; ++q)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) acc[q][v] = typename M::acc_t{0, 0, 0, 0};
+    // equator row (CENTER): per-lane partial sums over this lane's k values;
+    // its A row is As row 16 * NS of the kernel
+    V cpart = {};
+    const T* Ac = As + (long)st.center_lds_row() * AS;
+
+    // unit u -> (instance, first column of this lane, chunk)
+    auto unit = [&](int u, int& inst, int& col, int& ch) {
+        const int tw = u / nchunks;
+        ch = u - tw * nchunks;
+        const int t = wave + 4 * tw;
+        const int gi = t / ntile;
+        inst = g0 + gi;
+        col = (t - gi * ntile) * TNW + VW * j;
+    };
+
+    auto load = [&](V (&buf)[kKC], int u) {
+        int inst, col, ch;
+        unit(u, inst, col, ch);
+        const bool colok = col < N;  // N is a multiple of 16, so VW columns are all in or all out
+        const T* bb = st.b_base(inst);
+#pragma unroll
+        for (int s = 0; s < kKC; ++s) {
+            const int k = 4 * (ch * kKC + s) + g;
+            V x = {};
+            if (colok && k < K) x = *reinterpret_cast<const V*>(bb + (st.b_off(P, k) + col));
+            buf[s] = x;
+        }
+    };
+
+    auto compute = [&](const V (&buf)[kKC], int u) {
+        int inst, col, ch;
+        unit(u, inst, col, ch);
+#pragma unroll
+        for (int s = 0; s < kKC; ++s) {
+            const int ks = ch * kKC + s;
+            if (ks < steps) {
+#pragma unroll
+                for (int q = 0; q < NSA; ++q) {
+                    const T a = As[(16 * q + j) * AS + 4 * ks + g];
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) acc[q][v] = M::mma(a, buf[s][v], acc[q][v]);
+                }
+                if constexpr (CENTER) {
+                    const T a = Ac[4 * ks + g];
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) cpart[v] += a * buf[s][v];
+                }
+            }
+        }
+        if (ch != nchunks - 1) return;
+        // store the finished tile and clear the accumulators
+        const bool colok = col < N;
+        T* cb = st.c_base(inst);
+#pragma unroll
+        for (int q = 0; q < NSA; ++q) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * q + M::row(lane, r);
+                if (colok && i < rows) {
+                    V o;
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) o[v] = acc[q][v][r];
+                    *reinterpret_cast<V*>(cb + (st.c_off(P, m0 + i) + col)) = o;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < VW; ++v) acc[q][v] = typename M::acc_t{0, 0, 0, 0};
+        }
+        if constexpr (CENTER) {
+            // sum the four lane groups (k = 4s + g) and store from group 0
+#pragma unroll
+            for (int v = 0; v < VW; ++v) {
+                T x = cpart[v];
+                x += __shfl_xor(x, 16);
+                x += __shfl_xor(x, 32);
+                cpart[v] = x;
+            }
+            if (colok && g == 0)
+                *reinterpret_cast<V*>(cb + (st.c_off(P, -P.hoff) + col)) = cpart;
+            cpart = V{};
+        }
+    };
+
+    V b0[kKC], b1[kKC];
+    if (U > 0) load(b0, 0);
+    for (int u = 0; u < U; u += 2) {
+        if (u + 1 < U) load(b1, u + 1);
+        compute(b0, u);
+        if (u + 1 >= U) break;
+        if (u + 2 < U) load(b0, u + 2);
+        compute(b1, u + 1);
+    }
+}
+
+template <class S, int NS>
+__global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb) {
+    using T = typename S::T;
+    constexpr int TM = 16 * NS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* As = reinterpret_cast<T*>(smem_raw);
+
+    const int4 wk = work[blockIdx.x];
+    const int m0 = wk.y;
+    const typename S::Prob P = st.problem(wk.x, m0);
+    const int rows = min(TM, P.M - m0);
+    if (rows <= 0) return;
+    const int g0 = blockIdx.y * st.G;
+    const int gn = min(st.G, nb - g0);
+    if (gn <= 0) return;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int K = P.K, KP = (K + 3) & ~3;
+    const int AS = st.as;
+
+    // A tile: TM rows x KP, zero padded, As[i * AS + k]; the equator row
+    // (P.center) goes to As row TM.
+    if (P.a_sk == 1) {
+        for (int e = tid; e < TM * KP; e += kThreads) {
+            const int i = e / KP, k = e - i * KP;
+            As[i * AS + k] = (i < rows && k < K) ? P.A[(long)i * P.a_si + k] : T(0);
+        }
+    } else {
+        for (int e = tid; e < TM * KP; e += kThreads) {
+            const int k = e / TM, i = e - k * TM;
+            As[i * AS + k] = (i < rows && k < K) ? P.A[(long)k * P.a_sk + i] : T(0);
+        }
+    }
+    const bool center = S::kCenter && P.center;
+    if (center)
+        for (int k = tid; k < KP; k += kThreads) As[TM * AS + k] = k < K ? P.Ac[(long)k * P.a_sk] : T(0);
+    __syncthreads();
+
+    const int nstrips = (rows + 15) >> 4;
+    static_assert(NS >= 1 && NS <= 4, "strip count");
+#define ZGL_TILES(n)                                                                          \
+    (center ? stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane)      \
+            : stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane))
+    switch (nstrips) {
+        case 1: ZGL_TILES(1); break;
+        case 2: if constexpr (NS >= 2) ZGL_TILES(2); break;
+        case 3: if constexpr (NS >= 3) ZGL_TILES(3); break;
+        default: if constexpr (NS >= 4) ZGL_TILES(4); break;
+    }
+#undef ZGL_TILES
+}
+
+constexpr size_t kMaxStreamLds = 64 * 1024;
+
+template <int NS, class S>
+bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, hipStream_t s) {
+    using T = typename S::T;
+    const int kp = (kmax + 3) & ~3;
+    st.as = kp + 1;  // odd stride: the 16 rows of an A fragment fall in distinct banks
+    const size_t lds = (size_t)(16 * NS + (S::kCenter ? 1 : 0)) * st.as * sizeof(T);
+    if (lds > kMaxStreamLds) return false;
+    if (nwork == 0 || nb <= 0) return true;
+    st.G = nb < 4 ? nb : 4;
+    dim3 grid(nwork, (nb + st.G - 1) / st.G);
+    stream_kernel<S, NS><<<grid, kThreads, lds, s>>>(st, work, nb);
+    check_hip(hipGetLastError(), "stream launch");
+    return true;
+}
+
+template <class T>
+const T* ptab_sel(const OrderTables& t);
+template <> const double* ptab_sel<double>(const OrderTables& t) { return t.ptab; }
+template <> const float* ptab_sel<float>(const OrderTables& t) { return t.ptab32; }
+
+}  // namespace
+
+template <class T>
+bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
+    const Geom& g = a.t->g;
+    LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
+                 a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0};
+    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, s);
+}
+
+template <class T>
+bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
+    const Geom& g = a.t->g;
+    LegBwd<T> st{ptab_sel<T>(*a.t), (const T*)a.flm, (T*)a.spec, a.spec_stride * 2,
+                 a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0, g.nth};
+    return launch_stream<kNsStreamBwd>(st, plm_count(g.order, 0, 0), a.t->work_sb, a.t->n_sb,
+                                       a.nb, s);
+}
+
+template bool zt_legendre_forward_stream<double>(const ZtGemmArgs&, hipStream_t);
+template bool zt_legendre_forward_stream<float>(const ZtGemmArgs&, hipStream_t);
+template bool zt_legendre_backward_stream<double>(const ZtGemmArgs&, hipStream_t);
+template bool zt_legendre_backward_stream<float>(const ZtGemmArgs&, hipStream_t);
+
+}  // namespace zgl

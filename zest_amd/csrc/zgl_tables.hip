@@ -392,6 +392,10 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
             for (int m0 = 0; m0 < cnt; m0 += tm_lf) lf.push_back(make_int4(prob, m0, 0, 0));
             for (int m0 = 0; m0 < g.jh; m0 += tm_lb) lb.push_back(make_int4(prob, m0, 0, 0));
         }
+    std::vector<int4> sb;
+    if (kind == ZGL_ZERNIKE)
+        for (int prob = 0; prob < 2 * order; ++prob)
+            for (int m0 = 0; m0 < g.nth / 2; m0 += 16 * kNsStreamBwd) sb.push_back(make_int4(prob, m0, 0, 0));
     if (kind == ZGL_ZERNIKE) {
         for (int l = 0; l < order; ++l) {
             const int ncount = (order - l + 1) / 2;
@@ -403,6 +407,7 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
     t.work_lb = upload_work(lb, s); t.n_lb = (int)lb.size();
     t.work_rf = upload_work(rf, s); t.n_rf = (int)rf.size();
     t.work_rb = upload_work(rb, s); t.n_rb = (int)rb.size();
+    t.work_sb = upload_work(sb, s); t.n_sb = (int)sb.size();
     check_hip(hipStreamSynchronize(s), "work lists");
     if (want_fp32) ensure_fp32_tables(t, s);
 }
@@ -410,7 +415,7 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
 void free_tables(OrderTables& t) {
     for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.rtab, (void*)t.ptab32,
                     (void*)t.rtab32, (void*)t.work_lf, (void*)t.work_lb, (void*)t.work_rf,
-                    (void*)t.work_rb})
+                    (void*)t.work_rb, (void*)t.work_sb})
         if (p) (void)hipFree(p);
     fft_plan_destroy(t.fft);
     t = OrderTables{};
