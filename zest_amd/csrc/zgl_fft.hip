@@ -256,7 +256,19 @@ __device__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d
 struct PairMap {
     int ncol, nth, jh, lat_stride, lon_stride;
     int tiles_per_h;  // Zernike: tiles per latitude pair (ncol > 1)
+    int ntiles;       // tiles per instance
 };
+
+// Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD
+// b % 8), each with its own L2. The grid has 8 * ceil(ntiles / 8) workgroups
+// per instance and workgroup b takes tile (b % 8) * q + b / 8, so every XCD
+// walks a contiguous range of tiles: the column tiles of one latitude pair,
+// which share cache lines at their edges, meet in the same L2.
+__device__ inline int xcd_tile(const PairMap& pm) {
+    const int q = (pm.ntiles + 7) >> 3;
+    const int b = blockIdx.x;
+    return (b & 7) * q + (b >> 3);
+}
 
 // (h, first column) of a tile: Zernike tiles run over the radial nodes of one
 // latitude pair; SH tiles (ncol = 1) run over T consecutive latitude pairs.
@@ -299,12 +311,14 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     long spec_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
+    const int tile = xcd_tile(pm);
+    if (tile >= pm.ntiles) return;
     C *data, *twl, *twp;
     unsigned short* perm;
     stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
     int h0, c0;
-    tile_origin<T>(pm, blockIdx.x, h0, c0);
+    tile_origin<T>(pm, tile, h0, c0);
     const G* g = grid + (long)blockIdx.y * grid_stride;
     C* sp = spec + (long)blockIdx.y * spec_stride;
 
@@ -398,12 +412,14 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     const typename Cplx<R>::type* __restrict__ spec, long spec_stride, G* __restrict__ grid,
     long grid_stride, PairMap pm, int nm) {
     using C = typename Cplx<R>::type;
+    const int tile = xcd_tile(pm);
+    if (tile >= pm.ntiles) return;
     C *data, *twl, *twp;
     unsigned short* perm;
     stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
     int h0, c0;
-    tile_origin<T>(pm, blockIdx.x, h0, c0);
+    tile_origin<T>(pm, tile, h0, c0);
     G* g = grid + (long)blockIdx.y * grid_stride;
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
@@ -503,12 +519,12 @@ PairMap pair_map(const Geom& g, int T) {
     pm.lat_stride = g.lat_stride;
     pm.lon_stride = g.lon_stride;
     pm.tiles_per_h = (g.ncol + T - 1) / T;
+    pm.ntiles = pm.ncol > 1 ? pm.jh * pm.tiles_per_h : (pm.jh + T - 1) / T;
     return pm;
 }
 
-int num_tiles(const PairMap& pm, int T) {
-    return pm.ncol > 1 ? pm.jh * pm.tiles_per_h : (pm.jh + T - 1) / T;
-}
+// Workgroups per instance: a multiple of 8 (see xcd_tile).
+int grid_tiles(const PairMap& pm) { return (pm.ntiles + 7) / 8 * 8; }
 
 template <class R> const typename Cplx<R>::type* tw_of(const FftPlan& f);
 template <> const double2* tw_of<double>(const FftPlan& f) { return f.tw; }
@@ -524,7 +540,7 @@ void launch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long
     const FftPlan& f = L.t->fft;
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
-    dim3 grd(num_tiles(pm, T), nb);
+    dim3 grd(grid_tiles(pm), nb);
     auto k = L.layout_kind == GRID_COLS ? fft_forward_kernel<T, GRID_COLS, R, G>
                                         : fft_forward_kernel<T, GRID_ROWS, R, G>;
     if (lds > 65536)
@@ -542,7 +558,7 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, lon
     const FftPlan& f = L.t->fft;
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
-    dim3 grd(num_tiles(pm, T), nb);
+    dim3 grd(grid_tiles(pm), nb);
     auto k = L.layout_kind == GRID_COLS ? fft_backward_kernel<T, GRID_COLS, R, G>
                                         : fft_backward_kernel<T, GRID_ROWS, R, G>;
     if (lds > 65536)

@@ -440,6 +440,7 @@ void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s) {
 
 template <class T>
 void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s) {
+    if (legendre_stream_enabled() && zt_radial_forward_stream<T>(a, s)) return;
     const Geom& g = a.t->g;
     ZtRadialFwd<T> st{rtab_of<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2, a.coeff_stride,
                       a.nt, g.ncol, a.zernike_unnormed};
@@ -448,6 +449,7 @@ void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s) {
 
 template <class T>
 void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s) {
+    if (legendre_stream_enabled() && zt_radial_backward_stream<T>(a, s)) return;
     const Geom& g = a.t->g;
     ZtRadialBwd<T> st{rtab_of<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2, a.coeff_stride, a.nt,
                       g.ncol};

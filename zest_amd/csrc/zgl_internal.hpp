@@ -142,12 +142,15 @@ template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s);
-// Streaming forms of the two Legendre stages (zgl_stream.hip): false when the
-// P block of a workgroup does not fit the LDS budget (large orders), in which
-// case nothing was launched.
+// Streaming forms of the Legendre and radial stages (zgl_stream.hip): false
+// when the table block of a workgroup does not fit the LDS budget (large
+// orders), in which case nothing was launched.
 template <class T> bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s);
 template <class T> bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s);
-// ZGL_LEGENDRE_ENGINE=tiled selects the tiled LDS engine for every order.
+template <class T> bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s);
+template <class T> bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s);
+// ZGL_LEGENDRE_ENGINE=tiled selects the tiled LDS engine for every order and
+// stage.
 bool legendre_stream_enabled();
 
 struct StGemmArgs {

@@ -310,7 +310,17 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
 #undef ZGL_TILES
 }
 
-constexpr size_t kMaxStreamLds = 64 * 1024;
+// Two workgroups of 80 KB fit one CU's 160 KB of LDS.
+constexpr size_t kMaxStreamLds = 80 * 1024;
+
+// Dynamic LDS above 64 KB must be allowed per kernel.
+template <class F>
+void allow_lds(F* kernel, size_t lds) {
+    if (lds > 64 * 1024)
+        check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                  "LDS attribute");
+}
 
 template <int NS, class S>
 bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, hipStream_t s) {
@@ -322,9 +332,256 @@ bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, hipStrea
     if (nwork == 0 || nb <= 0) return true;
     st.G = nb < 4 ? nb : 4;
     dim3 grid(nwork, (nb + st.G - 1) / st.G);
+    allow_lds(&stream_kernel<S, NS>, lds);
     stream_kernel<S, NS><<<grid, kThreads, lds, s>>>(st, work, nb);
     check_hip(hipGetLastError(), "stream launch");
     return true;
+}
+
+// ------------------------------------------------------------------ radial
+// Replaces integrate_radial (include/zernike_glq_transformer.hpp:819-865) and
+// sum_n (include/zernike_glq_transformer.hpp:867-899).
+//
+// Forward: c[n,l,m] = kappa_n sum_k R_nl(r_k) flm[l,m][k], n = l + 2i.
+// A workgroup owns one l, one block of up to 16*NS rows i and G instances; the
+// R block sits in LDS and the flm block of each instance (rows m = 0..l,
+// contiguous) streams through registers, 16 m per tile. The contraction axis
+// k is contiguous in both operands, so in MFMA step t (t < KV) of an
+// iteration lane group g takes k = k0 + KV*g + t: one 16-byte load of
+// flm[l,m][k0 + KV*g ..] holds the (re, im) pairs of its KV k values, and the
+// A operand reads the same k from LDS.
+template <class T> struct RadCfg;
+template <> struct RadCfg<float> { static constexpr int KV = 2; };   // float4 = 2 complex
+template <> struct RadCfg<double> { static constexpr int KV = 1; };  // double2 = 1 complex
+
+template <class T>
+struct RadFwdArgs {
+    const T* rtab; const T* flm; double* coeffs;
+    long flm_stride, coeff_stride;  // elements of T, doubles
+    int nt, nr, unnormed, G, as;
+};
+
+template <int NSA, class T>
+__device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* As, int l, int i0,
+                                              int rows, int g0, int gn, int wave, int lane) {
+    using M = SM<T>;
+    using V = typename M::V;
+    constexpr int KV = RadCfg<T>::KV, KI = 4 * KV;
+    const int j = lane & 15, g = lane >> 4;
+    const int AS = a.as;
+    const int iters = (a.nr + KI - 1) / KI;
+    const int nmb = (l + 16) >> 4;  // blocks of 16 m in [0, l]
+    const int tiles = gn * nmb;
+    const int ntw = tiles > wave ? (tiles - wave + 3) >> 2 : 0;
+    const int nchunks = (iters + kKC - 1) / kKC;
+    const int U = ntw * nchunks;
+    const long rowlen = 2L * a.nr;
+
+    typename M::acc_t are[NSA], aim[NSA];
+#pragma unroll
+    for (int q = 0; q < NSA; ++q) are[q] = aim[q] = typename M::acc_t{0, 0, 0, 0};
+
+    auto unit = [&](int u, int& inst, int& m, int& ch) {
+        const int tw = u / nchunks;
+        ch = u - tw * nchunks;
+        const int t = wave + 4 * tw;
+        const int gi = t / nmb;
+        inst = g0 + gi;
+        m = 16 * (t - gi * nmb) + j;
+    };
+    auto load = [&](V (&buf)[kKC], int u) {
+        int inst, m, ch;
+        unit(u, inst, m, ch);
+        const T* row = a.flm + inst * a.flm_stride + tri_idx(l, m) * rowlen;
+#pragma unroll
+        for (int s = 0; s < kKC; ++s) {
+            const int k = (ch * kKC + s) * KI + KV * g;
+            V x = {};
+            if (m <= l && k < a.nr) x = *reinterpret_cast<const V*>(row + 2 * k);
+            buf[s] = x;
+        }
+    };
+    auto compute = [&](const V (&buf)[kKC], int u) {
+        int inst, m, ch;
+        unit(u, inst, m, ch);
+#pragma unroll
+        for (int s = 0; s < kKC; ++s) {
+            const int it = ch * kKC + s;
+            if (it < iters) {
+#pragma unroll
+                for (int t = 0; t < KV; ++t) {
+#pragma unroll
+                    for (int q = 0; q < NSA; ++q) {
+                        const T x = As[(16 * q + j) * AS + it * KI + KV * g + t];
+                        are[q] = M::mma(x, buf[s][2 * t], are[q]);
+                        aim[q] = M::mma(x, buf[s][2 * t + 1], aim[q]);
+                    }
+                }
+            }
+        }
+        if (ch != nchunks - 1) return;
+        double* cb = a.coeffs + inst * a.coeff_stride;
+#pragma unroll
+        for (int q = 0; q < NSA; ++q) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * q + M::row(lane, r);
+                if (i < rows && m <= l) {
+                    const int n = l + 2 * (i0 + i);
+                    const double kappa = a.unnormed ? (double)(2 * n + 3) : 1.0;
+                    double* o = cb + tet_idx(n, l, m) * 2;
+                    o[0] = (double)are[q][r] * kappa;
+                    o[1] = (double)aim[q][r] * kappa;
+                }
+            }
+            are[q] = aim[q] = typename M::acc_t{0, 0, 0, 0};
+        }
+    };
+
+    V b0[kKC], b1[kKC];
+    if (U > 0) load(b0, 0);
+    for (int u = 0; u < U; u += 2) {
+        if (u + 1 < U) load(b1, u + 1);
+        compute(b0, u);
+        if (u + 1 >= U) break;
+        if (u + 2 < U) load(b0, u + 2);
+        compute(b1, u + 1);
+    }
+}
+
+// FP64 operands need more registers than three waves per SIMD allow.
+template <class T> constexpr int kRadWaves = sizeof(T) == 8 ? 2 : 3;
+
+template <class T, int NS>
+__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdArgs<T> a, const int4* __restrict__ work,
+                                                             int nb) {
+    constexpr int TM = 16 * NS;
+    constexpr int KI = 4 * RadCfg<T>::KV;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* As = reinterpret_cast<T*>(smem_raw);
+    const int4 wk = work[blockIdx.x];
+    const int l = wk.x, i0 = wk.y;
+    const int M = l < a.nt ? (a.nt - l + 1) / 2 : 0;
+    const int rows = min(TM, M - i0);
+    if (rows <= 0) return;
+    const int g0 = blockIdx.y * a.G;
+    const int gn = min(a.G, nb - g0);
+    if (gn <= 0) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int KP = (a.nr + KI - 1) / KI * KI;
+    for (int e = tid; e < TM * KP; e += kThreads) {
+        const int i = e / KP, k = e - i * KP;
+        As[i * a.as + k] = (i < rows && k < a.nr) ? a.rtab[rad_idx(l + 2 * (i0 + i), l) * a.nr + k] : T(0);
+    }
+    __syncthreads();
+    switch ((rows + 15) >> 4) {
+        case 1: rad_fwd_tiles<1>(a, As, l, i0, rows, g0, gn, wave, lane); break;
+        case 2: if constexpr (NS >= 2) rad_fwd_tiles<2>(a, As, l, i0, rows, g0, gn, wave, lane); break;
+        case 3: if constexpr (NS >= 3) rad_fwd_tiles<3>(a, As, l, i0, rows, g0, gn, wave, lane); break;
+        default: if constexpr (NS >= 4) rad_fwd_tiles<4>(a, As, l, i0, rows, g0, gn, wave, lane); break;
+    }
+}
+
+// Backward: flm[l,m][k] = sum_i R_nl(r_k) c[n,l,m], n = l + 2i.
+// A workgroup owns one l and G instances; the R block (rows i, contiguous k)
+// sits in LDS. A wave's tile is one instance and 16 values of m: it holds the
+// (re, im) coefficients of its 16 m for every i in registers (the A operand)
+// and sweeps the k axis in tiles of 16*VW columns with the column permutation
+// of the Legendre kernels, storing VW interleaved (re, im) pairs per lane.
+constexpr int kRadBwdSteps = 16;  // i < 64
+
+template <class T>
+struct RadBwdArgs {
+    const T* rtab; const double* coeffs; T* flm;
+    long flm_stride, coeff_stride;
+    int nt, nr, G, rs;
+};
+
+template <class T>
+__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdArgs<T> a, int nb) {
+    using M = SM<T>;
+    using V = typename M::V;
+    constexpr int VW = M::VW, TNW = 16 * VW;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* Rs = reinterpret_cast<T*>(smem_raw);
+    const int l = blockIdx.x;
+    if (l >= a.nt) return;
+    const int K = (a.nt - l + 1) / 2;
+    const int steps = (K + 3) >> 2;
+    const int g0 = blockIdx.y * a.G;
+    const int gn = min(a.G, nb - g0);
+    if (gn <= 0) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int j = lane & 15, g = lane >> 4;
+    const int RS = a.rs;
+    // Rs[i * RS + k], RS = nr; columns of the last (partial) tile past nr read
+    // the next row or the zero tail, and their results are not stored
+    for (int e = tid; e < 4 * steps * RS + TNW; e += kThreads) {
+        const int i = e / RS, k = e - i * RS;
+        Rs[e] = (i < K && k < a.nr) ? a.rtab[rad_idx(l + 2 * i, l) * a.nr + k] : T(0);
+    }
+    __syncthreads();
+
+    const int nmb = (l + 16) >> 4;
+    const int tiles = gn * nmb;
+    const int ntile = (a.nr + TNW - 1) / TNW;
+    const long rowlen = 2L * a.nr;
+    for (int t = wave; t < tiles; t += 4) {
+        const int gi = t / nmb;
+        const int inst = g0 + gi;
+        const int mb = 16 * (t - gi * nmb);
+        // A operand: lane (row j -> m = mb + j, group g -> i = 4s + g)
+        const double* cb = a.coeffs + inst * a.coeff_stride;
+        T xre[kRadBwdSteps], xim[kRadBwdSteps];
+#pragma unroll
+        for (int s = 0; s < kRadBwdSteps; ++s) {
+            const int i = 4 * s + g, m = mb + j;
+            xre[s] = xim[s] = T(0);
+            if (s < steps && i < K && m <= l) {
+                const double* c = cb + tet_idx(l + 2 * i, l, m) * 2;
+                xre[s] = (T)c[0];
+                xim[s] = (T)c[1];
+            }
+        }
+        T* ob = a.flm + inst * a.flm_stride;
+        for (int ct = 0; ct < ntile; ++ct) {
+            typename M::acc_t are[VW], aim[VW];
+#pragma unroll
+            for (int v = 0; v < VW; ++v) are[v] = aim[v] = typename M::acc_t{0, 0, 0, 0};
+            const int col = ct * TNW + VW * j;
+#pragma unroll
+            for (int s = 0; s < kRadBwdSteps; ++s) {
+                if (s < steps) {
+                    const V b = *reinterpret_cast<const V*>(Rs + (4 * s + g) * RS + col);
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) {
+                        are[v] = M::mma(xre[s], b[v], are[v]);
+                        aim[v] = M::mma(xim[s], b[v], aim[v]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = mb + M::row(lane, r);
+                if (m <= l && col < a.nr) {
+                    // 2*VW reals: (re, im) of columns col .. col+VW-1, two 16-byte stores
+                    T* o = ob + tri_idx(l, m) * rowlen + 2 * col;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        V w;
+#pragma unroll
+                        for (int e = 0; e < VW; ++e) {
+                            const int v = (VW * h + e) >> 1;
+                            w[e] = ((VW * h + e) & 1) ? aim[v][r] : are[v][r];
+                        }
+                        *reinterpret_cast<V*>(o + VW * h) = w;
+                    }
+                }
+            }
+        }
+    }
 }
 
 template <class T>
@@ -333,6 +590,50 @@ template <> const double* ptab_sel<double>(const OrderTables& t) { return t.ptab
 template <> const float* ptab_sel<float>(const OrderTables& t) { return t.ptab32; }
 
 }  // namespace
+
+template <class T>
+const T* rtab_sel(const OrderTables& t);
+template <> const double* rtab_sel<double>(const OrderTables& t) { return t.rtab; }
+template <> const float* rtab_sel<float>(const OrderTables& t) { return t.rtab32; }
+
+template <class T>
+bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
+    const Geom& g = a.t->g;
+    constexpr int KI = 4 * RadCfg<T>::KV;
+    const int kp = (g.ncol + KI - 1) / KI * KI;
+    RadFwdArgs<T> r{rtab_sel<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2, a.coeff_stride,
+                    a.nt, g.ncol, a.zernike_unnormed, 1, kp + 2};
+    const size_t lds = (size_t)16 * kNsRadial * r.as * sizeof(T);
+    if (lds > kMaxStreamLds) return false;
+    if (a.t->n_rf == 0 || a.nb <= 0) return true;
+    r.G = a.nb < 4 ? a.nb : 4;
+    dim3 grid(a.t->n_rf, (a.nb + r.G - 1) / r.G);
+    allow_lds(&rad_fwd_kernel<T, kNsRadial>, lds);
+    rad_fwd_kernel<T, kNsRadial><<<grid, kThreads, lds, s>>>(r, a.t->work_rf, a.nb);
+    check_hip(hipGetLastError(), "radial forward launch");
+    return true;
+}
+
+template <class T>
+bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
+    const Geom& g = a.t->g;
+    constexpr int TNW = 16 * SM<T>::VW;
+    const int kmax = (g.order + 1) / 2;  // i values of l = 0
+    if (kmax > 4 * kRadBwdSteps) return false;
+    // rows of ncol reals, plus one column tile of zeros so that the last
+    // tile of the last row reads inside the array
+    const int rs = g.ncol;
+    const size_t lds = ((size_t)4 * ((kmax + 3) / 4) * rs + TNW) * sizeof(T);
+    if (lds > kMaxStreamLds) return false;
+    if (a.nt == 0 || a.nb <= 0) return true;
+    RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2, a.coeff_stride,
+                    a.nt, g.ncol, a.nb < 4 ? a.nb : 4, rs};
+    dim3 grid(a.nt, (a.nb + r.G - 1) / r.G);
+    allow_lds(&rad_bwd_kernel<T>, lds);
+    rad_bwd_kernel<T><<<grid, kThreads, lds, s>>>(r, a.nb);
+    check_hip(hipGetLastError(), "radial backward launch");
+    return true;
+}
 
 template <class T>
 bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
@@ -355,5 +656,9 @@ template bool zt_legendre_forward_stream<double>(const ZtGemmArgs&, hipStream_t)
 template bool zt_legendre_forward_stream<float>(const ZtGemmArgs&, hipStream_t);
 template bool zt_legendre_backward_stream<double>(const ZtGemmArgs&, hipStream_t);
 template bool zt_legendre_backward_stream<float>(const ZtGemmArgs&, hipStream_t);
+template bool zt_radial_forward_stream<double>(const ZtGemmArgs&, hipStream_t);
+template bool zt_radial_forward_stream<float>(const ZtGemmArgs&, hipStream_t);
+template bool zt_radial_backward_stream<double>(const ZtGemmArgs&, hipStream_t);
+template bool zt_radial_backward_stream<float>(const ZtGemmArgs&, hipStream_t);
 
 }  // namespace zgl
