@@ -133,9 +133,10 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
     const size_t per = (size_t)(c.spec_stride + c.flm_stride) * csize;
     size_t budget, cap;
     if (p->kind == ZGL_ZERNIKE) {
-        // Large chunks keep every CU busy through each stage; measured faster
-        // than chunks sized to stay in the 256 MB Infinity Cache.
-        budget = env_mb("ZGL_SCRATCH_MB", 1024) << 20;
+        // Large chunks keep every CU busy through each stage and cut launches;
+        // measured faster than chunks sized to stay in the 256 MB Infinity
+        // Cache (order 128: 64 instances of 27 MB FP32 / 54 MB FP64 scratch).
+        budget = env_mb("ZGL_SCRATCH_MB", 4096) << 20;
         cap = 64;
     } else {
         // Instances form the GEMM column axis: larger chunks reuse the P table more.
