@@ -129,6 +129,31 @@ zgl_status zgl_backward(zgl_plan *plan, const double *coeffs, size_t exp_order,
                         ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
                         zgl_mem mem, void *stream);
 
+/* Transposes of the two transforms, for gradients. No counterpart in zest;
+ * they are the adjoints of the members replaced above
+ * (include/zernike_glq_transformer.hpp:640-679,
+ *  include/sh_glq_transformer.hpp:632-668) under the plain Euclidean inner
+ * products on grids and on {cos, sin} coefficient arrays.
+ *
+ * zgl_backward_adjoint: coefficients = B^T grid, the transpose of the
+ *   backward transform B (sum over grid points of the grid value times each
+ *   basis function, without quadrature weights). Arguments as zgl_forward.
+ * zgl_forward_adjoint:  grid = F^T coefficients, the transpose of the forward
+ *   transform F. Since F = diag(nu) B^T W (W the quadrature weights of the
+ *   grid points, nu = prefactor * kappa_n), F^T c = W * B(nu * c). Arguments
+ *   as zgl_backward.
+ * The sin part at m = 0 is ignored on input and written as zero on output,
+ * as by the transforms themselves. Arithmetic follows the plan (FP64, or FP32
+ * for both FP32 modes: refinement does not apply); grids are FP64. */
+zgl_status zgl_backward_adjoint(zgl_plan *plan, const double *grid, size_t grid_order,
+                                double *coeffs, size_t exp_order, size_t batch,
+                                ptrdiff_t grid_stride, ptrdiff_t coeff_stride,
+                                zgl_mem mem, void *stream);
+zgl_status zgl_forward_adjoint(zgl_plan *plan, const double *coeffs, size_t exp_order,
+                               double *grid, size_t grid_order, size_t batch,
+                               ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
+                               zgl_mem mem, void *stream);
+
 /* zgl_forward / zgl_backward with the grids stored in FP32 (same layouts,
  * floats instead of doubles; strides in floats). Only for plans of precision
  * ZGL_FP32 or ZGL_FP32_REFINE (ZGL_ERROR_NOT_SUPPORTED otherwise); the

@@ -65,6 +65,10 @@ def _load():
     lib.zgl_forward.restype = i32
     lib.zgl_backward.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_backward.restype = i32
+    lib.zgl_backward_adjoint.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_backward_adjoint.restype = i32
+    lib.zgl_forward_adjoint.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_forward_adjoint.restype = i32
     lib.zgl_forward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_forward_grid32.restype = i32
     lib.zgl_backward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
@@ -90,7 +94,7 @@ LIB = _load()
 EXPORTS = (
     "zgl_plan_create", "zgl_plan_destroy", "zgl_plan_resize", "zgl_plan_order",
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
-    "zgl_forward_grid32", "zgl_backward_grid32",
+    "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -214,6 +218,31 @@ class Plan:
         fn = LIB.zgl_backward_grid32 if g32 else LIB.zgl_backward
         _check(fn(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                   grid_stride, gm, _stream_ptr(stream, gm)))
+
+
+    def backward_adjoint(self, grid, coeffs, grid_order: int, exp_order: int | None = None,
+                         batch: int = 1, grid_stride: int = 0, coeff_stride: int = 0, stream=None):
+        """coeffs = B^T grid, the transpose of the backward transform
+        (zgl_backward_adjoint); arguments as forward()."""
+        exp_order = grid_order if exp_order is None else exp_order
+        gp, gm, _ = _ptr_and_mem(grid)
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        if gm != cm:
+            raise ValueError("grid and coefficients must live in the same memory")
+        _check(LIB.zgl_backward_adjoint(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
+                                        coeff_stride, gm, _stream_ptr(stream, gm)))
+
+    def forward_adjoint(self, coeffs, grid, grid_order: int, exp_order: int | None = None,
+                        batch: int = 1, coeff_stride: int = 0, grid_stride: int = 0, stream=None):
+        """grid = F^T coeffs, the transpose of the forward transform
+        (zgl_forward_adjoint); arguments as backward()."""
+        exp_order = grid_order if exp_order is None else exp_order
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        gp, gm, _ = _ptr_and_mem(grid)
+        if gm != cm:
+            raise ValueError("grid and coefficients must live in the same memory")
+        _check(LIB.zgl_forward_adjoint(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
+                                       grid_stride, gm, _stream_ptr(stream, gm)))
 
 
 def _stream_ptr(stream, mem):

@@ -180,10 +180,21 @@ void timed(zgl_plan* p, Stage st, int nb, hipStream_t s, F&& launch) {
 
 double shnorm_value(int sh_norm) { return sh_norm == ZGL_SHNORM_QM ? 1.0 : 1.0 / (4.0 * M_PI); }
 
+// The forward transform's prefactor on the FFT: shnorm * 2 pi / Nphi, halved
+// for Zernike (include/zernike_glq_transformer.hpp:710-720,
+// include/sh_glq_transformer.hpp:757-765).
+double forward_prefactor(const zgl_plan* p, const Geom& g) {
+    const double twopi_n = 2.0 * M_PI / (double)g.nphi;
+    return p->kind == ZGL_ZERNIKE ? shnorm_value(p->shnorm) * 0.5 * twopi_n
+                                  : shnorm_value(p->shnorm) * twopi_n;
+}
+
 // T: arithmetic (double or float); G: grid element type in memory.
+// adjoint: the transpose of the backward transform (no quadrature weights,
+// no prefactor, no kappa_n), otherwise the forward transform.
 template <class T, class G>
 void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt,
-                 size_t batch, long gs, long cs, hipStream_t s) {
+                 size_t batch, long gs, long cs, hipStream_t s, bool adjoint = false) {
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
     const Geom& g = t.g;
@@ -195,15 +206,15 @@ void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt
     L.t = &t;
     L.layout_kind = (p->kind == ZGL_SH && p->layout == ZGL_LAYOUT_LATLON) ? GRID_ROWS : GRID_COLS;
     L.nm_used = (int)nt;
-    const double twopi_n = 2.0 * M_PI / (double)g.nphi;
-    L.scale = p->kind == ZGL_ZERNIKE ? shnorm_value(p->shnorm) * 0.5 * twopi_n
-                                     : shnorm_value(p->shnorm) * twopi_n;
-    L.center_factor = p->kind == ZGL_ZERNIKE ? 1.0 : 2.0;
+    L.weighted = !adjoint;
+    L.scale = adjoint ? 1.0 : forward_prefactor(p, g);
+    L.center_factor = (adjoint || p->kind == ZGL_ZERNIKE) ? 1.0 : 2.0;
+    const int kappa = !adjoint && p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0;
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
         timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T, G>(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
         if (p->kind == ZGL_ZERNIKE) {
-            ZtGemmArgs a{&t, (int)nt, p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0, p->spec, ch.spec_stride,
+            ZtGemmArgs a{&t, (int)nt, kappa, p->spec, ch.spec_stride,
                          p->flm, ch.flm_stride, coeffs + b0 * cs, cs, nb};
             timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
             timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward<T>(a, s); });
@@ -214,9 +225,13 @@ void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt
     }
 }
 
+// adjoint: the transpose of the forward transform, W * B(prefactor * kappa_n * c)
+// (the prefactor and the quadrature weights W applied to the output grid,
+// kappa_n to a scaled copy of the coefficients), otherwise the backward
+// transform.
 template <class T, class G>
 void run_backward(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t N,
-                  size_t batch, long cs, long gs, hipStream_t s) {
+                  size_t batch, long cs, long gs, hipStream_t s, bool adjoint = false) {
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
     const Geom& g = t.g;
@@ -228,13 +243,24 @@ void run_backward(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t 
     L.t = &t;
     L.layout_kind = (p->kind == ZGL_SH && p->layout == ZGL_LAYOUT_LATLON) ? GRID_ROWS : GRID_COLS;
     L.nm_used = (int)nt;
-    L.scale = 1.0;
-    L.center_factor = 1.0;
+    L.weighted = adjoint;
+    L.scale = adjoint ? forward_prefactor(p, g) : 1.0;
+    L.center_factor = (adjoint && p->kind == ZGL_SH) ? 2.0 : 1.0;
+    const bool kappa = adjoint && p->kind == ZGL_ZERNIKE && p->znorm == ZGL_ZNORM_UNNORMED;
+    const long ncw = 2 * (long)coeff_size(p, nt);
+    if (kappa) ensure(p->rcoef, p->rcoef_cap, (size_t)ncw * ch.nb);
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
         if (p->kind == ZGL_ZERNIKE) {
+            const double* cin = coeffs + b0 * cs;
+            long cstride = cs;
+            if (kappa) {
+                coeff_kappa_scale(cin, cs, p->rcoef, ncw, (int)nt, 1, nb, s);
+                cin = p->rcoef;
+                cstride = ncw;
+            }
             ZtGemmArgs a{&t, (int)nt, 0, p->spec, ch.spec_stride, p->flm, ch.flm_stride,
-                         const_cast<double*>(coeffs) + b0 * cs, cs, nb};
+                         const_cast<double*>(cin), cstride, nb};
             timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward<T>(a, s); });
             timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward<T>(a, s); });
         } else {
@@ -285,6 +311,20 @@ void backward_any(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t 
     } else {
         run_backward<float, G>(p, coeffs, nt, grid, N, batch, cs, gs, s);
     }
+}
+
+// The two transposes, in the plan's arithmetic (FP64, or FP32 for both FP32
+// modes: refinement does not apply to a transpose).
+void backward_adjoint_any(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
+                          size_t batch, long gs, long cs, hipStream_t s) {
+    if (p->precision == ZGL_FP64) run_forward<double, double>(p, grid, N, coeffs, nt, batch, gs, cs, s, true);
+    else run_forward<float, double>(p, grid, N, coeffs, nt, batch, gs, cs, s, true);
+}
+
+void forward_adjoint_any(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
+                         size_t batch, long cs, long gs, hipStream_t s) {
+    if (p->precision == ZGL_FP64) run_backward<double, double>(p, coeffs, nt, grid, N, batch, cs, gs, s, true);
+    else run_backward<float, double>(p, coeffs, nt, grid, N, batch, cs, gs, s, true);
 }
 
 template <class F>
@@ -386,10 +426,12 @@ size_t zgl_coeff_size(const zgl_plan* plan, size_t order) { return plan ? coeff_
 namespace zgl {
 namespace {
 
+// grid -> coefficients: the forward transform, or (adjoint) the transpose of
+// the backward transform
 template <class G>
 zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, double* coeffs,
                          size_t exp_order, size_t batch, ptrdiff_t grid_stride,
-                         ptrdiff_t coeff_stride, zgl_mem mem, void* stream) {
+                         ptrdiff_t coeff_stride, zgl_mem mem, void* stream, bool adjoint = false) {
     if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
     if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
     if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
@@ -409,7 +451,12 @@ zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, doubl
         if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            forward_any<G>(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+            if constexpr (std::is_same_v<G, double>) {
+                if (adjoint) backward_adjoint_any(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+                else forward_any<G>(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+            } else {
+                forward_any<G>(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+            }
             check_hip(hipGetLastError(), "forward");
             return ZGL_SUCCESS;
         }
@@ -420,7 +467,12 @@ zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, doubl
         check_hip(hipMemcpy2DAsync(dg, gsz * sizeof(G), grid, gs * sizeof(G), gsz * sizeof(G), batch,
                                    hipMemcpyHostToDevice, s),
                   "grid upload");
-        forward_any<G>(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+        if constexpr (std::is_same_v<G, double>) {
+            if (adjoint) backward_adjoint_any(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+            else forward_any<G>(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+        } else {
+            forward_any<G>(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+        }
         check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, ncw * sizeof(double),
                                    ncw * sizeof(double), batch, hipMemcpyDeviceToHost, s),
                   "coefficient download");
@@ -429,10 +481,12 @@ zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, doubl
     });
 }
 
+// coefficients -> grid: the backward transform, or (adjoint) the transpose of
+// the forward transform
 template <class G>
 zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order, G* grid,
                           size_t grid_order, size_t batch, ptrdiff_t coeff_stride,
-                          ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
+                          ptrdiff_t grid_stride, zgl_mem mem, void* stream, bool adjoint = false) {
     if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
     if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
     if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
@@ -451,7 +505,12 @@ zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order
         if (!grid || (!coeffs && nt > 0)) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            backward_any<G>(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+            if constexpr (std::is_same_v<G, double>) {
+                if (adjoint) forward_adjoint_any(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+                else backward_any<G>(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+            } else {
+                backward_any<G>(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+            }
             check_hip(hipGetLastError(), "backward");
             return ZGL_SUCCESS;
         }
@@ -463,7 +522,12 @@ zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order
             check_hip(hipMemcpy2DAsync(plan->dcoef, ncr * sizeof(double), coeffs, cs * sizeof(double),
                                        ncr * sizeof(double), batch, hipMemcpyHostToDevice, s),
                       "coefficient upload");
-        backward_any<G>(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+        if constexpr (std::is_same_v<G, double>) {
+            if (adjoint) forward_adjoint_any(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+            else backward_any<G>(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+        } else {
+            backward_any<G>(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+        }
         check_hip(hipMemcpy2DAsync(grid, gs * sizeof(G), dg, gsz * sizeof(G), gsz * sizeof(G), batch,
                                    hipMemcpyDeviceToHost, s),
                   "grid download");
@@ -489,6 +553,22 @@ zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, 
                         ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
     return backward_entry<double>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
                                   grid_stride, mem, stream);
+}
+
+zgl_status zgl_backward_adjoint(zgl_plan* plan, const double* grid, size_t grid_order,
+                                double* coeffs, size_t exp_order, size_t batch,
+                                ptrdiff_t grid_stride, ptrdiff_t coeff_stride, zgl_mem mem,
+                                void* stream) {
+    return forward_entry<double>(plan, grid, grid_order, coeffs, exp_order, batch, grid_stride,
+                                 coeff_stride, mem, stream, true);
+}
+
+zgl_status zgl_forward_adjoint(zgl_plan* plan, const double* coeffs, size_t exp_order,
+                               double* grid, size_t grid_order, size_t batch,
+                               ptrdiff_t coeff_stride, ptrdiff_t grid_stride, zgl_mem mem,
+                               void* stream) {
+    return backward_entry<double>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
+                                  grid_stride, mem, stream, true);
 }
 
 zgl_status zgl_forward_grid32(zgl_plan* plan, const float* grid, size_t grid_order, double* coeffs,

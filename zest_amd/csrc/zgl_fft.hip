@@ -387,7 +387,8 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
         // two-for-one split: A = (Z_m + conj Z_{n-m}) / 2, B = (Z_m - conj Z_{n-m}) / (2i)
         const C A = mk<C>(R(0.5) * (zm.x + zc.x), R(0.5) * (zm.y - zc.y));
         const C B = mk<C>(R(0.5) * (zm.y + zc.y), R(-0.5) * (zm.x - zc.x));
-        const R W = (R)(wlat[h] * (wrad ? wrad[c] : 1.0) * scale);
+        // quadrature weight (none for the transpose of the backward transform)
+        const R W = (R)((wlat ? wlat[h] * (wrad ? wrad[c] : 1.0) : 1.0) * scale);
         // H = weight * conj(scale * X)
         const C Ha = mk<C>(W * A.x, -W * A.y);
         const C Hb = mk<C>(W * B.x, -W * B.y);
@@ -410,7 +411,8 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
     const typename Cplx<R>::type* __restrict__ spec, long spec_stride, G* __restrict__ grid,
-    long grid_stride, PairMap pm, int nm) {
+    long grid_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
+    const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
     const int tile = xcd_tile(pm);
     if (tile >= pm.ntiles) return;
@@ -474,8 +476,14 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
             const int h = pm.ncol > 1 ? h0 : h0 + t;
             const int c = pm.ncol > 1 ? c0 + t : 0;
             if (h >= pm.jh || c >= pm.ncol) continue;
-            const C z = data[i * T + t];
+            C z = data[i * T + t];
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+            if (wlat) {  // output weights (the transpose of the forward transform)
+                R W = (R)(wlat[h] * (wrad ? wrad[c] : 1.0) * scale);
+                if (jn == js) W *= (R)center_factor;
+                z.x *= W;
+                z.y *= W;
+            }
             G* row = g + (long)i * pm.lon_stride + c;
             row[(long)jn * pm.lat_stride] = (G)z.x;
             if (js != jn) row[(long)js * pm.lat_stride] = (G)z.y;
@@ -485,8 +493,14 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
             const int t = e / n, i = e - t * n;
             const int h = h0 + t;
             if (h >= pm.jh) continue;
-            const C z = data[i * T + t];
+            C z = data[i * T + t];
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+            if (wlat) {
+                R W = (R)(wlat[h] * scale);
+                if (jn == js) W *= (R)center_factor;
+                z.x *= W;
+                z.y *= W;
+            }
             g[(long)jn * pm.lat_stride + i] = (G)z.x;
             if (js != jn) g[(long)js * pm.lat_stride + i] = (G)z.y;
         }
@@ -547,7 +561,8 @@ void launch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
     k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, gs, (C*)spec, ss, pm,
-                                 L.nm_used, L.scale, L.t->wlat, L.t->wrad, L.center_factor);
+                                 L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
+                                 L.center_factor);
     check_hip(hipGetLastError(), "fft_forward launch");
 }
 
@@ -565,7 +580,8 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, lon
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
     k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid, gs,
-                                 pm, L.nm_used);
+                                 pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
+                                 L.center_factor);
     check_hip(hipGetLastError(), "fft_backward launch");
 }
 

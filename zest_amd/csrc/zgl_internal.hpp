@@ -114,7 +114,10 @@ struct FftLaunch {
     const OrderTables* t;
     int layout_kind;
     int nm_used;            // modes written (forward) or read (backward)
-    double scale;           // forward prefactor
+    bool weighted;          // forward: quadrature weights on the input (off for the
+                            // transpose of the backward); backward: weights on the
+                            // output (on only for the transpose of the forward)
+    double scale;           // prefactor applied with the weights
     double center_factor;   // weight factor of an equator node (zest's SH quirk: 2)
 };
 
@@ -169,5 +172,9 @@ template <class G>
 void grid_residual(const G* g, long gs, double* r, long rs, long n, int nb, hipStream_t s);
 // c[b][i] += d[b][i] for i < n.
 void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s);
+// Zernike coefficients times kappa_n = 2n+3 (when `unnormed`), n < nt:
+// out[b][tet(n,l,m)] = kappa_n * c[b][tet(n,l,m)] (both parts).
+void coeff_kappa_scale(const double* c, long cs, double* out, long os, int nt, int unnormed,
+                       int nb, hipStream_t s);
 
 }  // namespace zgl

@@ -293,6 +293,27 @@ void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int 
     check_hip(hipGetLastError(), "accumulate launch");
 }
 
+namespace {
+// one block per (n, instance): the coefficients of radial degree n are the
+// contiguous range [tet(n,n%2,0), tet(n+1,(n+1)%2,0)) of pairs
+__global__ void kappa_kernel(const double* __restrict__ c, long cs, double* __restrict__ out, long os,
+                             int unnormed) {
+    const int n = blockIdx.x;
+    const long lo = tet_idx(n, n & 1, 0), hi = tet_idx(n + 1, (n + 1) & 1, 0);
+    const double k = unnormed ? (double)(2 * n + 3) : 1.0;
+    const double* src = c + blockIdx.y * cs;
+    double* dst = out + blockIdx.y * os;
+    for (long e = 2 * lo + threadIdx.x; e < 2 * hi; e += blockDim.x) dst[e] = k * src[e];
+}
+}  // namespace
+
+void coeff_kappa_scale(const double* c, long cs, double* out, long os, int nt, int unnormed,
+                       int nb, hipStream_t s) {
+    if (nt <= 0 || nb <= 0) return;
+    kappa_kernel<<<dim3(nt, nb), 256, 0, s>>>(c, cs, out, os, unnormed);
+    check_hip(hipGetLastError(), "kappa scale launch");
+}
+
 static long ptab_size(const Geom& g) {
     const int order = g.order;
     return plm_offset(order, order - 1, 1, g.jh) + (long)plm_count(order, order - 1, 1) * g.jh;
