@@ -129,6 +129,28 @@ zgl_status zgl_backward(zgl_plan *plan, const double *coeffs, size_t exp_order,
                         ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
                         zgl_mem mem, void *stream);
 
+/* Ragged batches: many instances of uneven orders in one call.
+ * Replaces a loop of forward_transform / backward_transform calls over
+ * transformers of different orders (include/zernike_glq_transformer.hpp:640-679,
+ * include/sh_glq_transformer.hpp:632-668); each job's expansion order equals
+ * its grid order. Device memory only; the work is enqueued on `stream`. The
+ * jobs are grouped by order (largest first) and the groups spread over the
+ * plan's internal streams; on a non-null stream the pipeline is captured into
+ * a HIP graph on the first call and replayed while the job list (orders and
+ * pointers) is unchanged. Jobs of order 0 are skipped. */
+typedef struct zgl_job {
+    size_t order;    /* grid order = expansion order of this instance */
+    double *grid;    /* device pointer: grid of that order in zest layout */
+    double *coeffs;  /* device pointer: {cos, sin} coefficients of that order */
+} zgl_job;
+
+zgl_status zgl_forward_ragged(zgl_plan *plan, const zgl_job *jobs, size_t njobs, void *stream);
+zgl_status zgl_backward_ragged(zgl_plan *plan, const zgl_job *jobs, size_t njobs, void *stream);
+/* Modelled work per internal stream of the last ragged call (its load
+ * balance); no counterpart in zest. */
+zgl_status zgl_ragged_lane_costs(const zgl_plan *plan, double *costs, int max_lanes,
+                                 int *num_lanes);
+
 /* Transposes of the two transforms, for gradients. No counterpart in zest;
  * they are the adjoints of the members replaced above
  * (include/zernike_glq_transformer.hpp:640-679,

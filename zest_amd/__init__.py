@@ -65,6 +65,12 @@ def _load():
     lib.zgl_forward.restype = i32
     lib.zgl_backward.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_backward.restype = i32
+    lib.zgl_forward_ragged.argtypes = [vp, ctypes.POINTER(Job), sz, vp]
+    lib.zgl_forward_ragged.restype = i32
+    lib.zgl_backward_ragged.argtypes = [vp, ctypes.POINTER(Job), sz, vp]
+    lib.zgl_backward_ragged.restype = i32
+    lib.zgl_ragged_lane_costs.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i32)]
+    lib.zgl_ragged_lane_costs.restype = i32
     lib.zgl_backward_adjoint.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_backward_adjoint.restype = i32
     lib.zgl_forward_adjoint.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
@@ -82,6 +88,11 @@ def _load():
     return lib
 
 
+class Job(ctypes.Structure):
+    """zgl_job of include/zgl.h."""
+    _fields_ = [("order", ctypes.c_size_t), ("grid", ctypes.c_void_p), ("coeffs", ctypes.c_void_p)]
+
+
 class StageStats(ctypes.Structure):
     """zgl_stage_stats of include/zgl.h."""
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong),
@@ -95,6 +106,7 @@ EXPORTS = (
     "zgl_plan_create", "zgl_plan_destroy", "zgl_plan_resize", "zgl_plan_order",
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
     "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
+    "zgl_forward_ragged", "zgl_backward_ragged", "zgl_ragged_lane_costs",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -219,6 +231,36 @@ class Plan:
         _check(fn(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                   grid_stride, gm, _stream_ptr(stream, gm)))
 
+
+    @staticmethod
+    def _jobs(jobs):
+        arr = (Job * max(1, len(jobs)))()
+        for i, (order, grid, coeffs) in enumerate(jobs):
+            for t in (grid, coeffs):
+                if t is not None and (not t.is_cuda or not t.is_contiguous() or t.dtype.itemsize != 8):
+                    raise ValueError("ragged jobs take contiguous float64 device tensors")
+            arr[i].order = order
+            arr[i].grid = grid.data_ptr() if grid is not None else None
+            arr[i].coeffs = coeffs.data_ptr() if coeffs is not None else None
+        return arr
+
+    def forward_ragged(self, jobs, stream=None):
+        """Forward transforms of [(order, grid, coeffs), ...] of uneven orders
+        (zgl_forward_ragged)."""
+        arr = self._jobs(jobs)
+        _check(LIB.zgl_forward_ragged(self._h, arr, len(jobs), _stream_ptr(stream, MEM_DEVICE)))
+
+    def backward_ragged(self, jobs, stream=None):
+        """Backward transforms of [(order, grid, coeffs), ...] (zgl_backward_ragged)."""
+        arr = self._jobs(jobs)
+        _check(LIB.zgl_backward_ragged(self._h, arr, len(jobs), _stream_ptr(stream, MEM_DEVICE)))
+
+    def ragged_lane_costs(self) -> list:
+        """Modelled work per internal stream of the last ragged call."""
+        arr = (ctypes.c_double * 16)()
+        n = ctypes.c_int()
+        _check(LIB.zgl_ragged_lane_costs(self._h, arr, 16, ctypes.byref(n)))
+        return [arr[i] for i in range(min(n.value, 16))]
 
     def backward_adjoint(self, grid, coeffs, grid_order: int, exp_order: int | None = None,
                          batch: int = 1, grid_stride: int = 0, coeff_stride: int = 0, stream=None):

@@ -46,20 +46,33 @@ size_t env_mb(const char* name, size_t dflt) {
 }  // namespace
 }  // namespace zgl
 
-struct zgl_plan {
-    zgl_kind kind;
-    int znorm, shnorm, phase, layout;
-    zgl_precision precision;
-    size_t order = 0;
-    std::map<size_t, zgl::OrderTables> tables;
+namespace zgl {
+// Device scratch of one pipeline: the spectrum and flm of a chunk of
+// instances, and the refinement residual grids / corrections.
+struct Scratch {
     void* spec = nullptr;
     size_t spec_cap = 0;  // bytes
     void* flm = nullptr;
     size_t flm_cap = 0;   // bytes
     double* rgrid = nullptr;  // refinement residual grids (chunk)
     size_t rgrid_cap = 0;     // doubles
-    double* rcoef = nullptr;  // refinement corrections (chunk)
+    double* rcoef = nullptr;  // refinement corrections / scaled coefficients (chunk)
     size_t rcoef_cap = 0;     // doubles
+    void release() {
+        for (void* q : {spec, flm, (void*)rgrid, (void*)rcoef})
+            if (q) (void)hipFree(q);
+        *this = Scratch{};
+    }
+};
+}  // namespace zgl
+
+struct zgl_plan {
+    zgl_kind kind;
+    int znorm, shnorm, phase, layout;
+    zgl_precision precision;
+    size_t order = 0;
+    std::map<size_t, zgl::OrderTables> tables;
+    zgl::Scratch main;                // the batched entry points
     double* dgrid = nullptr;
     size_t dgrid_cap = 0;  // doubles
     double* dcoef = nullptr;
@@ -69,6 +82,20 @@ struct zgl_plan {
     struct Rec { int stage; int nb; hipEvent_t start, stop; };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
+    // ragged batches (zgl_forward_ragged / zgl_backward_ragged)
+    struct Ragged {
+        std::vector<hipStream_t> lanes;   // internal streams, default priority
+        std::vector<zgl::Scratch> scratch;  // one per lane
+        std::vector<hipEvent_t> ev;       // fork, then one join event per lane
+        // the last job list and its captured pipeline
+        std::vector<zgl_job> jobs;
+        int dir = -1;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        long* offs = nullptr;  // device: grid offsets [njobs], coefficient offsets [njobs]
+        size_t offs_cap = 0;   // longs
+        std::vector<double> lane_cost;  // modelled work per lane of the last build
+    } rg;
 };
 
 namespace zgl {
@@ -109,6 +136,18 @@ void ensure(T*& ptr, size_t& cap, size_t need) {
     cap = 0;
     check_hip(hipMalloc((void**)&ptr, need * sizeof(T)), "scratch allocation");
     cap = need;
+}
+
+void ragged_release(zgl_plan::Ragged& r) {
+    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+    if (r.graph) (void)hipGraphDestroy(r.graph);
+    for (auto l : r.lanes)
+        if (l) (void)hipStreamDestroy(l);
+    for (auto e : r.ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& sc : r.scratch) sc.release();
+    if (r.offs) (void)hipFree(r.offs);
+    r = zgl_plan::Ragged{};
 }
 
 void ensure_bytes(void*& ptr, size_t& cap, size_t need) {
@@ -193,14 +232,14 @@ double forward_prefactor(const zgl_plan* p, const Geom& g) {
 // adjoint: the transpose of the backward transform (no quadrature weights,
 // no prefactor, no kappa_n), otherwise the forward transform.
 template <class T, class G>
-void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt,
-                 size_t batch, long gs, long cs, hipStream_t s, bool adjoint = false) {
+void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<double> coeffs,
+                 size_t nt, size_t batch, hipStream_t s, bool adjoint = false) {
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
     const Geom& g = t.g;
     const Chunking ch = chunking(p, g, batch, sizeof(C));
-    ensure_bytes(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb * sizeof(C));
-    if (ch.flm_stride) ensure_bytes(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb * sizeof(C));
+    ensure_bytes(sc.spec, sc.spec_cap, (size_t)ch.spec_stride * ch.nb * sizeof(C));
+    if (ch.flm_stride) ensure_bytes(sc.flm, sc.flm_cap, (size_t)ch.flm_stride * ch.nb * sizeof(C));
     FftLaunch L;
     L.g = &g;
     L.t = &t;
@@ -212,14 +251,14 @@ void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt
     const int kappa = !adjoint && p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0;
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
-        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T, G>(L, grid + b0 * gs, gs, p->spec, ch.spec_stride, nb, s); });
+        timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T, G>(L, grid.from(b0), sc.spec, ch.spec_stride, nb, s); });
         if (p->kind == ZGL_ZERNIKE) {
-            ZtGemmArgs a{&t, (int)nt, kappa, p->spec, ch.spec_stride,
-                         p->flm, ch.flm_stride, coeffs + b0 * cs, cs, nb};
+            ZtGemmArgs a{&t, (int)nt, kappa, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride,
+                         coeffs.from(b0), nb};
             timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
             timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward<T>(a, s); });
         } else {
-            StGemmArgs a{&t, (int)nt, p->spec, ch.spec_stride, coeffs + b0 * cs, cs, nb};
+            StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, coeffs.from(b0), nb};
             timed(p, ST_LEG_FWD, nb, s, [&] { st_legendre_forward<T>(a, s); });
         }
     }
@@ -230,14 +269,14 @@ void run_forward(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt
 // kappa_n to a scaled copy of the coefficients), otherwise the backward
 // transform.
 template <class T, class G>
-void run_backward(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t N,
-                  size_t batch, long cs, long gs, hipStream_t s, bool adjoint = false) {
+void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt, Inst<G> grid,
+                  size_t N, size_t batch, hipStream_t s, bool adjoint = false) {
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const OrderTables& t = tables_for(p, N);
     const Geom& g = t.g;
     const Chunking ch = chunking(p, g, batch, sizeof(C));
-    ensure_bytes(p->spec, p->spec_cap, (size_t)ch.spec_stride * ch.nb * sizeof(C));
-    if (ch.flm_stride) ensure_bytes(p->flm, p->flm_cap, (size_t)ch.flm_stride * ch.nb * sizeof(C));
+    ensure_bytes(sc.spec, sc.spec_cap, (size_t)ch.spec_stride * ch.nb * sizeof(C));
+    if (ch.flm_stride) ensure_bytes(sc.flm, sc.flm_cap, (size_t)ch.flm_stride * ch.nb * sizeof(C));
     FftLaunch L;
     L.g = &g;
     L.t = &t;
@@ -248,83 +287,83 @@ void run_backward(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t 
     L.center_factor = (adjoint && p->kind == ZGL_SH) ? 2.0 : 1.0;
     const bool kappa = adjoint && p->kind == ZGL_ZERNIKE && p->znorm == ZGL_ZNORM_UNNORMED;
     const long ncw = 2 * (long)coeff_size(p, nt);
-    if (kappa) ensure(p->rcoef, p->rcoef_cap, (size_t)ncw * ch.nb);
+    if (kappa) ensure(sc.rcoef, sc.rcoef_cap, (size_t)ncw * ch.nb);
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
+        // the kernels read coefficients through Inst<double>; they never write them
+        Inst<double> cin{const_cast<double*>(coeffs.base), coeffs.stride, coeffs.offs};
+        cin = cin.from(b0);
+        if (kappa) {
+            coeff_kappa_scale(cin, sc.rcoef, ncw, (int)nt, 1, nb, s);
+            cin = Inst<double>{sc.rcoef, ncw, nullptr};
+        }
         if (p->kind == ZGL_ZERNIKE) {
-            const double* cin = coeffs + b0 * cs;
-            long cstride = cs;
-            if (kappa) {
-                coeff_kappa_scale(cin, cs, p->rcoef, ncw, (int)nt, 1, nb, s);
-                cin = p->rcoef;
-                cstride = ncw;
-            }
-            ZtGemmArgs a{&t, (int)nt, 0, p->spec, ch.spec_stride, p->flm, ch.flm_stride,
-                         const_cast<double*>(cin), cstride, nb};
+            ZtGemmArgs a{&t, (int)nt, 0, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride, cin, nb};
             timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward<T>(a, s); });
             timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward<T>(a, s); });
         } else {
-            StGemmArgs a{&t, (int)nt, p->spec, ch.spec_stride, const_cast<double*>(coeffs) + b0 * cs,
-                         cs, nb};
+            StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, cin, nb};
             timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward<T>(a, s); });
         }
-        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward<T, G>(L, p->spec, ch.spec_stride, grid + b0 * gs, gs, nb, s); });
+        timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward<T, G>(L, sc.spec, ch.spec_stride, grid.from(b0), nb, s); });
     }
 }
 
 // Forward in the plan's precision. With ZGL_FP32_REFINE each chunk of
 // instances is refined once: c += F32(g - B64(c)), the residual in FP64.
 template <class G>
-void forward_any(zgl_plan* p, const G* grid, size_t N, double* coeffs, size_t nt,
-                 size_t batch, long gs, long cs, hipStream_t s) {
+void forward_any(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<double> coeffs,
+                 size_t nt, size_t batch, hipStream_t s) {
     if (p->precision == ZGL_FP64) {
-        if constexpr (std::is_same_v<G, double>) run_forward<double, G>(p, grid, N, coeffs, nt, batch, gs, cs, s);
+        if constexpr (std::is_same_v<G, double>) run_forward<double, G>(p, sc, grid, N, coeffs, nt, batch, s);
         return;
     }
     if (p->precision == ZGL_FP32) {
-        run_forward<float, G>(p, grid, N, coeffs, nt, batch, gs, cs, s);
+        run_forward<float, G>(p, sc, grid, N, coeffs, nt, batch, s);
         return;
     }
     const long gsz = (long)grid_size(p, N);
     const long ncw = 2 * (long)coeff_size(p, nt);
     const Chunking ch = chunking(p, tables_for(p, N).g, batch, sizeof(float2));
     const size_t nb_max = (size_t)ch.nb;
-    ensure(p->rgrid, p->rgrid_cap, (size_t)gsz * nb_max);
-    ensure(p->rcoef, p->rcoef_cap, (size_t)ncw * nb_max);
+    ensure(sc.rgrid, sc.rgrid_cap, (size_t)gsz * nb_max);
+    ensure(sc.rcoef, sc.rcoef_cap, (size_t)ncw * nb_max);
+    const Inst<double> rg{sc.rgrid, gsz, nullptr};
+    const Inst<double> rc{sc.rcoef, ncw, nullptr};
     for (size_t b0 = 0; b0 < batch; b0 += nb_max) {
         const int nb = (int)std::min(nb_max, batch - b0);
-        const G* g = grid + b0 * gs;
-        double* c = coeffs + b0 * cs;
-        run_forward<float, G>(p, g, N, c, nt, nb, gs, cs, s);
-        run_backward<double, double>(p, c, nt, p->rgrid, N, nb, cs, gsz, s);
-        grid_residual<G>(g, gs, p->rgrid, gsz, gsz, nb, s);
-        run_forward<float, double>(p, p->rgrid, N, p->rcoef, nt, nb, gsz, ncw, s);
-        coeff_accumulate(c, cs, p->rcoef, ncw, ncw, nb, s);
+        const Inst<const G> g = grid.from(b0);
+        const Inst<double> c = coeffs.from(b0);
+        run_forward<float, G>(p, sc, g, N, c, nt, nb, s);
+        run_backward<double, double>(p, sc, c, nt, rg, N, nb, s);
+        grid_residual<G>(g, sc.rgrid, gsz, gsz, nb, s);
+        run_forward<float, double>(p, sc, rg, N, rc, nt, nb, s);
+        coeff_accumulate(c, sc.rcoef, ncw, ncw, nb, s);
     }
 }
 
 template <class G>
-void backward_any(zgl_plan* p, const double* coeffs, size_t nt, G* grid, size_t N,
-                  size_t batch, long cs, long gs, hipStream_t s) {
+void backward_any(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt, Inst<G> grid,
+                  size_t N, size_t batch, hipStream_t s) {
     if (p->precision == ZGL_FP64) {
-        if constexpr (std::is_same_v<G, double>) run_backward<double, G>(p, coeffs, nt, grid, N, batch, cs, gs, s);
+        if constexpr (std::is_same_v<G, double>) run_backward<double, G>(p, sc, coeffs, nt, grid, N, batch, s);
     } else {
-        run_backward<float, G>(p, coeffs, nt, grid, N, batch, cs, gs, s);
+        run_backward<float, G>(p, sc, coeffs, nt, grid, N, batch, s);
     }
 }
 
 // The two transposes, in the plan's arithmetic (FP64, or FP32 for both FP32
 // modes: refinement does not apply to a transpose).
-void backward_adjoint_any(zgl_plan* p, const double* grid, size_t N, double* coeffs, size_t nt,
-                          size_t batch, long gs, long cs, hipStream_t s) {
-    if (p->precision == ZGL_FP64) run_forward<double, double>(p, grid, N, coeffs, nt, batch, gs, cs, s, true);
-    else run_forward<float, double>(p, grid, N, coeffs, nt, batch, gs, cs, s, true);
+void backward_adjoint_any(zgl_plan* p, Scratch& sc, Inst<const double> grid, size_t N,
+                          Inst<double> coeffs, size_t nt, size_t batch, hipStream_t s) {
+    if (p->precision == ZGL_FP64) run_forward<double, double>(p, sc, grid, N, coeffs, nt, batch, s, true);
+    else run_forward<float, double>(p, sc, grid, N, coeffs, nt, batch, s, true);
 }
 
-void forward_adjoint_any(zgl_plan* p, const double* coeffs, size_t nt, double* grid, size_t N,
-                         size_t batch, long cs, long gs, hipStream_t s) {
-    if (p->precision == ZGL_FP64) run_backward<double, double>(p, coeffs, nt, grid, N, batch, cs, gs, s, true);
-    else run_backward<float, double>(p, coeffs, nt, grid, N, batch, cs, gs, s, true);
+void forward_adjoint_any(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt,
+                         Inst<double> grid, size_t N, size_t batch, hipStream_t s) {
+    if (p->precision == ZGL_FP64) run_backward<double, double>(p, sc, coeffs, nt, grid, N, batch, s, true);
+    else run_backward<float, double>(p, sc, coeffs, nt, grid, N, batch, s, true);
 }
 
 template <class F>
@@ -398,8 +437,9 @@ zgl_status zgl_plan_destroy(zgl_plan* plan) {
         (void)hipEventDestroy(r.stop);
     }
     for (auto e : plan->pool) (void)hipEventDestroy(e);
-    for (void* q : {(void*)plan->spec, (void*)plan->flm, (void*)plan->dgrid, (void*)plan->dcoef,
-                    (void*)plan->rgrid, (void*)plan->rcoef})
+    plan->main.release();
+    ragged_release(plan->rg);
+    for (void* q : {(void*)plan->dgrid, (void*)plan->dcoef})
         if (q) (void)hipFree(q);
     delete plan;
     return ZGL_SUCCESS;
@@ -451,11 +491,13 @@ zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, doubl
         if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
+            const Inst<const G> gi{grid, gs, nullptr};
+            const Inst<double> ci{coeffs, cs, nullptr};
             if constexpr (std::is_same_v<G, double>) {
-                if (adjoint) backward_adjoint_any(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
-                else forward_any<G>(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+                if (adjoint) backward_adjoint_any(plan, plan->main, gi, grid_order, ci, nt, batch, s);
+                else forward_any<G>(plan, plan->main, gi, grid_order, ci, nt, batch, s);
             } else {
-                forward_any<G>(plan, grid, grid_order, coeffs, nt, batch, gs, cs, s);
+                forward_any<G>(plan, plan->main, gi, grid_order, ci, nt, batch, s);
             }
             check_hip(hipGetLastError(), "forward");
             return ZGL_SUCCESS;
@@ -467,11 +509,13 @@ zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, doubl
         check_hip(hipMemcpy2DAsync(dg, gsz * sizeof(G), grid, gs * sizeof(G), gsz * sizeof(G), batch,
                                    hipMemcpyHostToDevice, s),
                   "grid upload");
+        const Inst<const G> dgi{dg, gsz, nullptr};
+        const Inst<double> dci{plan->dcoef, (long)ncw, nullptr};
         if constexpr (std::is_same_v<G, double>) {
-            if (adjoint) backward_adjoint_any(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
-            else forward_any<G>(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+            if (adjoint) backward_adjoint_any(plan, plan->main, dgi, grid_order, dci, nt, batch, s);
+            else forward_any<G>(plan, plan->main, dgi, grid_order, dci, nt, batch, s);
         } else {
-            forward_any<G>(plan, dg, grid_order, plan->dcoef, nt, batch, gsz, (long)ncw, s);
+            forward_any<G>(plan, plan->main, dgi, grid_order, dci, nt, batch, s);
         }
         check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, ncw * sizeof(double),
                                    ncw * sizeof(double), batch, hipMemcpyDeviceToHost, s),
@@ -505,11 +549,13 @@ zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order
         if (!grid || (!coeffs && nt > 0)) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
+            const Inst<const double> ci{coeffs, cs, nullptr};
+            const Inst<G> gi{grid, gs, nullptr};
             if constexpr (std::is_same_v<G, double>) {
-                if (adjoint) forward_adjoint_any(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
-                else backward_any<G>(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+                if (adjoint) forward_adjoint_any(plan, plan->main, ci, nt, gi, grid_order, batch, s);
+                else backward_any<G>(plan, plan->main, ci, nt, gi, grid_order, batch, s);
             } else {
-                backward_any<G>(plan, coeffs, nt, grid, grid_order, batch, cs, gs, s);
+                backward_any<G>(plan, plan->main, ci, nt, gi, grid_order, batch, s);
             }
             check_hip(hipGetLastError(), "backward");
             return ZGL_SUCCESS;
@@ -518,20 +564,186 @@ zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order
         ensure(plan->dgrid, plan->dgrid_cap, ((size_t)gsz * batch * sizeof(G) + 7) / 8);
         ensure(plan->dcoef, plan->dcoef_cap, std::max<size_t>(ncr, 1) * batch);
         G* dg = reinterpret_cast<G*>(plan->dgrid);
+        const Inst<const double> dci{plan->dcoef, (long)ncr, nullptr};
+        const Inst<G> dgi{dg, gsz, nullptr};
         if (ncr)
             check_hip(hipMemcpy2DAsync(plan->dcoef, ncr * sizeof(double), coeffs, cs * sizeof(double),
                                        ncr * sizeof(double), batch, hipMemcpyHostToDevice, s),
                       "coefficient upload");
         if constexpr (std::is_same_v<G, double>) {
-            if (adjoint) forward_adjoint_any(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
-            else backward_any<G>(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+            if (adjoint) forward_adjoint_any(plan, plan->main, dci, nt, dgi, grid_order, batch, s);
+            else backward_any<G>(plan, plan->main, dci, nt, dgi, grid_order, batch, s);
         } else {
-            backward_any<G>(plan, plan->dcoef, nt, dg, grid_order, batch, (long)ncr, gsz, s);
+            backward_any<G>(plan, plan->main, dci, nt, dgi, grid_order, batch, s);
         }
         check_hip(hipMemcpy2DAsync(grid, gs * sizeof(G), dg, gsz * sizeof(G), gsz * sizeof(G), batch,
                                    hipMemcpyDeviceToHost, s),
                   "grid download");
         check_hip(hipStreamSynchronize(s), "backward");
+        return ZGL_SUCCESS;
+    });
+}
+
+// ------------------------------------------------------------ ragged batches
+// Jobs of uneven orders: an order-sorted queue of per-order groups (largest
+// order first), dealt to kLanes internal streams by longest-processing-time
+// on a modelled cost; each group runs the batched pipeline of its order with
+// per-instance addresses (Inst offsets). On a non-null stream the whole
+// fork/join pipeline is captured once into a hipGraph and replayed while the
+// job list is unchanged (ZGL_RAGGED_GRAPH=0 runs it without a graph).
+constexpr int kLanes = 4;
+
+bool ragged_graphs() {
+    const char* e = std::getenv("ZGL_RAGGED_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+// modelled work of `count` instances of order N: memory traffic ~ N^3
+double job_cost(size_t N, size_t count) {
+    const double n = (double)N + 1.0;
+    return (double)count * n * n * n;
+}
+
+void drop_graph(zgl_plan::Ragged& r) {
+    if (r.exec) (void)hipGraphExecDestroy(r.exec);
+    if (r.graph) (void)hipGraphDestroy(r.graph);
+    r.exec = nullptr;
+    r.graph = nullptr;
+    r.jobs.clear();
+    r.dir = -1;
+}
+
+bool same_jobs(const std::vector<zgl_job>& a, const zgl_job* b, size_t n) {
+    if (a.size() != n) return false;
+    for (size_t i = 0; i < n; ++i)
+        if (a[i].order != b[i].order || a[i].grid != b[i].grid || a[i].coeffs != b[i].coeffs) return false;
+    return true;
+}
+
+// scratch for `count` instances of order N in the plan's precision
+void reserve(zgl_plan* p, Scratch& sc, size_t N, size_t count) {
+    const Geom& g = tables_for(p, N).g;
+    for (size_t csize : {sizeof(double2), sizeof(float2)}) {
+        if (csize == sizeof(double2) && p->precision == ZGL_FP32) continue;
+        if (csize == sizeof(float2) && p->precision == ZGL_FP64) continue;
+        const Chunking ch = chunking(p, g, count, csize);
+        ensure_bytes(sc.spec, sc.spec_cap, (size_t)ch.spec_stride * ch.nb * csize);
+        if (ch.flm_stride) ensure_bytes(sc.flm, sc.flm_cap, (size_t)ch.flm_stride * ch.nb * csize);
+    }
+    if (p->precision == ZGL_FP32_REFINE) {
+        const Chunking ch = chunking(p, g, count, sizeof(float2));
+        ensure(sc.rgrid, sc.rgrid_cap, grid_size(p, N) * ch.nb);
+        ensure(sc.rcoef, sc.rcoef_cap, 2 * coeff_size(p, N) * ch.nb);
+    }
+}
+
+zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void* stream, bool forward) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (njobs && !jobs) return fail(ZGL_ERROR_INVALID_VALUE, "jobs is NULL");
+    for (size_t i = 0; i < njobs; ++i) {
+        if (jobs[i].order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "job order exceeds the supported maximum");
+        if (jobs[i].order && (!jobs[i].grid || !jobs[i].coeffs))
+            return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer in a job");
+    }
+    return guarded([&]() -> zgl_status {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        auto& R = plan->rg;
+        const bool use_graph = s != nullptr && !plan->profiling && ragged_graphs();
+        if (use_graph && R.exec && R.dir == (int)forward && same_jobs(R.jobs, jobs, njobs)) {
+            check_hip(hipGraphLaunch(R.exec, s), "ragged graph launch");
+            return ZGL_SUCCESS;
+        }
+        drop_graph(R);
+
+        // order-sorted queue of non-empty jobs, largest order first
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < njobs; ++i)
+            if (jobs[i].order) idx.push_back(i);
+        if (idx.empty()) return ZGL_SUCCESS;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t a, size_t b) { return jobs[a].order > jobs[b].order; });
+        struct Group { size_t order, first, count; int lane; };
+        std::vector<Group> groups;
+        for (size_t k = 0; k < idx.size(); ++k) {
+            if (groups.empty() || groups.back().order != jobs[idx[k]].order)
+                groups.push_back(Group{jobs[idx[k]].order, k, 0, 0});
+            ++groups.back().count;
+        }
+
+        // per-instance element offsets from the lowest grid / coefficient address
+        const size_t n = idx.size();
+        double* gbase = jobs[idx[0]].grid;
+        double* cbase = jobs[idx[0]].coeffs;
+        for (size_t k : idx) {
+            gbase = std::min(gbase, jobs[k].grid);
+            cbase = std::min(cbase, jobs[k].coeffs);
+        }
+        std::vector<long> offs(2 * n);
+        for (size_t k = 0; k < n; ++k) {
+            offs[k] = (long)(jobs[idx[k]].grid - gbase);
+            offs[n + k] = (long)(jobs[idx[k]].coeffs - cbase);
+        }
+        ensure(R.offs, R.offs_cap, 2 * n);
+        check_hip(hipMemcpy(R.offs, offs.data(), 2 * n * sizeof(long), hipMemcpyHostToDevice),
+                  "ragged offsets upload");
+
+        // lanes: streams, events, scratch; groups dealt by longest processing time
+        if (R.lanes.empty()) {
+            R.lanes.resize(kLanes, nullptr);
+            R.ev.resize(kLanes + 1, nullptr);
+            R.scratch.resize(kLanes);
+            for (auto& l : R.lanes) check_hip(hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "lane stream");
+            for (auto& e : R.ev) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
+        }
+        R.lane_cost.assign(kLanes, 0.0);
+        for (auto& g : groups) {
+            tables_for(plan, g.order);
+            const int l = (int)(std::min_element(R.lane_cost.begin(), R.lane_cost.end()) - R.lane_cost.begin());
+            g.lane = l;
+            R.lane_cost[l] += job_cost(g.order, g.count);
+            reserve(plan, R.scratch[l], g.order, g.count);
+        }
+
+        auto enqueue = [&]() {
+            check_hip(hipEventRecord(R.ev[0], s), "fork");
+            for (auto l : R.lanes) check_hip(hipStreamWaitEvent(l, R.ev[0], 0), "fork");
+            for (const auto& g : groups) {
+                hipStream_t ls = R.lanes[g.lane];
+                Scratch& sc = R.scratch[g.lane];
+                const long* go = R.offs + g.first;
+                const long* co = R.offs + n + g.first;
+                if (forward)
+                    forward_any<double>(plan, sc, Inst<const double>{gbase, 0, go}, g.order,
+                                        Inst<double>{cbase, 0, co}, g.order, g.count, ls);
+                else
+                    backward_any<double>(plan, sc, Inst<const double>{cbase, 0, co}, g.order,
+                                         Inst<double>{gbase, 0, go}, g.order, g.count, ls);
+            }
+            for (int l = 0; l < kLanes; ++l) {
+                check_hip(hipEventRecord(R.ev[1 + l], R.lanes[l]), "join");
+                check_hip(hipStreamWaitEvent(s, R.ev[1 + l], 0), "join");
+            }
+        };
+
+        if (!use_graph) {
+            enqueue();
+            check_hip(hipGetLastError(), "ragged");
+            return ZGL_SUCCESS;
+        }
+        check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
+        try {
+            enqueue();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check_hip(hipStreamEndCapture(s, &R.graph), "end capture");
+        check_hip(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0), "graph instantiate");
+        R.jobs.assign(jobs, jobs + njobs);
+        R.dir = (int)forward;
+        check_hip(hipGraphLaunch(R.exec, s), "ragged graph launch");
         return ZGL_SUCCESS;
     });
 }
@@ -553,6 +765,22 @@ zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, 
                         ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
     return backward_entry<double>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
                                   grid_stride, mem, stream);
+}
+
+zgl_status zgl_forward_ragged(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void* stream) {
+    return ragged_entry(plan, jobs, njobs, stream, true);
+}
+
+zgl_status zgl_backward_ragged(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void* stream) {
+    return ragged_entry(plan, jobs, njobs, stream, false);
+}
+
+zgl_status zgl_ragged_lane_costs(const zgl_plan* plan, double* costs, int max_lanes, int* num_lanes) {
+    if (!plan || !num_lanes) return fail(ZGL_ERROR_INVALID_VALUE, "NULL argument");
+    const auto& c = plan->rg.lane_cost;
+    *num_lanes = (int)c.size();
+    for (int i = 0; i < (int)c.size() && i < max_lanes && costs; ++i) costs[i] = c[i];
+    return ZGL_SUCCESS;
 }
 
 zgl_status zgl_backward_adjoint(zgl_plan* plan, const double* grid, size_t grid_order,

@@ -307,7 +307,7 @@ template <int T, int LK, class R, class G>
 __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
-    const G* __restrict__ grid, long grid_stride, typename Cplx<R>::type* __restrict__ spec,
+    Inst<const G> grid, typename Cplx<R>::type* __restrict__ spec,
     long spec_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     const int n = d.n;
     int h0, c0;
     tile_origin<T>(pm, tile, h0, c0);
-    const G* g = grid + (long)blockIdx.y * grid_stride;
+    const G* g = grid.at(blockIdx.y);
     C* sp = spec + (long)blockIdx.y * spec_stride;
 
     // Loads are issued in batches of kBatch per thread before any LDS store,
@@ -410,8 +410,7 @@ template <int T, int LK, class R, class G>
 __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
-    const typename Cplx<R>::type* __restrict__ spec, long spec_stride, G* __restrict__ grid,
-    long grid_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
+    const typename Cplx<R>::type* __restrict__ spec, long spec_stride, Inst<G> grid, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
     const int tile = xcd_tile(pm);
@@ -422,7 +421,7 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     const int n = d.n;
     int h0, c0;
     tile_origin<T>(pm, tile, h0, c0);
-    G* g = grid + (long)blockIdx.y * grid_stride;
+    G* g = grid.at(blockIdx.y);
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
     for (int e0 = threadIdx.x; e0 < F * T; e0 += kThreads * kBatch) {
@@ -548,7 +547,7 @@ template <> const double2* twp_of<double>(const FftPlan& f) { return f.twp; }
 template <> const float2* twp_of<float>(const FftPlan& f) { return f.twp32; }
 
 template <int T, class R, class G>
-void launch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long ss, int nb,
+void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss, int nb,
                     hipStream_t s) {
     using C = typename Cplx<R>::type;
     const FftPlan& f = L.t->fft;
@@ -560,14 +559,14 @@ void launch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
-    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, gs, (C*)spec, ss, pm,
+    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
                                  L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
     check_hip(hipGetLastError(), "fft_forward launch");
 }
 
 template <int T, class R, class G>
-void launch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, long gs, int nb,
+void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid, int nb,
                      hipStream_t s) {
     using C = typename Cplx<R>::type;
     const FftPlan& f = L.t->fft;
@@ -579,33 +578,33 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, lon
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
-    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid, gs,
+    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
                                  pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
     check_hip(hipGetLastError(), "fft_backward launch");
 }
 
 template <class R, class G>
-void dispatch_forward(const FftLaunch& L, const G* grid, long gs, void* spec, long ss, int nb,
+void dispatch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss, int nb,
                       hipStream_t s) {
     switch (L.t->fft.tile[sizeof(R) == 4]) {
-        case 16: launch_forward<16, R>(L, grid, gs, spec, ss, nb, s); break;
-        case 8: launch_forward<8, R>(L, grid, gs, spec, ss, nb, s); break;
-        case 4: launch_forward<4, R>(L, grid, gs, spec, ss, nb, s); break;
-        case 2: launch_forward<2, R>(L, grid, gs, spec, ss, nb, s); break;
-        default: launch_forward<1, R>(L, grid, gs, spec, ss, nb, s); break;
+        case 16: launch_forward<16, R>(L, grid, spec, ss, nb, s); break;
+        case 8: launch_forward<8, R>(L, grid, spec, ss, nb, s); break;
+        case 4: launch_forward<4, R>(L, grid, spec, ss, nb, s); break;
+        case 2: launch_forward<2, R>(L, grid, spec, ss, nb, s); break;
+        default: launch_forward<1, R>(L, grid, spec, ss, nb, s); break;
     }
 }
 
 template <class R, class G>
-void dispatch_backward(const FftLaunch& L, const void* spec, long ss, G* grid, long gs, int nb,
+void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid, int nb,
                        hipStream_t s) {
     switch (L.t->fft.tile[sizeof(R) == 4]) {
-        case 16: launch_backward<16, R>(L, spec, ss, grid, gs, nb, s); break;
-        case 8: launch_backward<8, R>(L, spec, ss, grid, gs, nb, s); break;
-        case 4: launch_backward<4, R>(L, spec, ss, grid, gs, nb, s); break;
-        case 2: launch_backward<2, R>(L, spec, ss, grid, gs, nb, s); break;
-        default: launch_backward<1, R>(L, spec, ss, grid, gs, nb, s); break;
+        case 16: launch_backward<16, R>(L, spec, ss, grid, nb, s); break;
+        case 8: launch_backward<8, R>(L, spec, ss, grid, nb, s); break;
+        case 4: launch_backward<4, R>(L, spec, ss, grid, nb, s); break;
+        case 2: launch_backward<2, R>(L, spec, ss, grid, nb, s); break;
+        default: launch_backward<1, R>(L, spec, ss, grid, nb, s); break;
     }
 }
 
@@ -717,23 +716,23 @@ void fft_plan_destroy(FftPlan& p) {
 }
 
 template <class R, class G>
-void fft_forward(const FftLaunch& L, const G* grid, long grid_stride, void* spec,
-                 long spec_stride, int nb, hipStream_t s) {
-    dispatch_forward<R>(L, grid, grid_stride, spec, spec_stride, nb, s);
+void fft_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long spec_stride, int nb,
+                 hipStream_t s) {
+    dispatch_forward<R>(L, grid, spec, spec_stride, nb, s);
 }
 
 template <class R, class G>
-void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, G* grid,
-                  long grid_stride, int nb, hipStream_t s) {
-    dispatch_backward<R>(L, spec, spec_stride, grid, grid_stride, nb, s);
+void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, Inst<G> grid, int nb,
+                  hipStream_t s) {
+    dispatch_backward<R>(L, spec, spec_stride, grid, nb, s);
 }
 
 // FP64 grids with FP64 or FP32 arithmetic; FP32 grids with FP32 arithmetic.
-template void fft_forward<double, double>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
-template void fft_forward<float, double>(const FftLaunch&, const double*, long, void*, long, int, hipStream_t);
-template void fft_forward<float, float>(const FftLaunch&, const float*, long, void*, long, int, hipStream_t);
-template void fft_backward<double, double>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
-template void fft_backward<float, double>(const FftLaunch&, const void*, long, double*, long, int, hipStream_t);
-template void fft_backward<float, float>(const FftLaunch&, const void*, long, float*, long, int, hipStream_t);
+template void fft_forward<double, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
+template void fft_forward<float, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
+template void fft_forward<float, float>(const FftLaunch&, Inst<const float>, void*, long, int, hipStream_t);
+template void fft_backward<double, double>(const FftLaunch&, const void*, long, Inst<double>, int, hipStream_t);
+template void fft_backward<float, double>(const FftLaunch&, const void*, long, Inst<double>, int, hipStream_t);
+template void fft_backward<float, float>(const FftLaunch&, const void*, long, Inst<float>, int, hipStream_t);
 
 }  // namespace zgl

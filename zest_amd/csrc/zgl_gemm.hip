@@ -274,8 +274,8 @@ struct ZtRadialFwd {
     using T = TT;
     static constexpr int A_KIND = ROWS_OUT, B_KIND = ROWS_OUT;
     static constexpr bool B_PAIR = true;
-    const T* rtab; const T* flm; double* coeffs;
-    long flm_stride, coeff_stride;  // elements of T, doubles
+    const T* rtab; const T* flm; Inst<double> coeffs;
+    long flm_stride;  // elements of T
     int nt, nr, unnormed;
     struct Prob { int active, M, N, K, l; const T* B; double* C; };
     __device__ Prob problem(int prob, int inst) const {
@@ -284,7 +284,7 @@ struct ZtRadialFwd {
         p.M = (nt - p.l + 1) / 2; p.K = nr; p.N = 2 * (p.l + 1);
         p.active = p.l < nt;
         p.B = flm + inst * flm_stride + tri_idx(p.l, 0) * 2 * nr;
-        p.C = coeffs + inst * coeff_stride;
+        p.C = coeffs.at(inst);
         return p;
     }
     __device__ const T* a_row(const Prob& p, int i) const {
@@ -305,8 +305,8 @@ struct ZtRadialBwd {
     using T = TT;
     static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_K;
     static constexpr bool B_PAIR = false;
-    const T* rtab; const double* coeffs; T* flm;
-    long flm_stride, coeff_stride;
+    const T* rtab; Inst<const double> coeffs; T* flm;
+    long flm_stride;
     int nt, nr;
     struct Prob { int active, M, N, K, l; const double* A; T* C; };
     __device__ Prob problem(int prob, int inst) const {
@@ -314,7 +314,7 @@ struct ZtRadialBwd {
         p.l = prob;
         p.M = 2 * (p.l + 1); p.K = (nt - p.l + 1) / 2; p.N = nr;
         p.active = p.l < nt;
-        p.A = coeffs + inst * coeff_stride;
+        p.A = coeffs.at(inst);
         p.C = flm + inst * flm_stride + tri_idx(p.l, 0) * 2 * nr;
         return p;
     }
@@ -336,8 +336,8 @@ struct StLegendreFwd {
     using T = TT;
     static constexpr int A_KIND = ROWS_OUT, B_KIND = ROWS_OUT;
     static constexpr bool B_PAIR = true;
-    const T* ptab; const T* spec; double* coeffs;
-    long spec_stride, coeff_stride;  // elements of T, doubles
+    const T* ptab; const T* spec; Inst<double> coeffs;
+    long spec_stride;  // elements of T
     int order, nt, jh, nb;
     struct Prob { int active, M, N, K, m, par; const T* A; long boff; };
     __device__ Prob problem(int prob, int) const {
@@ -353,7 +353,7 @@ struct StLegendreFwd {
     __device__ const T* b_row(const Prob& p, int b) const { return spec + (long)b * spec_stride + p.boff; }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         const int l = p.m + p.par + 2 * i;
-        coeffs[(long)(j >> 1) * coeff_stride + tri_idx(l, p.m) * 2 + (j & 1)] = (double)v;
+        coeffs.at(j >> 1)[tri_idx(l, p.m) * 2 + (j & 1)] = (double)v;
     }
 };
 
@@ -364,8 +364,8 @@ struct StLegendreBwd {
     using T = TT;
     static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_OUT;
     static constexpr bool B_PAIR = true;
-    const T* ptab; const double* coeffs; T* spec;
-    long spec_stride, coeff_stride;
+    const T* ptab; Inst<const double> coeffs; T* spec;
+    long spec_stride;
     int order, nt, jh, nb;
     struct Prob { int active, M, N, K, m, par; const T* A; long soff; };
     __device__ Prob problem(int prob, int) const {
@@ -389,7 +389,7 @@ struct StLegendreBwd {
         }
     };
     __device__ Gather b_row(const Prob& p, int b) const {
-        return Gather{coeffs + (long)b * coeff_stride, p.m, p.par};
+        return Gather{coeffs.at(b), p.m, p.par};
     }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         spec[(long)(j >> 1) * spec_stride + p.soff + 2 * i + (j & 1)] = v;
@@ -442,7 +442,7 @@ template <class T>
 void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s) {
     if (legendre_stream_enabled() && zt_radial_forward_stream<T>(a, s)) return;
     const Geom& g = a.t->g;
-    ZtRadialFwd<T> st{rtab_of<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2, a.coeff_stride,
+    ZtRadialFwd<T> st{rtab_of<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2,
                       a.nt, g.ncol, a.zernike_unnormed};
     launch<kNsRadial>(st, a.t->work_rf, a.t->n_rf, 2 * g.order, a.nb, s);
 }
@@ -451,7 +451,7 @@ template <class T>
 void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s) {
     if (legendre_stream_enabled() && zt_radial_backward_stream<T>(a, s)) return;
     const Geom& g = a.t->g;
-    ZtRadialBwd<T> st{rtab_of<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2, a.coeff_stride, a.nt,
+    ZtRadialBwd<T> st{rtab_of<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2, a.nt,
                       g.ncol};
     launch<kNsRadial>(st, a.t->work_rb, a.t->n_rb, g.ncol, a.nb, s);
 }
@@ -460,14 +460,14 @@ template <class T>
 void st_legendre_forward(const StGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     StLegendreFwd<T> st{ptab_of<T>(*a.t), (const T*)a.spec, a.coeffs, a.spec_stride * 2,
-                        a.coeff_stride, g.order, a.nt, g.jh, a.nb};
+                        g.order, a.nt, g.jh, a.nb};
     launch<kNsLegendreFwd>(st, a.t->work_lf, a.t->n_lf, 2 * a.nb, 1, s);
 }
 
 template <class T>
 void st_legendre_backward(const StGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
-    StLegendreBwd<T> st{ptab_of<T>(*a.t), a.coeffs, (T*)a.spec, a.spec_stride * 2, a.coeff_stride,
+    StLegendreBwd<T> st{ptab_of<T>(*a.t), a.coeffs, (T*)a.spec, a.spec_stride * 2,
                         g.order, a.nt, g.jh, a.nb};
     launch<kNsLegendreBwdSt>(st, a.t->work_lb, a.t->n_lb, 2 * a.nb, 1, s);
 }

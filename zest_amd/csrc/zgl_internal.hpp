@@ -104,6 +104,22 @@ void free_tables(OrderTables& t);
 void fft_plan_create(FftPlan& p, int n, hipStream_t s);
 void fft_plan_destroy(FftPlan& p);
 
+// Addressing of the instances of a batch: base + b * stride, or, for a
+// ragged batch (instances anywhere in memory), base + offs[b] with offs a
+// device array of element offsets from base.
+template <class P>
+struct Inst {
+    P* base = nullptr;
+    long stride = 0;
+    const long* offs = nullptr;
+    __host__ __device__ P* at(long b) const { return base + (offs ? offs[b] : b * stride); }
+    // the instances from b0 on
+    Inst from(long b0) const {
+        return offs ? Inst{base, stride, offs + b0} : Inst{base + b0 * stride, stride, nullptr};
+    }
+    operator Inst<const P>() const { return Inst<const P>{base, stride, offs}; }
+};
+
 enum GridLayoutKind { GRID_COLS = 0, GRID_ROWS = 1 };
 // GRID_COLS: sample i of latitude j, column c at i*lon_stride + j*lat_stride + c
 //            (Zernike LonLatRad and SH LonLat; the transform axis is the slow one).
@@ -125,11 +141,11 @@ struct FftLaunch {
 // float with R = float). spec holds complex R values and its stride counts
 // complex elements.
 template <class R, class G>
-void fft_forward(const FftLaunch& L, const G* grid, long grid_stride,
-                 void* spec, long spec_stride, int nb, hipStream_t s);
+void fft_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long spec_stride, int nb,
+                 hipStream_t s);
 template <class R, class G>
-void fft_backward(const FftLaunch& L, const void* spec, long spec_stride,
-                  G* grid, long grid_stride, int nb, hipStream_t s);
+void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, Inst<G> grid, int nb,
+                  hipStream_t s);
 
 // ---- contractions (zgl_gemm.hip) ----
 struct ZtGemmArgs {
@@ -138,7 +154,7 @@ struct ZtGemmArgs {
     int zernike_unnormed;   // scale coefficients by 2n+3
     void* spec; long spec_stride;   // complex elements of the compute type
     void* flm; long flm_stride;
-    double* coeffs; long coeff_stride;  // doubles
+    Inst<double> coeffs;    // strides in doubles
     int nb;
 };
 template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
@@ -160,7 +176,7 @@ struct StGemmArgs {
     const OrderTables* t;
     int nt;
     void* spec; long spec_stride;
-    double* coeffs; long coeff_stride;  // doubles
+    Inst<double> coeffs;    // strides in doubles
     int nb;
 };
 template <class T> void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
@@ -169,12 +185,12 @@ template <class T> void st_legendre_backward(const StGemmArgs& a, hipStream_t s)
 // ---- elementwise helpers (zgl_tables.hip) ----
 // r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in elements.
 template <class G>
-void grid_residual(const G* g, long gs, double* r, long rs, long n, int nb, hipStream_t s);
+void grid_residual(Inst<const G> g, double* r, long rs, long n, int nb, hipStream_t s);
 // c[b][i] += d[b][i] for i < n.
-void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s);
+void coeff_accumulate(Inst<double> c, const double* d, long ds, long n, int nb, hipStream_t s);
 // Zernike coefficients times kappa_n = 2n+3 (when `unnormed`), n < nt:
 // out[b][tet(n,l,m)] = kappa_n * c[b][tet(n,l,m)] (both parts).
-void coeff_kappa_scale(const double* c, long cs, double* out, long os, int nt, int unnormed,
-                       int nb, hipStream_t s);
+void coeff_kappa_scale(Inst<const double> c, double* out, long os, int nt, int unnormed, int nb,
+                       hipStream_t s);
 
 }  // namespace zgl

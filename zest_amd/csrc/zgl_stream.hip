@@ -356,8 +356,8 @@ template <> struct RadCfg<double> { static constexpr int KV = 1; };  // double2 
 
 template <class T>
 struct RadFwdArgs {
-    const T* rtab; const T* flm; double* coeffs;
-    long flm_stride, coeff_stride;  // elements of T, doubles
+    const T* rtab; const T* flm; Inst<double> coeffs;
+    long flm_stride;  // elements of T
     int nt, nr, unnormed, G, as;
 };
 
@@ -420,7 +420,7 @@ __device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* A
             }
         }
         if (ch != nchunks - 1) return;
-        double* cb = a.coeffs + inst * a.coeff_stride;
+        double* cb = a.coeffs.at(inst);
 #pragma unroll
         for (int q = 0; q < NSA; ++q) {
 #pragma unroll
@@ -493,8 +493,8 @@ constexpr int kRadBwdSteps = 16;  // i < 64
 
 template <class T>
 struct RadBwdArgs {
-    const T* rtab; const double* coeffs; T* flm;
-    long flm_stride, coeff_stride;
+    const T* rtab; Inst<const double> coeffs; T* flm;
+    long flm_stride;
     int nt, nr, G, rs;
 };
 
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdA
         const int inst = g0 + gi;
         const int mb = 16 * (t - gi * nmb);
         // A operand: lane (row j -> m = mb + j, group g -> i = 4s + g)
-        const double* cb = a.coeffs + inst * a.coeff_stride;
+        const double* cb = a.coeffs.at(inst);
         T xre[kRadBwdSteps], xim[kRadBwdSteps];
 #pragma unroll
         for (int s = 0; s < kRadBwdSteps; ++s) {
@@ -601,7 +601,7 @@ bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     constexpr int KI = 4 * RadCfg<T>::KV;
     const int kp = (g.ncol + KI - 1) / KI * KI;
-    RadFwdArgs<T> r{rtab_sel<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2, a.coeff_stride,
+    RadFwdArgs<T> r{rtab_sel<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2,
                     a.nt, g.ncol, a.zernike_unnormed, 1, kp + 2};
     const size_t lds = (size_t)16 * kNsRadial * r.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
@@ -626,7 +626,7 @@ bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const size_t lds = ((size_t)4 * ((kmax + 3) / 4) * rs + TNW) * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (a.nt == 0 || a.nb <= 0) return true;
-    RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2, a.coeff_stride,
+    RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
                     a.nt, g.ncol, a.nb < 4 ? a.nb : 4, rs};
     dim3 grid(a.nt, (a.nb + r.G - 1) / r.G);
     allow_lds(&rad_bwd_kernel<T>, lds);

@@ -244,15 +244,15 @@ __global__ void to_fp32_kernel(const double* in, float* out, long n) {
 }
 
 template <class G>
-__global__ void residual_kernel(const G* g, long gs, double* r, long rs, long n) {
-    const G* gb = g + blockIdx.y * gs;
+__global__ void residual_kernel(Inst<const G> g, double* r, long rs, long n) {
+    const G* gb = g.at(blockIdx.y);
     double* rb = r + blockIdx.y * rs;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         rb[i] = (double)gb[i] - rb[i];
 }
 
-__global__ void accumulate_kernel(double* c, long cs, const double* d, long ds, long n) {
-    double* cb = c + blockIdx.y * cs;
+__global__ void accumulate_kernel(Inst<double> c, const double* d, long ds, long n) {
+    double* cb = c.at(blockIdx.y);
     const double* db = d + blockIdx.y * ds;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         cb[i] += db[i];
@@ -279,38 +279,37 @@ int4* upload_work(const std::vector<int4>& w, hipStream_t s) {
 }  // namespace
 
 template <class G>
-void grid_residual(const G* g, long gs, double* r, long rs, long n, int nb, hipStream_t s) {
+void grid_residual(Inst<const G> g, double* r, long rs, long n, int nb, hipStream_t s) {
     if (n <= 0 || nb <= 0) return;
-    residual_kernel<G><<<dim3(grid_for(n), nb), 256, 0, s>>>(g, gs, r, rs, n);
+    residual_kernel<G><<<dim3(grid_for(n), nb), 256, 0, s>>>(g, r, rs, n);
     check_hip(hipGetLastError(), "residual launch");
 }
-template void grid_residual<double>(const double*, long, double*, long, long, int, hipStream_t);
-template void grid_residual<float>(const float*, long, double*, long, long, int, hipStream_t);
+template void grid_residual<double>(Inst<const double>, double*, long, long, int, hipStream_t);
+template void grid_residual<float>(Inst<const float>, double*, long, long, int, hipStream_t);
 
-void coeff_accumulate(double* c, long cs, const double* d, long ds, long n, int nb, hipStream_t s) {
+void coeff_accumulate(Inst<double> c, const double* d, long ds, long n, int nb, hipStream_t s) {
     if (n <= 0 || nb <= 0) return;
-    accumulate_kernel<<<dim3(grid_for(n), nb), 256, 0, s>>>(c, cs, d, ds, n);
+    accumulate_kernel<<<dim3(grid_for(n), nb), 256, 0, s>>>(c, d, ds, n);
     check_hip(hipGetLastError(), "accumulate launch");
 }
 
 namespace {
 // one block per (n, instance): the coefficients of radial degree n are the
 // contiguous range [tet(n,n%2,0), tet(n+1,(n+1)%2,0)) of pairs
-__global__ void kappa_kernel(const double* __restrict__ c, long cs, double* __restrict__ out, long os,
-                             int unnormed) {
+__global__ void kappa_kernel(Inst<const double> c, double* __restrict__ out, long os, int unnormed) {
     const int n = blockIdx.x;
     const long lo = tet_idx(n, n & 1, 0), hi = tet_idx(n + 1, (n + 1) & 1, 0);
     const double k = unnormed ? (double)(2 * n + 3) : 1.0;
-    const double* src = c + blockIdx.y * cs;
+    const double* src = c.at(blockIdx.y);
     double* dst = out + blockIdx.y * os;
     for (long e = 2 * lo + threadIdx.x; e < 2 * hi; e += blockDim.x) dst[e] = k * src[e];
 }
 }  // namespace
 
-void coeff_kappa_scale(const double* c, long cs, double* out, long os, int nt, int unnormed,
-                       int nb, hipStream_t s) {
+void coeff_kappa_scale(Inst<const double> c, double* out, long os, int nt, int unnormed, int nb,
+                       hipStream_t s) {
     if (nt <= 0 || nb <= 0) return;
-    kappa_kernel<<<dim3(nt, nb), 256, 0, s>>>(c, cs, out, os, unnormed);
+    kappa_kernel<<<dim3(nt, nb), 256, 0, s>>>(c, out, os, unnormed);
     check_hip(hipGetLastError(), "kappa scale launch");
 }
 
