@@ -141,13 +141,22 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy phase (profiling runs): the mixed forms then use the "
                          "plain FP32 forward and no accuracy is reported")
+    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad"],
+                    default="zernike128",
+                    help="zernike128: the headline (64 instances at order 128); ragged: 10,000 "
+                         "instances of orders U{8..96}; sh512: 4096 SH instances at L=512, "
+                         "precision report; grad: 256 least-squares gradients at order 64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the transforms are "
               "independent per instance, so each GPU would run this same workload on its own "
               "share of instances; no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
+    {"zernike128": run_headline, "ragged": run_ragged, "sh512": run_sh512,
+     "grad": run_grad}[args.config](args)
 
+
+def run_headline(args):
     import numpy as np
     import torch
 
@@ -283,6 +292,216 @@ def main():
             out["mixed_vs_fp64"] = round(value / out["fp64_value"], 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def _timed_steps(step, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def run_ragged(args):
+    """10,000 Zernike instances of orders N ~ U{8..96} (seed 2), coefficients
+    N(0,1)/(1+n): one ragged backward + forward per step through
+    zgl_backward_ragged / zgl_forward_ragged (order-sorted queue over four
+    internal streams, HIP graph captured on the first call, replayed after),
+    against the same work as per-order batched zgl_backward / zgl_forward calls
+    on one stream."""
+    import numpy as np
+    import torch
+    import zest_amd
+    from zest_amd import synth
+    dev = torch.device("cuda:0")
+    ninst = 10000
+    orders = (8 + np.floor(synth.uniform(2, ninst) * 89)).astype(int)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2)
+    groups = {}
+    for o in sorted(set(orders.tolist())):
+        ids = np.nonzero(orders == o)[0]
+        n, l, m = synth.zernike_nlm(o)
+        scale = torch.from_numpy(1.0 / (1.0 + n)).to(dev)
+        c = torch.randn((len(ids), len(n), 2), generator=gen, dtype=torch.float64, device=dev)
+        c *= scale[None, :, None]
+        c[:, torch.from_numpy(m == 0).to(dev), 1] = 0.0
+        g = torch.empty((len(ids),) + zest_amd.zt_shape(o), dtype=torch.float64, device=dev)
+        groups[o] = (ids, c, g, torch.zeros_like(c))
+    # jobs in instance order (not sorted): views into the per-order buffers
+    bwd, fwd = [None] * ninst, [None] * ninst
+    for o, (ids, c, g, co) in groups.items():
+        for k, i in enumerate(ids):
+            bwd[i] = (o, g[k], c[k])
+            fwd[i] = (o, g[k], co[k])
+    jb, jf = zest_amd.make_jobs(bwd), zest_amd.make_jobs(fwd)
+    points = sum(int(np.prod(zest_amd.zt_shape(o))) * len(v[0]) for o, v in groups.items())
+    out = {"metric": "Zernike ragged fwd+bwd transforms/s (10,000 instances, N~U{8..96})",
+           "unit": "transforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": "zernike_ragged_backward_forward", "instances": ninst,
+                      "orders": "U{8..96}, seed 2", "distinct_orders": len(groups),
+                      "grid_points": points, "conventions": "unnormed,geo,none"}}
+    stream = torch.cuda.Stream()
+    for name, prec in (("fp64", zest_amd.FP64), ("mixed", zest_amd.FP32)):
+        plan = zest_amd.Plan(zest_amd.ZERNIKE, 8, precision=prec)
+
+        def ragged_step():
+            plan.backward_ragged(jb, stream=stream)
+            plan.forward_ragged(jf, stream=stream)
+
+        def plain_step():
+            with torch.cuda.stream(stream):
+                for o, (ids, c, g, co) in groups.items():
+                    plan.backward(c, g, o, batch=len(ids), stream=stream)
+                    plan.forward(g, co, o, batch=len(ids), stream=stream)
+
+        t_rag = _timed_steps(ragged_step, args.steps, args.warmup)
+        costs = plan.ragged_lane_costs()
+        t_plain = _timed_steps(plain_step, args.steps, args.warmup)
+        out[name] = {"value": round(ninst * args.steps / t_rag, 2),
+                     "ms_per_step": round(t_rag / args.steps * 1e3, 3),
+                     "per_order_value": round(ninst * args.steps / t_plain, 2),
+                     "per_order_ms_per_step": round(t_plain / args.steps * 1e3, 3),
+                     "speedup_vs_per_order": round(t_plain / t_rag, 3),
+                     "lane_cost_max_over_mean": round(max(costs) / (sum(costs) / len(costs)), 4)}
+        del plan
+        torch.cuda.synchronize()
+    # accuracy of the mixed form on a sample: the FP64 grids forwarded in FP32
+    p64 = zest_amd.Plan(zest_amd.ZERNIKE, 8)
+    p32 = zest_amd.Plan(zest_amd.ZERNIKE, 8, precision=zest_amd.FP32)
+    worst = 0.0
+    for o in (8, 50, 96):
+        ids, c, g, co = groups[o]
+        p64.backward(c, g, o, batch=len(ids))
+        a, b = torch.zeros_like(c), torch.zeros_like(c)
+        p64.forward(g, a, o, batch=len(ids))
+        p32.forward(g, b, o, batch=len(ids))
+        torch.cuda.synchronize()
+        worst = max(worst, float(((b - a).abs().amax(dim=(1, 2)) / a.abs().amax(dim=(1, 2))).max()))
+    out["mixed"]["fwd_max_rel_err_vs_fp64"] = worst
+    out["value"] = out["fp64"]["value"]
+    out["ms_per_step"] = out["fp64"]["ms_per_step"]
+    out["dtype"] = "f64"
+    print(json.dumps(out), flush=True)
+
+
+def run_sh512(args):
+    """4096 spherical-harmonic expansions at L = 512 (LonLat grids 1023 x 512),
+    coefficients N(0,1) (1+l)^-1.5 (torch generator, seed 1); grids are the FP64
+    backward transforms. Reports the forward error of FP32 and FP32 + one
+    FP64-residual refinement against FP64 (max and RMS relative, per instance,
+    worst instance), and backward+forward pairs/s of each form."""
+    import numpy as np
+    import torch
+    import zest_amd
+    from zest_amd import synth
+    dev = torch.device("cuda:0")
+    L, B = 512, 4096
+    l, m = synth.sh_lm(L)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    c = torch.randn((B, len(l), 2), generator=gen, dtype=torch.float64, device=dev)
+    c *= torch.from_numpy((1.0 + l) ** -1.5).to(dev)[None, :, None]
+    c[:, torch.from_numpy(m == 0).to(dev), 1] = 0.0
+    g = torch.empty((B,) + zest_amd.st_shape(L), dtype=torch.float64, device=dev)
+    plans = {name: zest_amd.Plan(zest_amd.SH, L, precision=p)
+             for name, p in (("fp64", zest_amd.FP64), ("fp32", zest_amd.FP32),
+                             ("fp32+refine", zest_amd.FP32_REFINE))}
+    plans["fp64"].backward(c, g, L, batch=B)
+    c64 = torch.zeros_like(c)
+    plans["fp64"].forward(g, c64, L, batch=B)
+    out = {"metric": "SH L=512 fwd+bwd transforms/s and FP32 / FP32+refine error vs FP64",
+           "unit": "transforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": "sh_glq_backward_forward", "order": L, "instances": B,
+                      "grid": list(zest_amd.st_shape(L)), "conventions": "geo,none"}}
+    gw = torch.empty_like(g)
+    cw = torch.zeros_like(c)
+    for name, plan in plans.items():
+        cx = torch.zeros_like(c)
+        plan.forward(g, cx, L, batch=B)
+        torch.cuda.synchronize()
+        d = cx - c64
+        maxrel = float((d.abs().amax(dim=(1, 2)) / c64.abs().amax(dim=(1, 2))).max())
+        rms = float((d.square().sum(dim=(1, 2)).sqrt() / c64.square().sum(dim=(1, 2)).sqrt()).max())
+
+        def step():
+            plan.backward(c, gw, L, batch=B)
+            plan.forward(gw, cw, L, batch=B)
+
+        t = _timed_steps(step, args.steps, args.warmup)
+        out[name] = {"value": round(B * args.steps / t, 2), "ms_per_step": round(t / args.steps * 1e3, 3),
+                     "fwd_max_rel_err_vs_fp64": maxrel, "fwd_rms_rel_err_vs_fp64": rms}
+    out["value"] = out["fp32+refine"]["value"]
+    out["ms_per_step"] = out["fp32+refine"]["ms_per_step"]
+    out["dtype"] = "f32"
+    print(json.dumps(out), flush=True)
+
+
+def run_grad(args):
+    """256 least-squares fits at order 64 (seed 3): targets g = r exp(-(a.x)^2)
+    with a ~ U(0.5, 2)^3 on the ball grid; one step evaluates the loss
+    sum_x W_x (B c - g)_x^2 and its gradient 2 B^T (W (B c - g)) through the
+    hand-written transpose (zgl_backward_adjoint). The gradient is compared
+    with the closed form 2 diag(1/nu) F(B c - g)."""
+    import numpy as np
+    import torch
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    dev = torch.device("cuda:0")
+    N, B = 64, 256
+    nphi, nth, nr = zest_amd.zt_shape(N)
+    z, wz = oracle.gl_nodes_weights(nth)
+    x, wx = oracle.gl_nodes_weights(nr)
+    r = 0.5 * (1.0 + x)
+    phi = 2.0 * np.pi * np.arange(nphi) / nphi
+    st = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    X = torch.from_numpy(r[None, None, :] * st[None, :, None] * np.cos(phi)[:, None, None]).to(dev)
+    Y = torch.from_numpy(r[None, None, :] * st[None, :, None] * np.sin(phi)[:, None, None]).to(dev)
+    Z = torch.from_numpy(r[None, None, :] * z[None, :, None] * np.ones(nphi)[:, None, None]).to(dev)
+    R = torch.from_numpy(np.broadcast_to(r[None, None, :], (nphi, nth, nr)).copy()).to(dev)
+    a = torch.from_numpy(synth.uniform(3, 3 * B).reshape(B, 3) * 1.5 + 0.5).to(dev)
+    g = torch.stack([R * torch.exp(-(a[i, 0] * X + a[i, 1] * Y + a[i, 2] * Z) ** 2) for i in range(B)])
+    W = torch.from_numpy(wz[None, :, None] * (r * r * wx)[None, None, :] * np.ones(nphi)[:, None, None]).to(dev)
+    n, l, m = synth.zernike_nlm(N)
+    nu = torch.from_numpy((1.0 / (4.0 * np.pi)) * 0.5 * (2.0 * np.pi / nphi) * (2.0 * n + 3.0)).to(dev)
+    plan = zest_amd.Plan(zest_amd.ZERNIKE, N)
+    c = torch.zeros((B, len(n), 2), dtype=torch.float64, device=dev)
+    plan.forward(g, c, N, batch=B)  # start from the projection of the target
+    c *= 0.9
+    bc = torch.empty_like(g)
+    grad = torch.zeros_like(c)
+    loss = torch.empty(B, dtype=torch.float64, device=dev)
+
+    def step():
+        plan.backward(c, bc, N, batch=B)
+        res = bc - g
+        torch.sum((W * res * res).view(B, -1), dim=1, out=loss)
+        plan.backward_adjoint(W * res, grad, N, batch=B)
+        grad.mul_(2.0)
+
+    t = _timed_steps(step, args.steps, args.warmup)
+    closed = torch.zeros_like(c)
+    plan.forward(bc - g, closed, N, batch=B)
+    closed = 2.0 * closed / nu[None, :, None]
+    closed[:, torch.from_numpy(m == 0).to(dev), 1] = 0.0
+    torch.cuda.synchronize()
+    err = float(((grad - closed).abs().amax(dim=(1, 2)) / closed.abs().amax(dim=(1, 2))).max())
+    out = {"metric": "Zernike least-squares loss+gradient evaluations/s (256 fits, order 64)",
+           "value": round(B * args.steps / t, 2), "unit": "gradients/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "zernike_lsq_gradient", "order": N, "fits": B,
+                      "target": "r exp(-(a.x)^2), a ~ U(0.5,2)^3, seed 3"},
+           "gradient_max_rel_diff_vs_closed_form": err}
     print(json.dumps(out), flush=True)
 
 

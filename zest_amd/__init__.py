@@ -234,24 +234,19 @@ class Plan:
 
     @staticmethod
     def _jobs(jobs):
-        arr = (Job * max(1, len(jobs)))()
-        for i, (order, grid, coeffs) in enumerate(jobs):
-            for t in (grid, coeffs):
-                if t is not None and (not t.is_cuda or not t.is_contiguous() or t.dtype.itemsize != 8):
-                    raise ValueError("ragged jobs take contiguous float64 device tensors")
-            arr[i].order = order
-            arr[i].grid = grid.data_ptr() if grid is not None else None
-            arr[i].coeffs = coeffs.data_ptr() if coeffs is not None else None
-        return arr
+        if isinstance(jobs, ctypes.Array):
+            return jobs
+        return make_jobs(jobs)
 
     def forward_ragged(self, jobs, stream=None):
-        """Forward transforms of [(order, grid, coeffs), ...] of uneven orders
-        (zgl_forward_ragged)."""
+        """Forward transforms of [(order, grid, coeffs), ...] of uneven orders, or
+        of an array from make_jobs (zgl_forward_ragged)."""
         arr = self._jobs(jobs)
         _check(LIB.zgl_forward_ragged(self._h, arr, len(jobs), _stream_ptr(stream, MEM_DEVICE)))
 
     def backward_ragged(self, jobs, stream=None):
-        """Backward transforms of [(order, grid, coeffs), ...] (zgl_backward_ragged)."""
+        """Backward transforms of [(order, grid, coeffs), ...] or of an array from
+        make_jobs (zgl_backward_ragged)."""
         arr = self._jobs(jobs)
         _check(LIB.zgl_backward_ragged(self._h, arr, len(jobs), _stream_ptr(stream, MEM_DEVICE)))
 
@@ -285,6 +280,20 @@ class Plan:
             raise ValueError("grid and coefficients must live in the same memory")
         _check(LIB.zgl_forward_adjoint(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                                        grid_stride, gm, _stream_ptr(stream, gm)))
+
+
+def make_jobs(jobs):
+    """zgl_job array of [(order, grid, coeffs), ...] with contiguous float64
+    device tensors (None for order 0)."""
+    arr = (Job * max(1, len(jobs)))()
+    for i, (order, grid, coeffs) in enumerate(jobs):
+        for t in (grid, coeffs):
+            if t is not None and (not t.is_cuda or not t.is_contiguous() or t.dtype.itemsize != 8):
+                raise ValueError("ragged jobs take contiguous float64 device tensors")
+        arr[i].order = order
+        arr[i].grid = grid.data_ptr() if grid is not None else None
+        arr[i].coeffs = coeffs.data_ptr() if coeffs is not None else None
+    return arr
 
 
 def _stream_ptr(stream, mem):

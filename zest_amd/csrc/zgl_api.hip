@@ -87,13 +87,16 @@ struct zgl_plan {
         std::vector<hipStream_t> lanes;   // internal streams, default priority
         std::vector<zgl::Scratch> scratch;  // one per lane
         std::vector<hipEvent_t> ev;       // fork, then one join event per lane
-        // the last job list and its captured pipeline
-        std::vector<zgl_job> jobs;
-        int dir = -1;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        long* offs = nullptr;  // device: grid offsets [njobs], coefficient offsets [njobs]
-        size_t offs_cap = 0;   // longs
+        // per direction (0 backward, 1 forward): the last job list and its
+        // captured pipeline; the graphs hold the lane scratch addresses
+        struct Cached {
+            std::vector<zgl_job> jobs;
+            bool valid = false;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            long* offs = nullptr;  // device: grid offsets [njobs], coefficient offsets [njobs]
+            size_t offs_cap = 0;   // longs
+        } cache[2];
         std::vector<double> lane_cost;  // modelled work per lane of the last build
     } rg;
 };
@@ -128,9 +131,11 @@ const OrderTables& tables_for(zgl_plan* p, size_t order) {
     return p->tables.emplace(order, t).first->second;
 }
 
+// Growing a buffer frees the old one: wait for queued work that may use it.
 template <class T>
 void ensure(T*& ptr, size_t& cap, size_t need) {
     if (need <= cap) return;
+    if (ptr) check_hip(hipDeviceSynchronize(), "scratch growth");
     if (ptr) check_hip(hipFree(ptr), "free scratch");
     ptr = nullptr;
     cap = 0;
@@ -139,19 +144,22 @@ void ensure(T*& ptr, size_t& cap, size_t need) {
 }
 
 void ragged_release(zgl_plan::Ragged& r) {
-    if (r.exec) (void)hipGraphExecDestroy(r.exec);
-    if (r.graph) (void)hipGraphDestroy(r.graph);
+    for (auto& c : r.cache) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+        if (c.offs) (void)hipFree(c.offs);
+    }
     for (auto l : r.lanes)
         if (l) (void)hipStreamDestroy(l);
     for (auto e : r.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& sc : r.scratch) sc.release();
-    if (r.offs) (void)hipFree(r.offs);
     r = zgl_plan::Ragged{};
 }
 
 void ensure_bytes(void*& ptr, size_t& cap, size_t need) {
     if (need <= cap) return;
+    if (ptr) check_hip(hipDeviceSynchronize(), "scratch growth");
     if (ptr) check_hip(hipFree(ptr), "free scratch");
     ptr = nullptr;
     cap = 0;
@@ -604,13 +612,13 @@ double job_cost(size_t N, size_t count) {
     return (double)count * n * n * n;
 }
 
-void drop_graph(zgl_plan::Ragged& r) {
-    if (r.exec) (void)hipGraphExecDestroy(r.exec);
-    if (r.graph) (void)hipGraphDestroy(r.graph);
-    r.exec = nullptr;
-    r.graph = nullptr;
-    r.jobs.clear();
-    r.dir = -1;
+void drop_graph(zgl_plan::Ragged::Cached& c) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.exec = nullptr;
+    c.graph = nullptr;
+    c.jobs.clear();
+    c.valid = false;
 }
 
 bool same_jobs(const std::vector<zgl_job>& a, const zgl_job* b, size_t n) {
@@ -620,9 +628,11 @@ bool same_jobs(const std::vector<zgl_job>& a, const zgl_job* b, size_t n) {
     return true;
 }
 
-// scratch for `count` instances of order N in the plan's precision
-void reserve(zgl_plan* p, Scratch& sc, size_t N, size_t count) {
+// scratch for `count` instances of order N in the plan's precision; true
+// when a buffer was (re)allocated
+bool reserve(zgl_plan* p, Scratch& sc, size_t N, size_t count) {
     const Geom& g = tables_for(p, N).g;
+    const Scratch before = sc;
     for (size_t csize : {sizeof(double2), sizeof(float2)}) {
         if (csize == sizeof(double2) && p->precision == ZGL_FP32) continue;
         if (csize == sizeof(float2) && p->precision == ZGL_FP64) continue;
@@ -635,6 +645,8 @@ void reserve(zgl_plan* p, Scratch& sc, size_t N, size_t count) {
         ensure(sc.rgrid, sc.rgrid_cap, grid_size(p, N) * ch.nb);
         ensure(sc.rcoef, sc.rcoef_cap, 2 * coeff_size(p, N) * ch.nb);
     }
+    return sc.spec != before.spec || sc.flm != before.flm || sc.rgrid != before.rgrid ||
+           sc.rcoef != before.rcoef;
 }
 
 zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void* stream, bool forward) {
@@ -648,12 +660,13 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
     return guarded([&]() -> zgl_status {
         hipStream_t s = static_cast<hipStream_t>(stream);
         auto& R = plan->rg;
+        auto& C = R.cache[forward ? 1 : 0];
         const bool use_graph = s != nullptr && !plan->profiling && ragged_graphs();
-        if (use_graph && R.exec && R.dir == (int)forward && same_jobs(R.jobs, jobs, njobs)) {
-            check_hip(hipGraphLaunch(R.exec, s), "ragged graph launch");
+        if (use_graph && C.valid && same_jobs(C.jobs, jobs, njobs)) {
+            check_hip(hipGraphLaunch(C.exec, s), "ragged graph launch");
             return ZGL_SUCCESS;
         }
-        drop_graph(R);
+        drop_graph(C);
 
         // order-sorted queue of non-empty jobs, largest order first
         std::vector<size_t> idx;
@@ -683,8 +696,8 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
             offs[k] = (long)(jobs[idx[k]].grid - gbase);
             offs[n + k] = (long)(jobs[idx[k]].coeffs - cbase);
         }
-        ensure(R.offs, R.offs_cap, 2 * n);
-        check_hip(hipMemcpy(R.offs, offs.data(), 2 * n * sizeof(long), hipMemcpyHostToDevice),
+        ensure(C.offs, C.offs_cap, 2 * n);
+        check_hip(hipMemcpy(C.offs, offs.data(), 2 * n * sizeof(long), hipMemcpyHostToDevice),
                   "ragged offsets upload");
 
         // lanes: streams, events, scratch; groups dealt by longest processing time
@@ -696,13 +709,16 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
             for (auto& e : R.ev) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
         }
         R.lane_cost.assign(kLanes, 0.0);
+        bool moved = false;
         for (auto& g : groups) {
             tables_for(plan, g.order);
             const int l = (int)(std::min_element(R.lane_cost.begin(), R.lane_cost.end()) - R.lane_cost.begin());
             g.lane = l;
             R.lane_cost[l] += job_cost(g.order, g.count);
-            reserve(plan, R.scratch[l], g.order, g.count);
+            moved |= reserve(plan, R.scratch[l], g.order, g.count);
         }
+        // the other direction's graph points at the old scratch
+        if (moved) drop_graph(R.cache[forward ? 0 : 1]);
 
         auto enqueue = [&]() {
             check_hip(hipEventRecord(R.ev[0], s), "fork");
@@ -710,8 +726,8 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
             for (const auto& g : groups) {
                 hipStream_t ls = R.lanes[g.lane];
                 Scratch& sc = R.scratch[g.lane];
-                const long* go = R.offs + g.first;
-                const long* co = R.offs + n + g.first;
+                const long* go = C.offs + g.first;
+                const long* co = C.offs + n + g.first;
                 if (forward)
                     forward_any<double>(plan, sc, Inst<const double>{gbase, 0, go}, g.order,
                                         Inst<double>{cbase, 0, co}, g.order, g.count, ls);
@@ -739,11 +755,11 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
-        check_hip(hipStreamEndCapture(s, &R.graph), "end capture");
-        check_hip(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0), "graph instantiate");
-        R.jobs.assign(jobs, jobs + njobs);
-        R.dir = (int)forward;
-        check_hip(hipGraphLaunch(R.exec, s), "ragged graph launch");
+        check_hip(hipStreamEndCapture(s, &C.graph), "end capture");
+        check_hip(hipGraphInstantiate(&C.exec, C.graph, nullptr, nullptr, 0), "graph instantiate");
+        C.jobs.assign(jobs, jobs + njobs);
+        C.valid = true;
+        check_hip(hipGraphLaunch(C.exec, s), "ragged graph launch");
         return ZGL_SUCCESS;
     });
 }
