@@ -422,7 +422,8 @@ bool legendre_stream_enabled() {
 
 template <class T>
 void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s) {
-    if (legendre_stream_enabled() && zt_legendre_forward_stream<T>(a, s)) return;
+    if ((legendre_stream_enabled() || a.t->plm_in_kernel) && zt_legendre_forward_stream<T>(a, s)) return;
+    if (a.t->plm_in_kernel) throw Error{ZGL_ERROR_NOT_SUPPORTED, "P_lm recomputation needs the streaming kernels"};
     const Geom& g = a.t->g;
     ZtLegendreFwd<T> st{ptab_of<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
                         a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol};
@@ -431,7 +432,8 @@ void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s) {
 
 template <class T>
 void zt_legendre_backward(const ZtGemmArgs& a, hipStream_t s) {
-    if (legendre_stream_enabled() && zt_legendre_backward_stream<T>(a, s)) return;
+    if ((legendre_stream_enabled() || a.t->plm_in_kernel) && zt_legendre_backward_stream<T>(a, s)) return;
+    if (a.t->plm_in_kernel) throw Error{ZGL_ERROR_NOT_SUPPORTED, "P_lm recomputation needs the streaming kernels"};
     const Geom& g = a.t->g;
     ZtLegendreBwd<T> st{ptab_of<T>(*a.t), (const T*)a.flm, (T*)a.spec, a.spec_stride * 2,
                         a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol};

@@ -57,7 +57,12 @@ struct OrderTables {
     Geom g;
     double* wlat = nullptr;  // [jh] weights of the north-half latitude nodes
     double* wrad = nullptr;  // [ncol] r^2 v of the radial nodes (Zernike)
-    double* ptab = nullptr;  // P_lm(z_h), parity-split [m][par][l'][h]
+    double* ptab = nullptr;  // P_lm(z_h), parity-split [m][par][l'][h]; null when the
+                             // Legendre kernels recompute their P blocks (plm_in_kernel)
+    double* zh = nullptr;    // [jh] north-half latitude nodes z_h
+    int plm_qm = 0;          // P_lm conventions: qm normalisation, phase (+1 / -1)
+    double plm_phase = -1.0;
+    bool plm_in_kernel = false;
     double* rtab = nullptr;  // R_nl(r_k), zest layout [((n+1)^2>>2)+(l>>1)][nr]
     float* ptab32 = nullptr; // FP32 roundings of the two tables (FP32 modes)
     float* rtab32 = nullptr;
@@ -171,6 +176,9 @@ template <class T> bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream
 // ZGL_LEGENDRE_ENGINE=tiled selects the tiled LDS engine for every order and
 // stage.
 bool legendre_stream_enabled();
+// Whether both streaming Legendre kernels run at this order in FP64 (their P
+// blocks fit the LDS budget), which in-kernel P_lm recomputation requires.
+bool stream_legendre_fits(int order, int jh);
 
 struct StGemmArgs {
     const OrderTables* t;

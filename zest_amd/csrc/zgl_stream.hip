@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include "zgl_internal.hpp"
+#include "zgl_plm.hpp"
 
 namespace zgl {
 
@@ -69,6 +70,7 @@ struct LegFwd {
     const T* ptab; const T* spec; T* flm;
     long spec_stride, flm_stride;  // elements of T
     int order, nt, jh, ncol, G, as;
+    const double* zh; int qm; double phase;  // for P blocks recomputed in the kernel (ptab null)
     static constexpr bool kCenter = false;
     __device__ int center_lds_row() const { return 16 * kNsLegendreFwd; }
     struct Prob {
@@ -80,7 +82,7 @@ struct LegFwd {
         p.m = prob >> 1; p.par = prob & 1;
         p.M = plm_count(nt, p.m, p.par); p.K = jh; p.N = 2 * ncol;
         p.center = false; p.Ac = nullptr; p.hoff = 0;
-        p.A = ptab + plm_offset(order, p.m, p.par, jh) + (long)m0 * jh;
+        p.A = ptab ? ptab + plm_offset(order, p.m, p.par, jh) + (long)m0 * jh : nullptr;
         p.a_si = jh; p.a_sk = 1;
         p.boff = (long)(2 * p.m + p.par) * jh * 2 * ncol;
         return p;
@@ -91,6 +93,18 @@ struct LegFwd {
     __device__ T* c_base(int inst) const { return flm + inst * flm_stride; }
     __device__ unsigned c_off(const Prob& p, int i) const {
         return (unsigned)(tri_idx(p.m + p.par + 2 * i, p.m) * 2 * ncol);
+    }
+    // A block by recursion (zeroed beforehand): As[i][h] = P_l(z_h),
+    // l = m + par + 2 (m0 + i), one thread per half latitude h
+    __device__ void plm_fill(T* As, int AS, const Prob& p, int m0, int rows, int TM) const {
+        for (int h = threadIdx.x; h < p.K; h += blockDim.x)
+            plm_column(order, p.m, zh[h], qm, phase, [&](int l, double v) {
+                const int d = l - p.m - p.par;
+                if (d >= 0 && !(d & 1)) {
+                    const int i = (d >> 1) - m0;
+                    if (i >= 0 && i < rows) As[i * AS + h] = (T)v;
+                }
+            });
     }
 };
 
@@ -107,6 +121,7 @@ struct LegBwd {
     const T* ptab; const T* flm; T* spec;
     long spec_stride, flm_stride;
     int order, nt, jh, ncol, G, as, nth;
+    const double* zh; int qm; double phase;  // for P blocks recomputed in the kernel (ptab null)
     static constexpr bool kCenter = true;
     __device__ int center_lds_row() const { return 16 * kNsStreamBwd; }
     struct Prob {
@@ -119,8 +134,8 @@ struct LegBwd {
         p.M = p.m < nt ? nth / 2 : 0;
         p.K = plm_count(nt, p.m, p.par); p.N = 2 * ncol;
         p.hoff = nth & 1;
-        const T* a0 = ptab + plm_offset(order, p.m, p.par, jh);  // [i][h]
-        p.A = a0 + p.hoff + m0;
+        const T* a0 = ptab ? ptab + plm_offset(order, p.m, p.par, jh) : nullptr;  // [i][h]
+        p.A = a0 ? a0 + p.hoff + m0 : nullptr;
         p.a_si = 1; p.a_sk = jh;
         p.boff = (long)(2 * p.m + p.par) * jh * 2 * ncol;
         p.center = p.hoff && p.par == 0 && m0 == 0;
@@ -135,6 +150,19 @@ struct LegBwd {
     // output row r is half latitude h = r + hoff (r = -hoff: the equator)
     __device__ unsigned c_off(const Prob& p, int r) const {
         return (unsigned)(p.boff + (r + p.hoff) * 2 * ncol);
+    }
+    // A block by recursion (zeroed beforehand): As[r][k] = P_l(z_h), h = hoff +
+    // m0 + r, l = m + par + 2k; the equator row (P.center) goes to As row TM
+    __device__ void plm_fill(T* As, int AS, const Prob& p, int m0, int rows, int TM) const {
+        const int nrow = rows + (p.center ? 1 : 0);
+        for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+            const int h = r < rows ? p.hoff + m0 + r : 0;
+            T* dst = As + (long)(r < rows ? r : TM) * AS;
+            plm_column(order, p.m, zh[h], qm, phase, [&](int l, double v) {
+                const int d = l - p.m - p.par;
+                if (d >= 0 && !(d & 1) && (d >> 1) < p.K) dst[d >> 1] = (T)v;
+            });
+        }
     }
 };
 
@@ -257,7 +285,8 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
     }
 }
 
-template <class S, int NS>
+// PLM: the P block is recomputed in the workgroup (no table)
+template <class S, int NS, bool PLM>
 __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb) {
     using T = typename S::T;
     constexpr int TM = 16 * NS;
@@ -279,8 +308,14 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
     const int AS = st.as;
 
     // A tile: TM rows x KP, zero padded, As[i * AS + k]; the equator row
-    // (P.center) goes to As row TM.
-    if (P.a_sk == 1) {
+    // (P.center) goes to As row TM. Without a table the block is recomputed.
+    const bool center = S::kCenter && P.center;
+    if constexpr (PLM) {
+        const int nfill = (TM + (center ? 1 : 0)) * AS;
+        for (int e = tid; e < nfill; e += kThreads) As[e] = T(0);
+        __syncthreads();
+        st.plm_fill(As, AS, P, m0, rows, TM);
+    } else if (P.a_sk == 1) {
         for (int e = tid; e < TM * KP; e += kThreads) {
             const int i = e / KP, k = e - i * KP;
             As[i * AS + k] = (i < rows && k < K) ? P.A[(long)i * P.a_si + k] : T(0);
@@ -291,9 +326,9 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
             As[i * AS + k] = (i < rows && k < K) ? P.A[(long)k * P.a_sk + i] : T(0);
         }
     }
-    const bool center = S::kCenter && P.center;
-    if (center)
-        for (int k = tid; k < KP; k += kThreads) As[TM * AS + k] = k < K ? P.Ac[(long)k * P.a_sk] : T(0);
+    if constexpr (!PLM)
+        if (center)
+            for (int k = tid; k < KP; k += kThreads) As[TM * AS + k] = k < K ? P.Ac[(long)k * P.a_sk] : T(0);
     __syncthreads();
 
     const int nstrips = (rows + 15) >> 4;
@@ -332,8 +367,13 @@ bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, hipStrea
     if (nwork == 0 || nb <= 0) return true;
     st.G = nb < 4 ? nb : 4;
     dim3 grid(nwork, (nb + st.G - 1) / st.G);
-    allow_lds(&stream_kernel<S, NS>, lds);
-    stream_kernel<S, NS><<<grid, kThreads, lds, s>>>(st, work, nb);
+    if (st.ptab) {
+        allow_lds(&stream_kernel<S, NS, false>, lds);
+        stream_kernel<S, NS, false><<<grid, kThreads, lds, s>>>(st, work, nb);
+    } else {
+        allow_lds(&stream_kernel<S, NS, true>, lds);
+        stream_kernel<S, NS, true><<<grid, kThreads, lds, s>>>(st, work, nb);
+    }
     check_hip(hipGetLastError(), "stream launch");
     return true;
 }
@@ -589,6 +629,11 @@ const T* ptab_sel(const OrderTables& t);
 template <> const double* ptab_sel<double>(const OrderTables& t) { return t.ptab; }
 template <> const float* ptab_sel<float>(const OrderTables& t) { return t.ptab32; }
 
+size_t stream_lds(int rows_tm, bool center, int kmax, size_t tsize) {
+    const int kp = (kmax + 3) & ~3;
+    return (size_t)(rows_tm + (center ? 1 : 0)) * (kp + 1) * tsize;
+}
+
 }  // namespace
 
 template <class T>
@@ -639,7 +684,8 @@ template <class T>
 bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
-                 a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0};
+                 a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0,
+                 a.t->zh, a.t->plm_qm, a.t->plm_phase};
     return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, s);
 }
 
@@ -647,9 +693,15 @@ template <class T>
 bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     LegBwd<T> st{ptab_sel<T>(*a.t), (const T*)a.flm, (T*)a.spec, a.spec_stride * 2,
-                 a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0, g.nth};
+                 a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0, g.nth,
+                 a.t->zh, a.t->plm_qm, a.t->plm_phase};
     return launch_stream<kNsStreamBwd>(st, plm_count(g.order, 0, 0), a.t->work_sb, a.t->n_sb,
                                        a.nb, s);
+}
+
+bool stream_legendre_fits(int order, int jh) {
+    return stream_lds(16 * kNsLegendreFwd, false, jh, sizeof(double)) <= kMaxStreamLds &&
+           stream_lds(16 * kNsStreamBwd, true, plm_count(order, 0, 0), sizeof(double)) <= kMaxStreamLds;
 }
 
 template bool zt_legendre_forward_stream<double>(const ZtGemmArgs&, hipStream_t);

@@ -5,9 +5,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "zgl_internal.hpp"
+#include "zgl_plm.hpp"
 
 namespace zgl {
 
@@ -132,66 +135,19 @@ __global__ void radial_nodes_kernel(int nr, double* x, const double* v, double* 
     wr[k] = r * r * v[k];
 }
 
-// P_lm(z_h) for one (h, m): the column recursion of st::PlmRecursion::plm_impl
-// (include/plm_recursion.hpp:229-354) with its coefficients
-// (src/plm_recursion.cpp:181-247) and its 1e-280 underflow scaling, real
-// normalisation sqrt(2) for m > 0. Output in the parity-split layout
-// [m][par][l'][h], l = m + par + 2 l'.
+// P_lm(z_h) for one (h, m) (plm_column, zgl_plm.hpp), in the parity-split
+// layout [m][par][l'][h], l = m + par + 2 l'.
 __global__ void plm_kernel(int order, int jh, int qm, double phase, const double* zh,
                            double* ptab) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int h = gid % jh;
     const int m = gid / jh;
     if (m >= order) return;
-    const double inv_sqrt_4pi = 0.5 / sqrt(M_PI);
-    const double z = zh[h];
-    auto put = [&](int l, double v) {
+    plm_column(order, m, zh[h], qm, phase, [&](int l, double v) {
         const int par = (l - m) & 1;
         const int lp = (l - m - par) >> 1;
         ptab[plm_offset(order, m, par, jh) + (long)lp * jh + h] = v;
-    };
-    if (m == 0) {
-        double p2 = qm ? inv_sqrt_4pi : 1.0;
-        put(0, p2);
-        if (order == 1) return;
-        double p1 = qm ? z * (sqrl(3) * inv_sqrt_4pi) : z * sqrl(3);
-        put(1, p1);
-        for (int l = 2; l < order; ++l) {
-            const double a = sqrl(2 * l - 1) * sqrl(2 * l + 1) / (double)l;
-            const double b = ((double)l - 1.0) * sqrl(2 * l + 1) / (sqrl(2 * l - 3) * (double)l);
-            const double p = a * z * p1 - b * p2;
-            put(l, p);
-            p2 = p1;
-            p1 = p;
-        }
-        return;
-    }
-    const double u = sqrt((1.0 - z) * (1.0 + z));
-    double pmm = qm ? 1.0e-280 * sqrt(2.0) * inv_sqrt_4pi : 1.0e-280 * sqrt(2.0);
-    double us = 1.0 / 1.0e-280;
-    const int msteps = m < order - 1 ? m : order - 2;
-    for (int i = 1; i <= msteps; ++i) {
-        us *= u;
-        pmm *= phase * sqrl(2 * i + 1) / sqrl(2 * i);
-    }
-    if (m == order - 1) {
-        us *= u;
-        put(m, us * (phase * pmm * sqrl(2 * order - 1) / sqrl(2 * order - 2)));
-        return;
-    }
-    double p2 = pmm;
-    double p1 = z * (sqrl(2 * m + 3) * pmm);
-    put(m, p2 * us);
-    put(m + 1, p1 * us);
-    for (int l = m + 2; l < order; ++l) {
-        const double a = sqrl(2 * l - 1) * sqrl(2 * l + 1) / (sqrl(l - m) * sqrl(l + m));
-        const double b = sqrl(2 * l + 1) * sqrl(l + m - 1) * sqrl(l - m - 1) /
-                         (sqrl(l - m) * sqrl(l + m) * sqrl(2 * l - 3));
-        const double p = z * a * p1 - b * p2;
-        put(l, p * us);
-        p2 = p1;
-        p1 = p;
-    }
+    });
 }
 
 // R_nl(r_k) for one radial node: the recursion of
@@ -319,10 +275,12 @@ static long ptab_size(const Geom& g) {
 }
 
 void ensure_fp32_tables(OrderTables& t, hipStream_t s) {
-    if (t.g.order == 0 || t.ptab32) return;
-    const long np = ptab_size(t.g);
-    t.ptab32 = dev_alloc<float>(np);
-    to_fp32_kernel<<<grid_for(np), 256, 0, s>>>(t.ptab, t.ptab32, np);
+    if (t.g.order == 0 || t.ptab32 || t.rtab32) return;
+    if (t.ptab) {
+        const long np = ptab_size(t.g);
+        t.ptab32 = dev_alloc<float>(np);
+        to_fp32_kernel<<<grid_for(np), 256, 0, s>>>(t.ptab, t.ptab32, np);
+    }
     if (t.rtab) {
         const long nr = (long)((t.g.order + 1) * (t.g.order + 1) / 4) * t.g.ncol;
         t.rtab32 = dev_alloc<float>(nr);
@@ -368,12 +326,26 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
     t.wlat = dev_alloc<double>(g.jh);
     half_nodes_kernel<<<g.jh / 64 + 1, 64, 0, s>>>(g.nth, z, w, zh, t.wlat);
 
-    // P table.
-    t.ptab = dev_alloc<double>(ptab_size(g));
+    // P table, unless the Legendre kernels recompute their P blocks: at high
+    // order (the table is O(N^3): 67.6 MB at N = 256 in FP64) when the
+    // streaming kernels run; ZGL_PLM=kernel / table forces either.
+    t.zh = zh;
+    t.plm_qm = sh_norm == ZGL_SHNORM_QM ? 1 : 0;
+    t.plm_phase = (double)sh_phase;
     {
+        const char* e = std::getenv("ZGL_PLM");
+        const bool can = kind == ZGL_ZERNIKE && legendre_stream_enabled() &&
+                         stream_legendre_fits(order, g.jh);
+        const bool want = e && std::strcmp(e, "kernel") == 0 ? true
+                        : e && std::strcmp(e, "table") == 0  ? false
+                                                             : order >= 192;
+        t.plm_in_kernel = can && want;
+    }
+    if (!t.plm_in_kernel) {
+        t.ptab = dev_alloc<double>(ptab_size(g));
         const long threads = (long)order * g.jh;
-        plm_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(
-            order, g.jh, sh_norm == ZGL_SHNORM_QM ? 1 : 0, (double)sh_phase, zh, t.ptab);
+        plm_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(order, g.jh, t.plm_qm,
+                                                                    t.plm_phase, zh, t.ptab);
     }
 
     if (kind == ZGL_ZERNIKE) {
@@ -396,7 +368,6 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
     check_hip(hipStreamSynchronize(s), "table generation");
     check_hip(hipFree(z), "free");
     check_hip(hipFree(w), "free");
-    check_hip(hipFree(zh), "free");
 
     fft_plan_create(t.fft, g.nphi, s);
 
@@ -433,7 +404,7 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
 }
 
 void free_tables(OrderTables& t) {
-    for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.rtab, (void*)t.ptab32,
+    for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.zh, (void*)t.rtab, (void*)t.ptab32,
                     (void*)t.rtab32, (void*)t.work_lf, (void*)t.work_lb, (void*)t.work_rf,
                     (void*)t.work_rb, (void*)t.work_sb})
         if (p) (void)hipFree(p);
