@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["fp64", "mixed", "mixed_grid32"], default=None,
                     help="time one form only (profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile shapes (no timing of candidates on first use)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy phase (profiling runs): the mixed forms then use the "
                          "plain FP32 forward and no accuracy is reported")
@@ -174,6 +176,9 @@ def run_headline(args):
     p64 = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP64, **conv)
     p32 = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP32, **conv)
     p32r = zest_amd.Plan(zest_amd.ZERNIKE, N, precision=zest_amd.FP32_REFINE, **conv)
+    if not args.no_autotune:
+        for p in (p64, p32, p32r):
+            p.autotune(True)
 
     def relerr(a, ref):
         dims = tuple(range(1, ref.dim()))
@@ -275,6 +280,7 @@ def run_headline(args):
                      "kernel": dom_name},
         "accuracy": acc,
         "stages": stages,
+        "tuning": {"fp64": [list(t) for t in p64.tuning()], "fp32": [list(t) for t in p32.tuning()]},
     }
     for form in mixed:
         if form == head:

@@ -195,6 +195,19 @@ zgl_status zgl_backward_grid32(zgl_plan *plan, const double *coeffs, size_t exp_
                                ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
                                zgl_mem mem, void *stream);
 
+/* Autotuning. No counterpart in the reference (zest has nothing to tune).
+ * While enabled (also: environment ZGL_AUTOTUNE=1 at plan creation), the
+ * first call of each stage at a new order and arithmetic times the candidate
+ * tile shapes on that call's own operands -- the FFT's column pairs per
+ * workgroup, the streaming contractions' instances per workgroup -- and keeps
+ * the fastest for later calls. Results do not depend on the choice.
+ * zgl_plan_tuning lists the choices made: 5 ints per entry (order, bytes per
+ * real of the arithmetic, bytes per grid value, stage index as in
+ * zgl_profile_stats, choice). */
+zgl_status zgl_plan_autotune(zgl_plan *plan, int enable);
+zgl_status zgl_plan_tuning(const zgl_plan *plan, int *entries, int max_entries,
+                           int *num_entries);
+
 /* Stage timing. No counterpart in the reference: it serves benchmarks and
  * profilers. While enabled, every kernel launch of the plan is bracketed by
  * HIP events on the launch stream; zgl_profile_stats() waits for them and

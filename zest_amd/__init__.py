@@ -81,6 +81,10 @@ def _load():
     lib.zgl_backward_grid32.restype = i32
     lib.zgl_last_error_string.argtypes = []
     lib.zgl_last_error_string.restype = ctypes.c_char_p
+    lib.zgl_plan_autotune.argtypes = [vp, i32]
+    lib.zgl_plan_autotune.restype = i32
+    lib.zgl_plan_tuning.argtypes = [vp, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]
+    lib.zgl_plan_tuning.restype = i32
     lib.zgl_profile_enable.argtypes = [vp, i32]
     lib.zgl_profile_enable.restype = i32
     lib.zgl_profile_stats.argtypes = [vp, ctypes.POINTER(StageStats), i32, ctypes.POINTER(i32)]
@@ -107,6 +111,7 @@ EXPORTS = (
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
     "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
     "zgl_forward_ragged", "zgl_backward_ragged", "zgl_ragged_lane_costs",
+    "zgl_plan_autotune", "zgl_plan_tuning",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -189,6 +194,22 @@ class Plan:
 
     def coeff_size(self, order: int) -> int:
         return LIB.zgl_coeff_size(self._h, order)
+
+    def autotune(self, enable: bool = True):
+        """Time the candidate tile shapes on first use of each order (zgl_plan_autotune)."""
+        _check(LIB.zgl_plan_autotune(self._h, int(enable)))
+
+    def tuning(self) -> list:
+        """[(order, bytes per real, bytes per grid value, stage name, choice), ...]
+        (zgl_plan_tuning)."""
+        n = ctypes.c_int()
+        _check(LIB.zgl_plan_tuning(self._h, None, 0, ctypes.byref(n)))
+        arr = (ctypes.c_int * (5 * max(1, n.value)))()
+        _check(LIB.zgl_plan_tuning(self._h, arr, n.value, ctypes.byref(n)))
+        names = ("fft_forward", "legendre_forward", "radial_forward", "radial_backward",
+                 "legendre_backward", "fft_backward")
+        return [(arr[5 * i], arr[5 * i + 1], arr[5 * i + 2], names[arr[5 * i + 3]], arr[5 * i + 4])
+                for i in range(n.value)]
 
     def profile(self, enable: bool = True):
         """Bracket every kernel launch with HIP events (zgl_profile_enable)."""

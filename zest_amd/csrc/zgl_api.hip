@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
 #include <map>
+#include <tuple>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -77,6 +79,11 @@ struct zgl_plan {
     size_t dgrid_cap = 0;  // doubles
     double* dcoef = nullptr;
     size_t dcoef_cap = 0;  // doubles
+    // autotuning (zgl_plan_autotune): (order, bytes per real of the
+    // arithmetic, stage, bytes per grid value) -> chosen FFT tile width or
+    // instances per workgroup
+    bool autotune = false;
+    std::map<std::tuple<size_t, int, int, int>, int> tuned;
     // stage timing (zgl_profile_enable)
     bool profiling = false;
     struct Rec { int stage; int nb; hipEvent_t start, stop; };
@@ -236,6 +243,55 @@ double forward_prefactor(const zgl_plan* p, const Geom& g) {
                                   : shnorm_value(p->shnorm) * twopi_n;
 }
 
+// ---------------------------------------------------------------- autotuner
+// The first call of a stage at a given order and arithmetic, with autotuning
+// on, times every candidate (one untimed run, one timed with events) on the
+// call's own operands and keeps the fastest. The stage kernels only read
+// their input and overwrite their output, so running one several times leaves
+// the same result. Nothing is tuned while the stream is being captured.
+template <class F>
+int tune(hipStream_t s, const std::vector<int>& cands, F&& launch) {
+    hipEvent_t a, b;
+    check_hip(hipEventCreate(&a), "tune event");
+    check_hip(hipEventCreate(&b), "tune event");
+    float best = std::numeric_limits<float>::max();
+    int pick = cands[0];
+    for (int c : cands) {
+        launch(c);
+        for (int rep = 0; rep < 2; ++rep) {
+            check_hip(hipEventRecord(a, s), "tune");
+            launch(c);
+            check_hip(hipEventRecord(b, s), "tune");
+            check_hip(hipEventSynchronize(b), "tune");
+            float ms = 0;
+            check_hip(hipEventElapsedTime(&ms, a, b), "tune");
+            if (ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return pick;
+}
+
+template <class F>
+int tuned_choice(zgl_plan* p, size_t order, int rbytes, int gbytes, int stage, hipStream_t s,
+                 const std::vector<int>& cands, F&& launch) {
+    const auto key = std::make_tuple(order, rbytes, stage, gbytes);
+    auto it = p->tuned.find(key);
+    if (it != p->tuned.end()) return it->second;
+    if (!p->autotune || cands.size() < 2) return 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+    const int c = tune(s, cands, launch);
+    p->tuned[key] = c;
+    return c;
+}
+
+const std::vector<int> kGroupCandidates = {1, 2, 4, 8};
+
 // T: arithmetic (double or float); G: grid element type in memory.
 // adjoint: the transpose of the backward transform (no quadrature weights,
 // no prefactor, no kappa_n), otherwise the forward transform.
@@ -259,11 +315,27 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
     const int kappa = !adjoint && p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0;
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
+        L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_FWD, s, fft_tiles(t.fft, sizeof(T) == 4),
+                              [&](int c) {
+                                  FftLaunch L2 = L;
+                                  L2.tile = c;
+                                  fft_forward<T, G>(L2, grid.from(b0), sc.spec, ch.spec_stride, nb, s);
+                              });
         timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T, G>(L, grid.from(b0), sc.spec, ch.spec_stride, nb, s); });
         if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, kappa, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride,
                          coeffs.from(b0), nb};
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_FWD, s, kGroupCandidates, [&](int c) {
+                ZtGemmArgs b = a;
+                b.group = c;
+                zt_legendre_forward<T>(b, s);
+            });
             timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_FWD, s, kGroupCandidates, [&](int c) {
+                ZtGemmArgs b = a;
+                b.group = c;
+                zt_radial_forward<T>(b, s);
+            });
             timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward<T>(a, s); });
         } else {
             StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, coeffs.from(b0), nb};
@@ -307,12 +379,28 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
         }
         if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, 0, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride, cin, nb};
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_BWD, s, kGroupCandidates, [&](int c) {
+                ZtGemmArgs b = a;
+                b.group = c;
+                zt_radial_backward<T>(b, s);
+            });
             timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward<T>(a, s); });
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_BWD, s, kGroupCandidates, [&](int c) {
+                ZtGemmArgs b = a;
+                b.group = c;
+                zt_legendre_backward<T>(b, s);
+            });
             timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward<T>(a, s); });
         } else {
             StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, cin, nb};
             timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward<T>(a, s); });
         }
+        L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_BWD, s, fft_tiles(t.fft, sizeof(T) == 4),
+                              [&](int c) {
+                                  FftLaunch L2 = L;
+                                  L2.tile = c;
+                                  fft_backward<T, G>(L2, sc.spec, ch.spec_stride, grid.from(b0), nb, s);
+                              });
         timed(p, ST_FFT_BWD, nb, s, [&] { fft_backward<T, G>(L, sc.spec, ch.spec_stride, grid.from(b0), nb, s); });
     }
 }
@@ -425,6 +513,10 @@ zgl_status zgl_plan_create(zgl_plan** plan, zgl_kind kind, int zernike_norm, int
         p->layout = kind == ZGL_SH ? layout : 0;
         p->precision = precision;
         p->order = order;
+        {
+            const char* e = std::getenv("ZGL_AUTOTUNE");
+            p->autotune = e && e[0] == '1';
+        }
         try {
             if (order) tables_for(p, order);
         } catch (...) {
@@ -827,6 +919,29 @@ zgl_status zgl_backward_grid32(zgl_plan* plan, const double* coeffs, size_t exp_
                                ptrdiff_t grid_stride, zgl_mem mem, void* stream) {
     return backward_entry<float>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
                                  grid_stride, mem, stream);
+}
+
+zgl_status zgl_plan_autotune(zgl_plan* plan, int enable) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    plan->autotune = enable != 0;
+    return ZGL_SUCCESS;
+}
+
+zgl_status zgl_plan_tuning(const zgl_plan* plan, int* entries, int max_entries, int* num_entries) {
+    if (!plan || !num_entries) return fail(ZGL_ERROR_INVALID_VALUE, "NULL argument");
+    int i = 0;
+    for (const auto& kv : plan->tuned) {
+        if (entries && i < max_entries) {
+            entries[5 * i + 0] = (int)std::get<0>(kv.first);
+            entries[5 * i + 1] = std::get<1>(kv.first);
+            entries[5 * i + 2] = std::get<3>(kv.first);
+            entries[5 * i + 3] = std::get<2>(kv.first);
+            entries[5 * i + 4] = kv.second;
+        }
+        ++i;
+    }
+    *num_entries = i;
+    return ZGL_SUCCESS;
 }
 
 zgl_status zgl_profile_enable(zgl_plan* plan, int enable) {

@@ -587,7 +587,7 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
 template <class R, class G>
 void dispatch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss, int nb,
                       hipStream_t s) {
-    switch (L.t->fft.tile[sizeof(R) == 4]) {
+    switch (L.tile ? L.tile : L.t->fft.tile[sizeof(R) == 4]) {
         case 16: launch_forward<16, R>(L, grid, spec, ss, nb, s); break;
         case 8: launch_forward<8, R>(L, grid, spec, ss, nb, s); break;
         case 4: launch_forward<4, R>(L, grid, spec, ss, nb, s); break;
@@ -599,7 +599,7 @@ void dispatch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long s
 template <class R, class G>
 void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid, int nb,
                        hipStream_t s) {
-    switch (L.t->fft.tile[sizeof(R) == 4]) {
+    switch (L.tile ? L.tile : L.t->fft.tile[sizeof(R) == 4]) {
         case 16: launch_backward<16, R>(L, spec, ss, grid, nb, s); break;
         case 8: launch_backward<8, R>(L, spec, ss, grid, nb, s); break;
         case 4: launch_backward<4, R>(L, spec, ss, grid, nb, s); break;
@@ -636,6 +636,15 @@ C* upload(const std::vector<C>& v, hipStream_t s, const char* what) {
 }
 
 }  // namespace
+
+std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
+    std::vector<int> out;
+    const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
+    for (int T = p.tile[fp32 ? 1 : 0]; T >= 1 && out.size() < 2; T /= 2)
+        if (lds_bytes(p.desc.n, p.desc.npass, T, fp32 ? sizeof(float2) : sizeof(double2)) <= limit)
+            out.push_back(T);
+    return out;
+}
 
 void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
     p = FftPlan{};

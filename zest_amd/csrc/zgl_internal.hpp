@@ -140,7 +140,11 @@ struct FftLaunch {
                             // output (on only for the transpose of the forward)
     double scale;           // prefactor applied with the weights
     double center_factor;   // weight factor of an equator node (zest's SH quirk: 2)
+    int tile = 0;           // column pairs per workgroup (0: the plan's choice)
 };
+// Tile widths the FFT kernels are built for that fit the LDS at this length
+// and arithmetic (widest first).
+std::vector<int> fft_tiles(const FftPlan& p, bool fp32);
 
 // R = double or float: the arithmetic; G: the grid element type (double, or
 // float with R = float). spec holds complex R values and its stride counts
@@ -161,6 +165,7 @@ struct ZtGemmArgs {
     void* flm; long flm_stride;
     Inst<double> coeffs;    // strides in doubles
     int nb;
+    int group = 0;          // instances per workgroup of the streaming kernels (0: default)
 };
 template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);

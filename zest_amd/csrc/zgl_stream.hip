@@ -357,15 +357,20 @@ void allow_lds(F* kernel, size_t lds) {
                   "LDS attribute");
 }
 
+int group_of(int requested, int nb) {
+    const int g = requested > 0 ? requested : 4;
+    return nb < g ? nb : g;
+}
+
 template <int NS, class S>
-bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, hipStream_t s) {
+bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, int group, hipStream_t s) {
     using T = typename S::T;
     const int kp = (kmax + 3) & ~3;
     st.as = kp + 1;  // odd stride: the 16 rows of an A fragment fall in distinct banks
     const size_t lds = (size_t)(16 * NS + (S::kCenter ? 1 : 0)) * st.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (nwork == 0 || nb <= 0) return true;
-    st.G = nb < 4 ? nb : 4;
+    st.G = group_of(group, nb);
     dim3 grid(nwork, (nb + st.G - 1) / st.G);
     if (st.ptab) {
         allow_lds(&stream_kernel<S, NS, false>, lds);
@@ -651,7 +656,7 @@ bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const size_t lds = (size_t)16 * kNsRadial * r.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (a.t->n_rf == 0 || a.nb <= 0) return true;
-    r.G = a.nb < 4 ? a.nb : 4;
+    r.G = group_of(a.group, a.nb);
     dim3 grid(a.t->n_rf, (a.nb + r.G - 1) / r.G);
     allow_lds(&rad_fwd_kernel<T, kNsRadial>, lds);
     rad_fwd_kernel<T, kNsRadial><<<grid, kThreads, lds, s>>>(r, a.t->work_rf, a.nb);
@@ -672,7 +677,7 @@ bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     if (lds > kMaxStreamLds) return false;
     if (a.nt == 0 || a.nb <= 0) return true;
     RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
-                    a.nt, g.ncol, a.nb < 4 ? a.nb : 4, rs};
+                    a.nt, g.ncol, group_of(a.group, a.nb), rs};
     dim3 grid(a.nt, (a.nb + r.G - 1) / r.G);
     allow_lds(&rad_bwd_kernel<T>, lds);
     rad_bwd_kernel<T><<<grid, kThreads, lds, s>>>(r, a.nb);
@@ -686,7 +691,7 @@ bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
                  a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0,
                  a.t->zh, a.t->plm_qm, a.t->plm_phase};
-    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, s);
+    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, a.group, s);
 }
 
 template <class T>
@@ -696,7 +701,7 @@ bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
                  a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0, g.nth,
                  a.t->zh, a.t->plm_qm, a.t->plm_phase};
     return launch_stream<kNsStreamBwd>(st, plm_count(g.order, 0, 0), a.t->work_sb, a.t->n_sb,
-                                       a.nb, s);
+                                       a.nb, a.group, s);
 }
 
 bool stream_legendre_fits(int order, int jh) {
