@@ -195,6 +195,28 @@ zgl_status zgl_backward_grid32(zgl_plan *plan, const double *coeffs, size_t exp_
                                ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
                                zgl_mem mem, void *stream);
 
+/* Evaluation on arbitrary grids. Replaces zt::GridEvaluator::evaluate
+ * (include/grid_evaluator.hpp:227-265) for a ZGL_ZERNIKE plan and
+ * st::GridEvaluator::evaluate (include/grid_evaluator.hpp:104-133) for a
+ * ZGL_SH plan, batched over `batch` expansions of order exp_order in the
+ * plan's conventions. The grid is the tensor product of the given longitudes,
+ * colatitudes and (Zernike) radii, in radians and ball radius units; any
+ * values are accepted. Values of instance b start at values + b *
+ * value_stride, row-major [nlon][nlat][nrad] (Zernike) or [nlon][nlat] (SH,
+ * whatever the plan's grid layout; radii and nrad are ignored), as zest
+ * returns them. Strides 0 mean packed (2 * zgl_coeff_size(exp_order) and the
+ * point count). An empty axis writes nothing; order 0 writes zeros. FP64
+ * throughout, whatever the plan's precision. `mem` applies to every pointer,
+ * the point arrays included. Independent of the plan's transform order and
+ * tables: the evaluator keeps its own buffers in the plan. */
+zgl_status zgl_evaluate(zgl_plan *plan, const double *coeffs, size_t exp_order,
+                        const double *longitudes, size_t nlon,
+                        const double *colatitudes, size_t nlat,
+                        const double *radii, size_t nrad,
+                        double *values, size_t batch,
+                        ptrdiff_t coeff_stride, ptrdiff_t value_stride,
+                        zgl_mem mem, void *stream);
+
 /* Autotuning. No counterpart in the reference (zest has nothing to tune).
  * While enabled (also: environment ZGL_AUTOTUNE=1 at plan creation), the
  * first call of each stage at a new order and arithmetic times the candidate

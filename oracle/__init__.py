@@ -47,6 +47,9 @@ def _load_oracle():
     lib.zo_zt_backward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
     lib.zo_st_forward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
     lib.zo_st_backward.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_zt_evaluate.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr,
+                                   _c_size, _ptr, _c_size, _ptr]
+    lib.zo_st_evaluate.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr, _c_size, _ptr]
     return lib
 
 
@@ -69,6 +72,9 @@ def _load_ref():
     lib.zref_gl_angles.argtypes = [_c_size, _ptr]
     lib.zref_plm_table.argtypes = [_c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
     lib.zref_radial_table.argtypes = [_c_int, _c_size, _ptr, _c_size, _ptr]
+    lib.zref_zt_evaluate.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr,
+                                     _c_size, _ptr, _c_size, _ptr]
+    lib.zref_st_evaluate.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr, _c_size, _ptr]
     return lib
 
 
@@ -188,6 +194,34 @@ def st_backward(coeffs: np.ndarray, order: int, exp_order: int | None = None,
     return grid
 
 
+# ---- evaluation on arbitrary grids (zt/st::GridEvaluator,
+#      include/grid_evaluator.hpp:104-133, 227-265) ----
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def zt_evaluate(coeffs: np.ndarray, order: int, lon, colat, r, zernike_norm: int = 1,
+                sh_norm: int = 0, sh_phase: int = -1) -> np.ndarray:
+    """Values [len(lon)][len(colat)][len(r)] of one Zernike expansion."""
+    coeffs, lon, colat, r = _f64(coeffs), _f64(lon), _f64(colat), _f64(r)
+    assert coeffs.size >= 2 * zt_num_coeffs(order)
+    out = np.zeros((lon.size, colat.size, r.size))
+    ORACLE.zo_zt_evaluate(zernike_norm, sh_norm, sh_phase, order, _p(coeffs), _p(lon), lon.size,
+                          _p(colat), colat.size, _p(r), r.size, _p(out))
+    return out
+
+
+def st_evaluate(coeffs: np.ndarray, order: int, lon, colat, sh_norm: int = 0,
+                sh_phase: int = -1) -> np.ndarray:
+    """Values [len(lon)][len(colat)] of one spherical-harmonic expansion."""
+    coeffs, lon, colat = _f64(coeffs), _f64(lon), _f64(colat)
+    assert coeffs.size >= 2 * st_num_coeffs(order)
+    out = np.zeros((lon.size, colat.size))
+    ORACLE.zo_st_evaluate(sh_norm, sh_phase, order, _p(coeffs), _p(lon), lon.size, _p(colat),
+                          colat.size, _p(out))
+    return out
+
+
 # ---- the upstream zest library (compiled reference) ----
 class ZestZernike:
     """One zest ``zt::GLQTransformer`` instance (keeps its tables between calls)."""
@@ -289,4 +323,24 @@ def ref_radial_table(zernike_norm: int, order: int, r: np.ndarray) -> np.ndarray
     r = np.ascontiguousarray(r, dtype=np.float64)
     out = np.zeros(((order + 1) ** 2 // 4, r.size))
     REF.zref_radial_table(zernike_norm, order, _p(r), r.size, _p(out))
+    return out
+
+
+def ref_zt_evaluate(coeffs: np.ndarray, order: int, lon, colat, r, zernike_norm: int = 1,
+                    sh_norm: int = 0, sh_phase: int = -1) -> np.ndarray:
+    coeffs, lon, colat, r = _f64(coeffs), _f64(lon), _f64(colat), _f64(r)
+    out = np.zeros((lon.size, colat.size, r.size))
+    if out.size:
+        REF.zref_zt_evaluate(zernike_norm, sh_norm, sh_phase, order, _p(coeffs), _p(lon), lon.size,
+                             _p(colat), colat.size, _p(r), r.size, _p(out))
+    return out
+
+
+def ref_st_evaluate(coeffs: np.ndarray, order: int, lon, colat, sh_norm: int = 0,
+                    sh_phase: int = -1) -> np.ndarray:
+    coeffs, lon, colat = _f64(coeffs), _f64(lon), _f64(colat)
+    out = np.zeros((lon.size, colat.size))
+    if out.size:
+        REF.zref_st_evaluate(sh_norm, sh_phase, order, _p(coeffs), _p(lon), lon.size, _p(colat),
+                             colat.size, _p(out))
     return out

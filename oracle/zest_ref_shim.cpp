@@ -15,6 +15,8 @@
 //   gl::gl_nodes_and_weights (include/gauss_legendre.hpp:1549-1562)
 //   st::PlmRecursion::plm_real (include/plm_recursion.hpp:108-113)
 //   zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-154)
+//   zt::GridEvaluator::evaluate (include/grid_evaluator.hpp:227-265)
+//   st::GridEvaluator::evaluate (include/grid_evaluator.hpp:104-133)
 #include <cstddef>
 #include <cstring>
 #include <new>
@@ -26,6 +28,7 @@
 #include "gauss_legendre.hpp"
 #include "plm_recursion.hpp"
 #include "radial_zernike_recursion.hpp"
+#include "grid_evaluator.hpp"
 
 namespace {
 
@@ -113,6 +116,24 @@ template <typename Layout>
 StBase* make_st_layout(int sh_norm, int phase, std::size_t order) {
     if (sh_norm == 1) return make_st_phase<SHNorm::qm, Layout>(phase, order);
     return make_st_phase<SHNorm::geo, Layout>(phase, order);
+}
+
+template <ZernikeNorm zn, SHNorm sn, SHPhase sp>
+void zt_eval(std::size_t order, const double* coeffs, std::span<const double> lon,
+             std::span<const double> colat, std::span<const double> r, double* values) {
+    using Exp = zest::zt::RealZernikeSpan<const std::array<double, 2>, zn, sn, sp>;
+    const auto v = zest::zt::GridEvaluator(order).evaluate(
+        Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), order), lon, colat, r);
+    std::memcpy(values, v.data(), v.size() * sizeof(double));
+}
+
+template <SHNorm sn, SHPhase sp>
+void st_eval(std::size_t order, const double* coeffs, std::span<const double> lon,
+             std::span<const double> colat, double* values) {
+    using Exp = zest::st::RealSHSpan<const std::array<double, 2>, sn, sp>;
+    const auto v = zest::st::GridEvaluator(order).evaluate(
+        Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), order), lon, colat);
+    std::memcpy(values, v.data(), v.size() * sizeof(double));
 }
 
 }  // namespace
@@ -230,6 +251,39 @@ void zref_radial_table(int zernike_norm, std::size_t order, const double* r,
     } else {
         zest::zt::RadialZernikeVecSpan<double, ZernikeNorm::normed> z(buf, order, nr);
         rec.zernike<ZernikeNorm::normed>(rs, z);
+    }
+}
+
+// A fresh GridEvaluator per call: zest's evaluators keep per-order buffers
+// between calls.
+void zref_zt_evaluate(int zernike_norm, int sh_norm, int sh_phase, std::size_t order,
+                      const double* coeffs, const double* lon, std::size_t nlon,
+                      const double* colat, std::size_t nlat, const double* r,
+                      std::size_t nrad, double* values) {
+    std::span<const double> L(lon, nlon), C(colat, nlat), R(r, nrad);
+    const int key = (zernike_norm == 1 ? 4 : 0) + (sh_norm == 1 ? 2 : 0) + (sh_phase > 0 ? 1 : 0);
+    switch (key) {
+        case 0: zt_eval<ZernikeNorm::normed, SHNorm::geo, SHPhase::none>(order, coeffs, L, C, R, values); break;
+        case 1: zt_eval<ZernikeNorm::normed, SHNorm::geo, SHPhase::cs>(order, coeffs, L, C, R, values); break;
+        case 2: zt_eval<ZernikeNorm::normed, SHNorm::qm, SHPhase::none>(order, coeffs, L, C, R, values); break;
+        case 3: zt_eval<ZernikeNorm::normed, SHNorm::qm, SHPhase::cs>(order, coeffs, L, C, R, values); break;
+        case 4: zt_eval<ZernikeNorm::unnormed, SHNorm::geo, SHPhase::none>(order, coeffs, L, C, R, values); break;
+        case 5: zt_eval<ZernikeNorm::unnormed, SHNorm::geo, SHPhase::cs>(order, coeffs, L, C, R, values); break;
+        case 6: zt_eval<ZernikeNorm::unnormed, SHNorm::qm, SHPhase::none>(order, coeffs, L, C, R, values); break;
+        default: zt_eval<ZernikeNorm::unnormed, SHNorm::qm, SHPhase::cs>(order, coeffs, L, C, R, values); break;
+    }
+}
+
+void zref_st_evaluate(int sh_norm, int sh_phase, std::size_t order, const double* coeffs,
+                      const double* lon, std::size_t nlon, const double* colat,
+                      std::size_t nlat, double* values) {
+    std::span<const double> L(lon, nlon), C(colat, nlat);
+    const int key = (sh_norm == 1 ? 2 : 0) + (sh_phase > 0 ? 1 : 0);
+    switch (key) {
+        case 0: st_eval<SHNorm::geo, SHPhase::none>(order, coeffs, L, C, values); break;
+        case 1: st_eval<SHNorm::geo, SHPhase::cs>(order, coeffs, L, C, values); break;
+        case 2: st_eval<SHNorm::qm, SHPhase::none>(order, coeffs, L, C, values); break;
+        default: st_eval<SHNorm::qm, SHPhase::cs>(order, coeffs, L, C, values); break;
     }
 }
 

@@ -654,3 +654,115 @@ int zo_st_backward(int sn, int sp, int layout, size_t exp_order, const double *c
     st_tables_free(&t);
     return 0;
 }
+
+/* ---- evaluation on arbitrary tensor-product grids -----------------------------
+ * zt::GridEvaluator / st::GridEvaluator (include/grid_evaluator.hpp:104-133,
+ * 227-318; src/grid_evaluator.cpp:44-256): the expansion is summed directly
+ * at the given longitudes, colatitudes (and radii), no quadrature grid, no FFT.
+ * cos(m phi), sin(m phi) come from the complex-multiplication recurrence of
+ * detail::recursive_trig (src/grid_evaluator.cpp:44-70). */
+static void eval_trig(size_t nm, const double *lon, size_t nlon, double *cs) {
+    /* cs layout [m][nlon][2] */
+    for (size_t i = 0; i < nlon; ++i) { cs[2 * i] = 1.0; cs[2 * i + 1] = 0.0; }
+    if (nm < 2) return;
+    double *c1 = cs + 2 * nlon;
+    for (size_t i = 0; i < nlon; ++i) { c1[2 * i] = cos(lon[i]); c1[2 * i + 1] = sin(lon[i]); }
+    for (size_t m = 2; m < nm; ++m) {
+        const double *p = cs + (m - 1) * 2 * nlon;
+        double *q = cs + m * 2 * nlon;
+        for (size_t i = 0; i < nlon; ++i) {
+            q[2 * i] = p[2 * i] * c1[2 * i] - p[2 * i + 1] * c1[2 * i + 1];
+            q[2 * i + 1] = p[2 * i] * c1[2 * i + 1] + p[2 * i + 1] * c1[2 * i];
+        }
+    }
+}
+
+/* zt::GridEvaluator::evaluate, include/grid_evaluator.hpp:227-265: values
+ * [nlon][nlat][nrad]; sum_n (268-302), sum_l and sum_m
+ * (src/grid_evaluator.cpp:185-256). */
+void zo_zt_evaluate(int zn, int sn, int sp, size_t order, const double *coeffs,
+                    const double *lon, size_t nlon, const double *colat, size_t nlat,
+                    const double *r, size_t nrad, double *values) {
+    if (nlon == 0 || nlat == 0 || nrad == 0) return;
+    memset(values, 0, nlon * nlat * nrad * sizeof(double));
+    if (order == 0) return;
+    double *rad = (double *)calloc((((order + 1) * (order + 1)) >> 2) * nrad + 1, sizeof(double));
+    double *z = (double *)malloc(nlat * sizeof(double));
+    double *plm = (double *)calloc(((order * (order + 1)) >> 1) * nlat + 1, sizeof(double));
+    double *cs = (double *)malloc(order * nlon * 2 * sizeof(double));
+    zo_radial_table(zn, order, r, nrad, rad);
+    for (size_t j = 0; j < nlat; ++j) z[j] = cos(colat[j]);
+    zo_plm_table(sn, sp, order, z, nlat, plm);
+    eval_trig(order, lon, nlon, cs);
+    /* sum_n: flm[l,m][k] = sum_n R_nl(r_k) c_nlm */
+    double *flm = (double *)calloc(((order * (order + 1)) >> 1) * nrad * 2, sizeof(double));
+    for (size_t n = 0; n < order; ++n)
+        for (size_t l = n & 1; l <= n; l += 2) {
+            const double *rr = rad + rad_idx(n, l) * nrad;
+            for (size_t m = 0; m <= l; ++m) {
+                const double *c = coeffs + tet_idx(n, l, m) * 2;
+                double *f = flm + tri_idx(l, m) * nrad * 2;
+                for (size_t k = 0; k < nrad; ++k) {
+                    f[2 * k] += rr[k] * c[0];
+                    f[2 * k + 1] += rr[k] * c[1];
+                }
+            }
+        }
+    /* sum_l: fm[m][j][k] = sum_l P_lm(z_j) flm[l,m][k] */
+    double *fm = (double *)calloc(order * nlat * nrad * 2, sizeof(double));
+    for (size_t l = 0; l < order; ++l)
+        for (size_t m = 0; m <= l; ++m) {
+            const double *f = flm + tri_idx(l, m) * nrad * 2;
+            const double *p = plm + tri_idx(l, m) * nlat;
+            for (size_t j = 0; j < nlat; ++j) {
+                double *x = fm + (m * nlat + j) * nrad * 2;
+                for (size_t k = 0; k < nrad; ++k) {
+                    x[2 * k] += p[j] * f[2 * k];
+                    x[2 * k + 1] += p[j] * f[2 * k + 1];
+                }
+            }
+        }
+    /* sum_m: values[i][j][k] += fm.re cos(m phi_i) + fm.im sin(m phi_i) */
+    for (size_t m = 0; m < order; ++m)
+        for (size_t i = 0; i < nlon; ++i) {
+            const double c = cs[(m * nlon + i) * 2], s = cs[(m * nlon + i) * 2 + 1];
+            for (size_t j = 0; j < nlat; ++j) {
+                const double *x = fm + (m * nlat + j) * nrad * 2;
+                double *v = values + (i * nlat + j) * nrad;
+                for (size_t k = 0; k < nrad; ++k) v[k] += x[2 * k] * c + x[2 * k + 1] * s;
+            }
+        }
+    free(rad); free(z); free(plm); free(cs); free(flm); free(fm);
+}
+
+/* st::GridEvaluator::evaluate, include/grid_evaluator.hpp:104-159: values
+ * [nlon][nlat]; sum_m src/grid_evaluator.cpp:105-128. */
+void zo_st_evaluate(int sn, int sp, size_t order, const double *coeffs, const double *lon,
+                    size_t nlon, const double *colat, size_t nlat, double *values) {
+    if (nlon == 0 || nlat == 0) return;
+    memset(values, 0, nlon * nlat * sizeof(double));
+    if (order == 0) return;
+    double *z = (double *)malloc(nlat * sizeof(double));
+    double *plm = (double *)calloc(((order * (order + 1)) >> 1) * nlat + 1, sizeof(double));
+    double *cs = (double *)malloc(order * nlon * 2 * sizeof(double));
+    for (size_t j = 0; j < nlat; ++j) z[j] = cos(colat[j]);
+    zo_plm_table(sn, sp, order, z, nlat, plm);
+    eval_trig(order, lon, nlon, cs);
+    double *fm = (double *)calloc(order * nlat * 2, sizeof(double));
+    for (size_t l = 0; l < order; ++l)
+        for (size_t m = 0; m <= l; ++m) {
+            const double *c = coeffs + tri_idx(l, m) * 2;
+            const double *p = plm + tri_idx(l, m) * nlat;
+            for (size_t j = 0; j < nlat; ++j) {
+                fm[(m * nlat + j) * 2] += p[j] * c[0];
+                fm[(m * nlat + j) * 2 + 1] += p[j] * c[1];
+            }
+        }
+    for (size_t m = 0; m < order; ++m)
+        for (size_t i = 0; i < nlon; ++i) {
+            const double c = cs[(m * nlon + i) * 2], s = cs[(m * nlon + i) * 2 + 1];
+            for (size_t j = 0; j < nlat; ++j)
+                values[i * nlat + j] += fm[(m * nlat + j) * 2] * c + fm[(m * nlat + j) * 2 + 1] * s;
+        }
+    free(z); free(plm); free(cs); free(fm);
+}

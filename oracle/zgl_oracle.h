@@ -43,6 +43,14 @@ int zo_st_forward(int sh_norm, int sh_phase, int layout, size_t grid_order,
 int zo_st_backward(int sh_norm, int sh_phase, int layout, size_t exp_order,
                    const double *coeffs, size_t grid_order, double *grid);
 
+void zo_zt_evaluate(int zernike_norm, int sh_norm, int sh_phase, size_t order,
+                    const double *coeffs, const double *lon, size_t nlon,
+                    const double *colat, size_t nlat, const double *r, size_t nrad,
+                    double *values);
+void zo_st_evaluate(int sh_norm, int sh_phase, size_t order, const double *coeffs,
+                    const double *lon, size_t nlon, const double *colat, size_t nlat,
+                    double *values);
+
 #ifdef __cplusplus
 }
 #endif

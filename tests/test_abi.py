@@ -42,6 +42,7 @@ def test_invalid_arguments_fail_before_device_work():
     assert not h.value
     assert lib.zgl_forward(None, None, 4, None, 4, 1, 0, 0, 0, None) == 1
     assert lib.zgl_backward(None, None, 4, None, 4, 1, 0, 0, 0, None) == 1
+    assert lib.zgl_evaluate(None, None, 4, None, 1, None, 1, None, 1, None, 1, 0, 0, 0, None) == 1
     assert lib.zgl_plan_resize(None, 3) == 1
     assert lib.zgl_plan_order(None) == 0
     assert lib.zgl_plan_destroy(None) == 0

@@ -329,3 +329,143 @@ def test_zernike_brute_force_quadrature():
                 ss = (g * np.sin(m * lon)[:, None, None] * base).sum()
                 idx = (n + 1) * (n + 3) * (2 * n + 1) // 24 + l * l // 4 + m
                 assert abs(c[idx, 0] - cc) < 1e-12 and abs(c[idx, 1] - ss) < 1e-12
+
+
+# ---------------------------------------------------------------- grid evaluator
+def _linspace(a, b, n):
+    # tests/test_grid_evaluator.cpp:41-55 (last point exactly b)
+    x = np.linspace(a, b, n)
+    x[-1] = b
+    return x
+
+
+@pytest.mark.parametrize("conv", CONV_ST)
+def test_sh_evaluate_known_answers(conv):
+    """tests/test_grid_evaluator.cpp:57-281: constant, Y_10 and Y_31 + Y_4,-3 on a
+    30 x 15 grid that includes both poles, tolerance 1e-13."""
+    sn, sp = conv
+    order = 6
+    lon, col = _linspace(0, 2 * np.pi, 30), _linspace(0, np.pi, 15)
+    L, C = np.meshgrid(lon, col, indexing="ij")
+    z = np.cos(C)
+    shnorm = 0.5 / math.sqrt(math.pi) if sn == 1 else 1.0
+    phase = -1.0 if sp == -1 else 1.0
+    c = np.zeros((oracle.st_num_coeffs(order), 2))
+    c[0, 0] = 1.0
+    assert np.abs(oracle.st_evaluate(c, order, lon, col, sn, sp) - shnorm).max() < 1e-13
+    c[:] = 0
+    c[1, 0] = 1.0
+    assert np.abs(oracle.st_evaluate(c, order, lon, col, sn, sp) - shnorm * math.sqrt(3) * z).max() < 1e-13
+    c[:] = 0
+    c[3 * 4 // 2 + 1, 0] = 1.0
+    c[4 * 5 // 2 + 3, 1] = 1.0
+    u = np.sqrt(1 - z * z)
+    f = phase * shnorm * (math.sqrt(21 / 8) * u * (5 * z * z - 1) * np.cos(L)
+                          + math.sqrt(315 / 8) * u * (1 - z * z) * z * np.sin(3 * L))
+    assert np.abs(oracle.st_evaluate(c, order, lon, col, sn, sp) - f).max() < 1e-13
+
+
+@pytest.mark.parametrize("conv", CONV_ZT)
+def test_zernike_evaluate_known_answers(conv):
+    """tests/test_grid_evaluator.cpp:283-415: constant and Z_33,-2 + Z_531 on a
+    14 x 7 x 7 grid including the poles, r = 0 and r = 1, tolerance 1e-13."""
+    zn, sn, sp = conv
+    order = 6
+    lon, col, r = _linspace(0, 2 * np.pi, 14), _linspace(0, np.pi, 7), _linspace(0, 1, 7)
+    L, C, R = np.meshgrid(lon, col, r, indexing="ij")
+    z = np.cos(C)
+    shnorm = 0.5 / math.sqrt(math.pi) if sn == 1 else 1.0
+    phase = -1.0 if sp == -1 else 1.0
+    c = np.zeros((oracle.zt_num_coeffs(order), 2))
+    c[0, 0] = 1.0
+    const = (math.sqrt(3) if zn == 0 else 1.0) * shnorm
+    assert np.abs(oracle.zt_evaluate(c, order, lon, col, r, zn, sn, sp) - const).max() < 1e-13
+    c[:] = 0
+    tet = lambda n, l, m: (n + 1) * (n + 3) * (2 * n + 1) // 24 + l * l // 4 + m
+    c[tet(3, 3, 2), 1] = 1.0
+    c[tet(5, 3, 1), 0] = 1.0
+    zn3 = 3.0 if zn == 0 else 1.0
+    zn5 = math.sqrt(13) if zn == 0 else 1.0
+    f = shnorm * (zn3 * R ** 3 * math.sqrt(105 / 4) * (1 - z * z) * z * np.sin(2 * L)
+                  + zn5 * phase * (5.5 * R * R - 4.5) * R ** 3 * math.sqrt(21 / 8)
+                  * np.sqrt(1 - z * z) * (5 * z * z - 1) * np.cos(L))
+    assert np.abs(oracle.zt_evaluate(c, order, lon, col, r, zn, sn, sp) - f).max() < 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 2, 7, 16])
+def test_zernike_evaluate_equals_backward_on_quadrature_nodes(order):
+    """On the quadrature grid (longitudes 2 pi i / nphi, colatitudes arccos z_j,
+    radial GL nodes) direct evaluation and the GLQ backward transform are the
+    same sum, reached by different routes (FFT vs direct trig sums)."""
+    lon, z, r = ball_grid_coords(order)
+    for zn, sn, sp in [(0, 0, -1), (1, 1, 1)]:
+        c = synth.zernike_coeffs(order, order + 1)
+        g = oracle.zt_backward(c, order, zernike_norm=zn, sh_norm=sn, sh_phase=sp)
+        v = oracle.zt_evaluate(c, order, lon, np.arccos(z), r, zn, sn, sp)
+        assert rel(v, g) < 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 5, 16])
+def test_sh_evaluate_equals_backward_on_quadrature_nodes(order):
+    nlon, nlat = oracle.st_shape(order, 0)
+    lon = 2 * np.pi * np.arange(nlon) / nlon
+    z, _ = oracle.gl_nodes_weights(nlat)
+    c = synth.sh_coeffs(order, order + 2)
+    g = oracle.st_backward(c, order, sh_norm=1, sh_phase=1)
+    v = oracle.st_evaluate(c, order, lon, np.arccos(z), 1, 1)
+    assert rel(v, g) < 1e-13
+
+
+def test_evaluate_brute_force_small():
+    """Triple sum over (n, l, m) of scipy's R_nl P_lm cos/sin at random points."""
+    order = 5
+    rng = np.random.default_rng(7)
+    lon, col, r = rng.uniform(0, 7, 4), rng.uniform(-1, 4, 3), rng.uniform(0, 1.2, 5)
+    c = synth.zernike_coeffs(order, 11)
+    v = oracle.zt_evaluate(c, order, lon, col, r, 1, 0, -1)
+    ref = np.zeros_like(v)
+    z = np.cos(col)
+    for n in range(order):
+        for l in range(n & 1, n + 1, 2):
+            R = rnl_scipy(n, l, r, False)
+            for m in range(l + 1):
+                # scipy's P_l^m carries |sin|, the recursion sqrt((1-z)(1+z)): equal
+                P = plm_scipy(l, m, z, 0, -1)
+                idx = (n + 1) * (n + 3) * (2 * n + 1) // 24 + l * l // 4 + m
+                ang = c[idx, 0] * np.cos(m * lon) + c[idx, 1] * np.sin(m * lon)
+                ref += ang[:, None, None] * P[None, :, None] * R[None, None, :]
+    assert rel(v, ref) < 1e-12
+
+
+def test_evaluate_empty_and_order0():
+    c = np.zeros((1, 2))
+    assert oracle.zt_evaluate(c, 0, [0.1], [0.2], [0.3]).tolist() == [[[0.0]]]
+    assert oracle.zt_evaluate(np.zeros((oracle.zt_num_coeffs(3), 2)), 3, [], [0.2], [0.3]).shape == (0, 1, 1)
+    assert oracle.st_evaluate(c, 0, [0.1, 0.2], [0.2]).tolist() == [[0.0], [0.0]]
+
+
+@needs_ref
+@pytest.mark.parametrize("conv", CONV_ZT)
+@pytest.mark.parametrize("order", [1, 3, 10, 33])
+def test_zernike_evaluate_matches_compiled_zest(conv, order):
+    zn, sn, sp = conv
+    rng = np.random.default_rng(order)
+    lon = np.concatenate([[0.0, 2 * np.pi], rng.uniform(-1, 8, 9)])
+    col = np.concatenate([[0.0, np.pi], rng.uniform(0, np.pi, 6)])
+    r = np.concatenate([[0.0, 1.0], rng.uniform(0, 1.1, 4)])
+    c = synth.zernike_coeffs(order, 5 + order)
+    assert rel(oracle.zt_evaluate(c, order, lon, col, r, zn, sn, sp),
+               oracle.ref_zt_evaluate(c, order, lon, col, r, zn, sn, sp)) < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("conv", CONV_ST)
+@pytest.mark.parametrize("order", [1, 2, 17, 100])
+def test_sh_evaluate_matches_compiled_zest(conv, order):
+    sn, sp = conv
+    rng = np.random.default_rng(order + 1)
+    lon = np.concatenate([[0.0, 2 * np.pi], rng.uniform(-1, 8, 13)])
+    col = np.concatenate([[0.0, np.pi], rng.uniform(0, np.pi, 10)])
+    c = synth.sh_coeffs(order, 2 + order)
+    assert rel(oracle.st_evaluate(c, order, lon, col, sn, sp),
+               oracle.ref_st_evaluate(c, order, lon, col, sn, sp)) < 1e-13

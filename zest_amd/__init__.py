@@ -78,6 +78,8 @@ def _load():
     lib.zgl_forward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_forward_grid32.restype = i32
     lib.zgl_backward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
+    lib.zgl_evaluate.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, sz, pd, pd, i32, vp]
+    lib.zgl_evaluate.restype = i32
     lib.zgl_backward_grid32.restype = i32
     lib.zgl_last_error_string.argtypes = []
     lib.zgl_last_error_string.restype = ctypes.c_char_p
@@ -111,7 +113,7 @@ EXPORTS = (
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
     "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
     "zgl_forward_ragged", "zgl_backward_ragged", "zgl_ragged_lane_costs",
-    "zgl_plan_autotune", "zgl_plan_tuning",
+    "zgl_plan_autotune", "zgl_plan_tuning", "zgl_evaluate",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -252,6 +254,40 @@ class Plan:
         _check(fn(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                   grid_stride, gm, _stream_ptr(stream, gm)))
 
+
+    def evaluate(self, coeffs, order: int, longitudes, colatitudes, radii=None, values=None,
+                 batch: int = 1, coeff_stride: int = 0, value_stride: int = 0, stream=None):
+        """Values of `batch` expansions of `order` on the grid longitudes x
+        colatitudes (x radii for a Zernike plan), zgl_evaluate. Every array
+        lives in the same memory (torch tensors on the GPU, or host arrays);
+        returns `values`, allocated like `coeffs` when not given, shaped
+        (batch, nlon, nlat[, nrad])."""
+        zt = self.kind == ZERNIKE
+        if zt and radii is None:
+            raise ValueError("a Zernike plan needs radii")
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        lp, lm, _ = _ptr_and_mem(longitudes)
+        tp, tm, _ = _ptr_and_mem(colatitudes)
+        nlon, nlat = int(np.prod(longitudes.shape)), int(np.prod(colatitudes.shape))
+        rp, nrad, mems = 0, 1, {cm, lm, tm}
+        if zt:
+            rp, rm, _ = _ptr_and_mem(radii)
+            nrad = int(np.prod(radii.shape))
+            mems.add(rm)
+        shape = (batch, nlon, nlat) + ((nrad,) if zt else ())
+        if values is None:
+            if isinstance(coeffs, np.ndarray):
+                values = np.empty(shape)
+            else:
+                import torch
+                values = torch.empty(shape, dtype=torch.float64, device=coeffs.device)
+        vp, vm, _ = _ptr_and_mem(values)
+        mems.add(vm)
+        if len(mems) != 1:
+            raise ValueError("coefficients, points and values must live in the same memory")
+        _check(LIB.zgl_evaluate(self._h, cp, order, lp, nlon, tp, nlat, rp or None, nrad, vp, batch,
+                                coeff_stride, value_stride, vm, _stream_ptr(stream, vm)))
+        return values
 
     @staticmethod
     def _jobs(jobs):

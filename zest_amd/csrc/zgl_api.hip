@@ -106,6 +106,10 @@ struct zgl_plan {
         } cache[2];
         std::vector<double> lane_cost;  // modelled work per lane of the last build
     } rg;
+    // zgl_evaluate: tables and staging at the last evaluation grid
+    zgl::EvalCache eval;
+    double* dpts = nullptr;  // host-memory calls: the points, uploaded
+    size_t dpts_cap = 0;
 };
 
 namespace zgl {
@@ -539,7 +543,8 @@ zgl_status zgl_plan_destroy(zgl_plan* plan) {
     for (auto e : plan->pool) (void)hipEventDestroy(e);
     plan->main.release();
     ragged_release(plan->rg);
-    for (void* q : {(void*)plan->dgrid, (void*)plan->dcoef})
+    plan->eval.release();
+    for (void* q : {(void*)plan->dgrid, (void*)plan->dcoef, (void*)plan->dpts})
         if (q) (void)hipFree(q);
     delete plan;
     return ZGL_SUCCESS;
@@ -987,3 +992,68 @@ zgl_status zgl_profile_stats(zgl_plan* plan, zgl_stage_stats* out, int max_stage
 const char* zgl_last_error_string(void) { return g_last_error.c_str(); }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- evaluation
+extern "C" zgl_status zgl_evaluate(zgl_plan* plan, const double* coeffs, size_t exp_order,
+                                   const double* longitudes, size_t nlon,
+                                   const double* colatitudes, size_t nlat, const double* radii,
+                                   size_t nrad, double* values, size_t batch,
+                                   ptrdiff_t coeff_stride, ptrdiff_t value_stride, zgl_mem mem,
+                                   void* stream) {
+    using namespace zgl;
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (exp_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
+    if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    const bool zt = plan->kind == ZGL_ZERNIKE;
+    if (!zt) nrad = 1;
+    const size_t kMaxPoints = (size_t)1 << 30;
+    if (nlon > kMaxPoints || nlat > kMaxPoints || nrad > kMaxPoints || batch > kMaxPoints)
+        return fail(ZGL_ERROR_INVALID_VALUE, "point or instance count too large");
+    const long npts = (long)(nlon * nlat * nrad);
+    const long csz = 2 * (long)coeff_size(plan, exp_order);
+    const long cs = coeff_stride ? (long)coeff_stride : csz;
+    const long vs = value_stride ? (long)value_stride : npts;
+    if (batch > 1 && (vs < npts || cs < csz)) return fail(ZGL_ERROR_INVALID_VALUE, "instance strides overlap");
+    if (npts == 0 || batch == 0) return ZGL_SUCCESS;
+    if (!values || !longitudes || !colatitudes || (zt && !radii) || (!coeffs && exp_order > 0))
+        return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    return guarded([&]() -> zgl_status {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        EvalCall e{plan->kind, plan->znorm, plan->shnorm == ZGL_SHNORM_QM ? 1 : 0, (double)plan->phase,
+                   (int)exp_order, Inst<const double>{coeffs, cs, nullptr},
+                   longitudes, (int)nlon, colatitudes, (int)nlat, zt ? radii : nullptr, (int)nrad,
+                   values, vs, (int)batch};
+        if (mem == ZGL_MEM_DEVICE) {
+            evaluate(plan->eval, e, s);
+            check_hip(hipGetLastError(), "evaluate");
+            return ZGL_SUCCESS;
+        }
+        const size_t np = nlon + nlat + (zt ? nrad : 0);
+        ensure(plan->dpts, plan->dpts_cap, np);
+        ensure(plan->dcoef, plan->dcoef_cap, std::max<size_t>((size_t)csz, 1) * batch);
+        ensure(plan->dgrid, plan->dgrid_cap, (size_t)npts * batch);
+        double* dp = plan->dpts;
+        check_hip(hipMemcpyAsync(dp, longitudes, nlon * sizeof(double), hipMemcpyHostToDevice, s), "points");
+        check_hip(hipMemcpyAsync(dp + nlon, colatitudes, nlat * sizeof(double), hipMemcpyHostToDevice, s),
+                  "points");
+        if (zt)
+            check_hip(hipMemcpyAsync(dp + nlon + nlat, radii, nrad * sizeof(double), hipMemcpyHostToDevice, s),
+                      "points");
+        if (csz)
+            check_hip(hipMemcpy2DAsync(plan->dcoef, csz * sizeof(double), coeffs, cs * sizeof(double),
+                                       csz * sizeof(double), batch, hipMemcpyHostToDevice, s),
+                      "coefficient upload");
+        e.coeffs = Inst<const double>{plan->dcoef, csz, nullptr};
+        e.lon = dp;
+        e.colat = dp + nlon;
+        e.radii = zt ? dp + nlon + nlat : nullptr;
+        e.values = plan->dgrid;
+        e.value_stride = npts;
+        evaluate(plan->eval, e, s);
+        check_hip(hipMemcpy2DAsync(values, vs * sizeof(double), plan->dgrid, npts * sizeof(double),
+                                   npts * sizeof(double), batch, hipMemcpyDeviceToHost, s),
+                  "value download");
+        check_hip(hipStreamSynchronize(s), "evaluate");
+        return ZGL_SUCCESS;
+    });
+}

@@ -5,7 +5,10 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "zgl.h"
@@ -194,6 +197,60 @@ struct StGemmArgs {
 };
 template <class T> void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
 template <class T> void st_legendre_backward(const StGemmArgs& a, hipStream_t s);
+
+// ---- evaluation at arbitrary points (zgl_eval.hip, zgl_stream.hip) ----
+// S[inst][m][j][c] = sum_{l=m}^{nt-1} P_lm(z_j) flm[inst][l,m][c], complex
+// columns c < ncol; P from the table [m][l-m][nlat] (rows for l < order) or,
+// with ptab null, recomputed from z in the kernel. Work items (m, first
+// point) step by 16 * ns points.
+struct EvalLegArgs {
+    const double* ptab;
+    const double* z;
+    int qm; double phase;
+    const double* flm; long flm_stride;  // doubles per instance
+    double* spec; long spec_stride;      // doubles per instance
+    int order, nt, nlat, ncol, nb;
+    const int4* work; int nwork; int ns;
+};
+bool eval_legendre(const EvalLegArgs& a, hipStream_t s);
+// The 16-point strips per workgroup (4, 2 or 1) whose P block fits the LDS
+// budget at this order; 0 when none does.
+int eval_legendre_strips(int nt);
+// R_nl(r_k) table [((n+1)^2>>2)+(l>>1)][nr] at nr given radii (device).
+void radial_table(int order, int nr, int normed, const double* r, double* rtab, hipStream_t s);
+// P_lm(z_j) table [m][l-m][nlat] at nlat given z (device).
+void eval_plm_table(int order, int nlat, int qm, double phase, const double* z, double* ptab,
+                    hipStream_t s);
+
+// One evaluation call: nb expansions of one order at the tensor-product grid
+// lon x colat (x radii for Zernike). All pointers are device memory; values of
+// instance b start at values + b * value_stride, shaped [nlon][nlat][nrad]
+// (Zernike) or [nlon][nlat] (SH).
+struct EvalCall {
+    zgl_kind kind;
+    int znorm, qm;
+    double phase;
+    int order;
+    Inst<const double> coeffs;
+    const double* lon; int nlon;
+    const double* colat; int nlat;
+    const double* radii; int nrad;
+    double* values; long value_stride;
+    int nb;
+};
+// Device buffers and work lists of the evaluator, kept between calls.
+struct EvalCache {
+    double* pts = nullptr; size_t pts_cap = 0;    // doubles: z, padded radii
+    double* ptab = nullptr; size_t ptab_cap = 0;
+    double* rtab = nullptr; size_t rtab_cap = 0;
+    double* trig = nullptr; size_t trig_cap = 0;
+    double* flm = nullptr; size_t flm_cap = 0;
+    double* spec = nullptr; size_t spec_cap = 0;
+    std::map<std::tuple<int, int, int>, std::pair<int4*, int>> leg_work;  // (order, nlat, strips)
+    std::map<int, std::pair<int4*, int>> rad_work;                      // order
+    void release();
+};
+void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s);
 
 // ---- elementwise helpers (zgl_tables.hip) ----
 // r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in elements.

@@ -166,6 +166,57 @@ struct LegBwd {
     }
 };
 
+// Evaluation at arbitrary colatitudes (zt::GridEvaluator::sum_l,
+// src/grid_evaluator.cpp:185-219; st::GridEvaluator::sum_l,
+// include/grid_evaluator.hpp:136-159):
+//   S[m][j][c] = sum_{l=m}^{nt-1} P_lm(z_j) flm[l,m][c].
+// The points have no north/south symmetry, so one problem per m takes every
+// l (no parity split) and the rows are the nlat points. P comes from the
+// table [m][l-m][nlat] or is recomputed from z in the workgroup.
+template <class TT>
+struct LegEval {
+    using T = TT;
+    const T* ptab; const T* flm; T* spec;
+    long spec_stride, flm_stride;
+    int order, nt, jh, ncol, G, as;  // jh: the number of points nlat
+    const double* zh; int qm; double phase;
+    static constexpr bool kCenter = false;
+    __device__ int center_lds_row() const { return 0; }
+    struct Prob {
+        int M, K, N, m, par; const T* A; long a_si, a_sk; long boff;
+        bool center; const T* Ac; int hoff;
+    };
+    __device__ Prob problem(int prob, int m0) const {
+        Prob p;
+        p.m = prob; p.par = 0;
+        p.M = p.m < nt ? jh : 0;
+        p.K = nt - p.m; p.N = 2 * ncol;
+        p.hoff = 0; p.center = false; p.Ac = nullptr;
+        // rows of m' < m: sum (order - m') of them, each jh long
+        const long off = ((long)p.m * order - ((long)p.m * (p.m - 1)) / 2) * jh;
+        p.A = ptab ? ptab + off + m0 : nullptr;
+        p.a_si = 1; p.a_sk = jh;
+        p.boff = (long)p.m * jh * 2 * ncol;
+        return p;
+    }
+    __device__ const T* b_base(int inst) const { return flm + inst * flm_stride; }
+    __device__ unsigned b_off(const Prob& p, int k) const {
+        return (unsigned)(tri_idx(p.m + k, p.m) * 2 * ncol);
+    }
+    __device__ T* c_base(int inst) const { return spec + inst * spec_stride; }
+    __device__ unsigned c_off(const Prob& p, int r) const { return (unsigned)(p.boff + (long)r * 2 * ncol); }
+    // As[r][k] = P_{m+k,m}(z_{m0+r})
+    __device__ void plm_fill(T* As, int AS, const Prob& p, int m0, int rows, int) const {
+        for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+            T* dst = As + (long)r * AS;
+            plm_column(order, p.m, zh[m0 + r], qm, phase, [&](int l, double v) {
+                const int k = l - p.m;
+                if (k < p.K) dst[k] = (T)v;
+            });
+        }
+    }
+};
+
 // The tile loop of one wave, for NSA active 16-row strips (a compile-time
 // count, so the MFMA sequence has no per-strip branches).
 template <int NSA, bool CENTER, class S>
@@ -707,6 +758,23 @@ bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
 bool stream_legendre_fits(int order, int jh) {
     return stream_lds(16 * kNsLegendreFwd, false, jh, sizeof(double)) <= kMaxStreamLds &&
            stream_lds(16 * kNsStreamBwd, true, plm_count(order, 0, 0), sizeof(double)) <= kMaxStreamLds;
+}
+
+int eval_legendre_strips(int nt) {
+    for (int ns : {4, 2, 1})
+        if (stream_lds(16 * ns, false, nt, sizeof(double)) <= kMaxStreamLds) return ns;
+    return 0;
+}
+
+bool eval_legendre(const EvalLegArgs& a, hipStream_t s) {
+    LegEval<double> st{a.ptab, a.flm, a.spec, a.spec_stride, a.flm_stride, a.order, a.nt, a.nlat,
+                       a.ncol, 1, 0, a.z, a.qm, a.phase};
+    switch (a.ns) {
+        case 4: return launch_stream<4>(st, a.nt, a.work, a.nwork, a.nb, 0, s);
+        case 2: return launch_stream<2>(st, a.nt, a.work, a.nwork, a.nb, 0, s);
+        case 1: return launch_stream<1>(st, a.nt, a.work, a.nwork, a.nb, 0, s);
+        default: return false;
+    }
 }
 
 template bool zt_legendre_forward_stream<double>(const ZtGemmArgs&, hipStream_t);

@@ -16,8 +16,6 @@ namespace zgl {
 
 namespace {
 
-__device__ inline double sqrl(int k) { return sqrt((double)k); }
-
 // Legendre P_n and P_n' by the three-term recurrence.
 __device__ void legendre_p_dp(int n, double x, double& p, double& dp) {
     double p0 = 1.0, p1 = x;
@@ -267,6 +265,35 @@ void coeff_kappa_scale(Inst<const double> c, double* out, long os, int nt, int u
     if (nt <= 0 || nb <= 0) return;
     kappa_kernel<<<dim3(nt, nb), 256, 0, s>>>(c, out, os, unnormed);
     check_hip(hipGetLastError(), "kappa scale launch");
+}
+
+namespace {
+// P_lm(z_j) for one (j, m), in the layout [m][l-m][nlat] of the evaluator.
+__global__ void eval_plm_kernel(int order, int nlat, int qm, double phase, const double* z,
+                                double* ptab) {
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int j = (int)(gid % nlat);
+    const int m = (int)(gid / nlat);
+    if (m >= order) return;
+    const long off = ((long)m * order - ((long)m * (m - 1)) / 2) * nlat;
+    plm_column(order, m, z[j], qm, phase, [&](int l, double v) {
+        ptab[off + (long)(l - m) * nlat + j] = v;
+    });
+}
+}  // namespace
+
+void radial_table(int order, int nr, int normed, const double* r, double* rtab, hipStream_t s) {
+    if (order == 0 || nr == 0) return;
+    radial_kernel<<<nr / 64 + 1, 64, 0, s>>>(order, nr, normed, r, rtab);
+    check_hip(hipGetLastError(), "radial table");
+}
+
+void eval_plm_table(int order, int nlat, int qm, double phase, const double* z, double* ptab,
+                    hipStream_t s) {
+    if (order == 0 || nlat == 0) return;
+    const long threads = (long)order * nlat;
+    eval_plm_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(order, nlat, qm, phase, z, ptab);
+    check_hip(hipGetLastError(), "P table");
 }
 
 static long ptab_size(const Geom& g) {
