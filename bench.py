@@ -143,11 +143,13 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy phase (profiling runs): the mixed forms then use the "
                          "plain FP32 forward and no accuracy is reported")
-    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad"],
+    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad", "eval"],
                     default="zernike128",
                     help="zernike128: the headline (64 instances at order 128); ragged: 10,000 "
                          "instances of orders U{8..96}; sh512: 4096 SH instances at L=512, "
-                         "precision report; grad: 256 least-squares gradients at order 64")
+                         "precision report; grad: 256 least-squares gradients at order 64; "
+                         "eval: grid evaluation of 64 order-128 Zernike expansions on "
+                         "256 x 128 x 128 points (and 64 SH expansions, L=512, on 1024 x 512)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the transforms are "
@@ -155,7 +157,7 @@ def main():
               "share of instances; no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
     {"zernike128": run_headline, "ragged": run_ragged, "sh512": run_sh512,
-     "grad": run_grad}[args.config](args)
+     "grad": run_grad, "eval": run_eval}[args.config](args)
 
 
 def run_headline(args):
@@ -509,6 +511,144 @@ def run_grad(args):
                       "target": "r exp(-(a.x)^2), a ~ U(0.5,2)^3, seed 3"},
            "gradient_max_rel_diff_vs_closed_form": err}
     print(json.dumps(out), flush=True)
+
+
+def eval_work(order: int, nlon: int, nlat: int, nrad: int | None):
+    """Algorithmic flops of one evaluation per stage (FP64): a real factor
+    times a complex value is 2 multiply-adds, 4 flops."""
+    ntri = order * (order + 1) // 2
+    cols = nrad if nrad else 1
+    w = {"eval_legendre": 4 * ntri * nlat * cols, "eval_longitude": 4 * order * nlon * nlat * cols}
+    if nrad:
+        w["eval_radial"] = 4 * ((order + 1) * (order + 3) * (2 * order + 1) // 24) * nrad
+    return w
+
+
+def _eval_grid(nlon, nlat, nrad):
+    import numpy as np
+    lon = 2.0 * np.pi * np.arange(nlon) / nlon
+    col = np.pi * (np.arange(nlat) + 0.5) / nlat
+    r = (np.arange(nrad) + 0.5) / nrad if nrad else None
+    return lon, col, r
+
+
+def run_eval(args):
+    """zt::GridEvaluator::evaluate on a plotting-style grid: 64 Zernike
+    expansions at order 128 (coefficients N(0,1)/(1+n), SplitMix64 seeds
+    0..63, conventions unnormed, geo, none) on longitudes 2 pi i / 256,
+    colatitudes pi (j + 1/2) / 128 and radii (k + 1/2) / 128. One step = one
+    zgl_evaluate call for all 64 instances, inputs resident in HBM. Beside it,
+    st::GridEvaluator for 64 SH expansions at L = 512 on 1024 x 512 points.
+    The roofline is the longitude sum's (FP64 MFMA), timed per launch with HIP
+    events; the CPU baseline is zest's GridEvaluator (oracle/_ref) on the host
+    cores over a bounded sample."""
+    import numpy as np
+    import torch
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    dev = torch.device("cuda:0")
+    N, B, nlon, nlat, nrad = 128, 64, 256, 128, 128
+    lon, col, r = _eval_grid(nlon, nlat, nrad)
+    cs = np.stack([synth.zernike_coeffs(N, i) for i in range(B)])
+    dc = torch.from_numpy(cs).to(dev)
+    dl, dt, dr = (torch.from_numpy(a).to(dev) for a in (lon, col, r))
+    vals = torch.empty((B, nlon, nlat, nrad), dtype=torch.float64, device=dev)
+    plan = zest_amd.Plan(zest_amd.ZERNIKE, N, zernike_norm=1, sh_norm=0, sh_phase=-1)
+
+    def step():
+        plan.evaluate(dc, N, dl, dt, dr, values=vals, batch=B)
+
+    t = _timed_steps(step, args.steps, args.warmup)
+    # accuracy: two instances against the oracle on a sub-grid
+    sub = (slice(0, nlon, 16), slice(0, nlat, 8), slice(0, nrad, 8))
+    worst = 0.0
+    for b in (0, B - 1):
+        ref = oracle.zt_evaluate(cs[b], N, lon[sub[0]], col[sub[1]], r[sub[2]])
+        got = vals[b][sub].cpu().numpy()
+        worst = max(worst, float(np.abs(got - ref).max() / np.abs(ref).max()))
+    # per-stage kernel times over a few profiled steps
+    plan.profile(True)
+    for _ in range(3):
+        step()
+    stats = plan.stage_stats()
+    plan.profile(False)
+    work = eval_work(N, nlon, nlat, nrad)
+    stages = {}
+    for name, (launches, inst, ms) in stats.items():
+        if name not in work or launches == 0:
+            continue
+        per_launch = work[name] * inst / launches
+        ach = per_launch / (ms / launches / 1e3) / 1e12
+        stages[name] = {"ms_per_launch": round(ms / launches, 4), "launches": launches,
+                        "instances_per_launch": round(inst / launches, 3), "bound": "mfma",
+                        "achieved": round(ach, 3), "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                        "frac": round(ach / PEAK_TFLOPS["f64"], 4)}
+    lon_st = stages["eval_longitude"]
+    out = {"metric": "Zernike grid evaluations/s (order 128, 256x128x128 points)",
+           "value": round(B * args.steps / t, 2), "unit": "evaluations/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "zernike_grid_evaluate", "order": N, "instances": B,
+                      "grid": [nlon, nlat, nrad], "conventions": "unnormed,geo,none"},
+           "max_rel_err_vs_oracle_subgrid": worst,
+           "stages": stages,
+           "roofline": {"bound": "mfma", "kernel": "eval_lon_kernel",
+                        "achieved": lon_st["achieved"], "peak": PEAK_TFLOPS["f64"],
+                        "unit": "TFLOP/s", "frac": lon_st["frac"], "traffic": None}}
+    del vals
+    torch.cuda.synchronize()
+    # SH: 64 expansions at L = 512 on 1024 x 512
+    L, SB, slon, slat = 512, 64, 1024, 512
+    slon_a, scol_a, _ = _eval_grid(slon, slat, 0)
+    sc = np.stack([synth.sh_coeffs(L, i) for i in range(SB)])
+    dsc = torch.from_numpy(sc).to(dev)
+    dsl, dst = torch.from_numpy(slon_a).to(dev), torch.from_numpy(scol_a).to(dev)
+    svals = torch.empty((SB, slon, slat), dtype=torch.float64, device=dev)
+    splan = zest_amd.Plan(zest_amd.SH, 8)
+
+    def sstep():
+        splan.evaluate(dsc, L, dsl, dst, values=svals, batch=SB)
+
+    ts = _timed_steps(sstep, max(2, args.steps // 4), 1)
+    ref = oracle.st_evaluate(sc[0], L, slon_a[::32], scol_a[::16])
+    serr = float(np.abs(svals[0][::32, ::16].cpu().numpy() - ref).max() / np.abs(ref).max())
+    out["sh"] = {"value": round(SB * max(2, args.steps // 4) / ts, 2), "unit": "evaluations/s",
+                 "order": L, "instances": SB, "grid": [slon, slat],
+                 "max_rel_err_vs_oracle_subgrid": serr}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_eval_baseline(N, cs[0], lon, col, r, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_eval_baseline(order, c, lon, col, r, seconds):
+    """zest's zt::GridEvaluator (or the oracle) on the host, one evaluator per thread."""
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    kind = "reference" if oracle.REF is not None else "port"
+    fn = oracle.ref_zt_evaluate if kind == "reference" else oracle.zt_evaluate
+    counts = [0] * threads
+    t0 = time.perf_counter()
+    deadline = t0 + seconds
+
+    def work(i):
+        while time.perf_counter() < deadline:
+            fn(c, order, lon, col, r, 1, 0, -1)
+            counts[i] += 1
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    elapsed = time.perf_counter() - t0
+    total = sum(counts)
+    return {"value": round(total / elapsed, 4), "unit": "evaluations/s", "cores": threads, "kind": kind,
+            "sample": (f"{total} evaluations of one order-{order} expansion on the "
+                       f"{len(lon)}x{len(col)}x{len(r)} grid in {elapsed:.1f} s, {threads} threads, "
+                       + ("zest GridEvaluator compiled from its sources" if kind == "reference"
+                          else "C restatement of zest"))}
 
 
 if __name__ == "__main__":

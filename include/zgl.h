@@ -217,6 +217,25 @@ zgl_status zgl_evaluate(zgl_plan *plan, const double *coeffs, size_t exp_order,
                         ptrdiff_t coeff_stride, ptrdiff_t value_stride,
                         zgl_mem mem, void *stream);
 
+/* Power spectra, batched. zgl_power_spectrum replaces st::power_spectrum
+ * (include/power_spectra.hpp:80-96) for an SH plan and zt::power_spectrum
+ * (include/power_spectra.hpp:118-141) for a Zernike plan;
+ * zgl_cross_power_spectrum replaces st::cross_power_spectrum
+ * (include/power_spectra.hpp:43-58) and serves Zernike plans too.
+ * Per instance: SH out[l], l < order; Zernike out[((n+1)^2)/4 + l/2] for
+ * l <= n < order (RadialZernikeLayout), every l <= n written (zest's loop
+ * stops at l < n and leaves the l = n entries unwritten). Each entry is
+ * norm * (a_l0 b_l0 + sum_{m>=1} a_lm . b_lm) over the {cos, sin} pairs, norm =
+ * 1 (qm) or 1/(4 pi) (geo) from the plan's SH convention. Strides 0 mean
+ * packed (2 * zgl_coeff_size(order) and the entry count); batch <= 65535. */
+zgl_status zgl_power_spectrum(zgl_plan *plan, const double *coeffs, size_t order,
+                              double *out, size_t batch, ptrdiff_t coeff_stride,
+                              ptrdiff_t out_stride, zgl_mem mem, void *stream);
+zgl_status zgl_cross_power_spectrum(zgl_plan *plan, const double *a, const double *b,
+                                    size_t order, double *out, size_t batch,
+                                    ptrdiff_t coeff_stride, ptrdiff_t out_stride,
+                                    zgl_mem mem, void *stream);
+
 /* Autotuning. No counterpart in the reference (zest has nothing to tune).
  * While enabled (also: environment ZGL_AUTOTUNE=1 at plan creation), the
  * first call of each stage at a new order and arithmetic times the candidate

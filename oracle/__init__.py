@@ -50,6 +50,8 @@ def _load_oracle():
     lib.zo_zt_evaluate.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr,
                                    _c_size, _ptr, _c_size, _ptr]
     lib.zo_st_evaluate.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr, _c_size, _ptr]
+    lib.zo_st_power_spectrum.argtypes = [_c_int, _c_size, _ptr, _ptr, _ptr]
+    lib.zo_zt_power_spectrum.argtypes = [_c_int, _c_size, _ptr, _ptr, _ptr]
     return lib
 
 
@@ -75,6 +77,7 @@ def _load_ref():
     lib.zref_zt_evaluate.argtypes = [_c_int, _c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr,
                                      _c_size, _ptr, _c_size, _ptr]
     lib.zref_st_evaluate.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr, _c_size, _ptr]
+    lib.zref_zt_power_spectrum.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr]
     return lib
 
 
@@ -222,6 +225,25 @@ def st_evaluate(coeffs: np.ndarray, order: int, lon, colat, sh_norm: int = 0,
     return out
 
 
+# ---- power spectra (include/power_spectra.hpp:43-141) ----
+def st_power_spectrum(a: np.ndarray, order: int, sh_norm: int = 0, b: np.ndarray | None = None):
+    """[order] (cross) power spectrum of SH coefficients a (and b)."""
+    a = _f64(a)
+    b = a if b is None else _f64(b)
+    out = np.zeros(order)
+    ORACLE.zo_st_power_spectrum(sh_norm, order, _p(a), _p(b), _p(out))
+    return out
+
+
+def zt_power_spectrum(a: np.ndarray, order: int, sh_norm: int = 0, b: np.ndarray | None = None):
+    """[((order+1)^2)//4] (cross) power spectrum per (n, l) of Zernike coefficients."""
+    a = _f64(a)
+    b = a if b is None else _f64(b)
+    out = np.zeros((order + 1) ** 2 // 4)
+    ORACLE.zo_zt_power_spectrum(sh_norm, order, _p(a), _p(b), _p(out))
+    return out
+
+
 # ---- the upstream zest library (compiled reference) ----
 class ZestZernike:
     """One zest ``zt::GLQTransformer`` instance (keeps its tables between calls)."""
@@ -343,4 +365,13 @@ def ref_st_evaluate(coeffs: np.ndarray, order: int, lon, colat, sh_norm: int = 0
     if out.size:
         REF.zref_st_evaluate(sh_norm, sh_phase, order, _p(coeffs), _p(lon), lon.size, _p(colat),
                              colat.size, _p(out))
+    return out
+
+
+def ref_zt_power_spectrum(coeffs: np.ndarray, order: int, zernike_norm: int = 1, sh_norm: int = 0,
+                          fill: float = np.nan) -> np.ndarray:
+    """zest's zt::power_spectrum; the l = n entries it skips keep `fill`."""
+    coeffs = _f64(coeffs)
+    out = np.full((order + 1) ** 2 // 4, fill)
+    REF.zref_zt_power_spectrum(zernike_norm, sh_norm, order, _p(coeffs), _p(out))
     return out

@@ -17,6 +17,8 @@
 //   zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-154)
 //   zt::GridEvaluator::evaluate (include/grid_evaluator.hpp:227-265)
 //   st::GridEvaluator::evaluate (include/grid_evaluator.hpp:104-133)
+//   zt::power_spectrum (include/power_spectra.hpp:118-141; zest's st power
+//   spectra name a member the SH expansion types lack and do not compile)
 #include <cstddef>
 #include <cstring>
 #include <new>
@@ -29,6 +31,7 @@
 #include "plm_recursion.hpp"
 #include "radial_zernike_recursion.hpp"
 #include "grid_evaluator.hpp"
+#include "power_spectra.hpp"
 
 namespace {
 
@@ -134,6 +137,17 @@ void st_eval(std::size_t order, const double* coeffs, std::span<const double> lo
     const auto v = zest::st::GridEvaluator(order).evaluate(
         Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), order), lon, colat);
     std::memcpy(values, v.data(), v.size() * sizeof(double));
+}
+
+template <ZernikeNorm zn>
+void zt_power(int sh_norm, std::size_t order, const double* coeffs, double* out) {
+    auto run = [&]<SHNorm sn>() {
+        using Exp = zest::zt::RealZernikeSpan<const std::array<double, 2>, zn, sn, SHPhase::none>;
+        zest::zt::power_spectrum(Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), order),
+                                 zest::zt::RadialZernikeSpan<double, zn>(out, order));
+    };
+    if (sh_norm == 1) run.template operator()<SHNorm::qm>();
+    else run.template operator()<SHNorm::geo>();
 }
 
 }  // namespace
@@ -285,6 +299,14 @@ void zref_st_evaluate(int sh_norm, int sh_phase, std::size_t order, const double
         case 2: st_eval<SHNorm::qm, SHPhase::none>(order, coeffs, L, C, values); break;
         default: st_eval<SHNorm::qm, SHPhase::cs>(order, coeffs, L, C, values); break;
     }
+}
+
+// Writes the (n, l < n) entries of zest's zt::power_spectrum; the l = n
+// entries of `out` are left as they were (zest's loop bound).
+void zref_zt_power_spectrum(int zernike_norm, int sh_norm, std::size_t order,
+                            const double* coeffs, double* out) {
+    if (zernike_norm == 1) zt_power<ZernikeNorm::unnormed>(sh_norm, order, coeffs, out);
+    else zt_power<ZernikeNorm::normed>(sh_norm, order, coeffs, out);
 }
 
 }  // extern "C"

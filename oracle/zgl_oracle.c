@@ -766,3 +766,33 @@ void zo_st_evaluate(int sn, int sp, size_t order, const double *coeffs, const do
         }
     free(z); free(plm); free(cs); free(fm);
 }
+
+/* ---- power spectra -----------------------------------------------------------
+ * st::cross_power_spectrum / power_spectrum (include/power_spectra.hpp:43-96):
+ * out[l] = norm * (a_l0 b_l0 + sum_{m>=1} (a_lm b_lm + a'_lm b'_lm)) over the
+ * {cos, sin} pairs, norm = 1 (qm) or 1/(4 pi) (geo). zt::power_spectrum
+ * (include/power_spectra.hpp:118-141): the same per (n, l), out[rad_idx(n,l)];
+ * zest's loop stops at l < n and leaves the l = n entries unwritten, here
+ * every l <= n is written. Pass b = a for the power spectrum. */
+void zo_st_power_spectrum(int sh_norm, size_t order, const double *a, const double *b,
+                          double *out) {
+    const double norm = (sh_norm == 1) ? 1.0 : 1.0 / (4.0 * ZO_PI);
+    for (size_t l = 0; l < order; ++l) {
+        const double *x = a + tri_idx(l, 0) * 2, *y = b + tri_idx(l, 0) * 2;
+        double s = x[0] * y[0];
+        for (size_t m = 1; m <= l; ++m) s += x[2 * m] * y[2 * m] + x[2 * m + 1] * y[2 * m + 1];
+        out[l] = norm * s;
+    }
+}
+
+void zo_zt_power_spectrum(int sh_norm, size_t order, const double *a, const double *b,
+                          double *out) {
+    const double norm = (sh_norm == 1) ? 1.0 : 1.0 / (4.0 * ZO_PI);
+    for (size_t n = 0; n < order; ++n)
+        for (size_t l = n & 1; l <= n; l += 2) {
+            const double *x = a + tet_idx(n, l, 0) * 2, *y = b + tet_idx(n, l, 0) * 2;
+            double s = x[0] * y[0];
+            for (size_t m = 1; m <= l; ++m) s += x[2 * m] * y[2 * m] + x[2 * m + 1] * y[2 * m + 1];
+            out[rad_idx(n, l)] = norm * s;
+        }
+}

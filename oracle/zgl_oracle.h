@@ -50,6 +50,10 @@ void zo_zt_evaluate(int zernike_norm, int sh_norm, int sh_phase, size_t order,
 void zo_st_evaluate(int sh_norm, int sh_phase, size_t order, const double *coeffs,
                     const double *lon, size_t nlon, const double *colat, size_t nlat,
                     double *values);
+void zo_st_power_spectrum(int sh_norm, size_t order, const double *a, const double *b,
+                          double *out);
+void zo_zt_power_spectrum(int sh_norm, size_t order, const double *a, const double *b,
+                          double *out);
 
 #ifdef __cplusplus
 }

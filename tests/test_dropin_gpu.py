@@ -1,11 +1,14 @@
-"""zest's own transformer test programs, compiled unchanged against the drop-in
-headers (include/zest_amd/) and libzgl.so, run on the GPU.
+"""zest's own transformer and grid-evaluator test programs, compiled unchanged
+against the drop-in headers (include/zest_amd/) and libzgl.so, run on the GPU.
 
 The programs are zest's tests/test_zernike_glq_transformer.cpp (known-answer
 forward and backward transforms of single basis functions at order 6 and an
 order-100 round trip at 1e-13, all for (normed, geo, none) and
 (unnormed, qm, cs)) and tests/test_sh_glq_transformer.cpp (the same for
-spherical harmonics over both grid layouts and all four conventions). They are
+spherical harmonics over both grid layouts and all four conventions) and
+tests/test_grid_evaluator.cpp (constant, Y_10, Y_31 + Y_4,-3, Z_33,-2 + Z_531
+on grids through both poles, r = 0 and r = 1, at 1e-13, for every
+convention combination). They are
 built by tests/dropin/Makefile where the zest sources are present; the test
 skips when the binaries were not built.
 """
@@ -19,7 +22,8 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "_bin")
 
 
-@pytest.mark.parametrize("name", ["zest_test_zernike_glq_transformer", "zest_test_sh_glq_transformer"])
+@pytest.mark.parametrize("name", ["zest_test_zernike_glq_transformer", "zest_test_sh_glq_transformer",
+                                  "zest_test_grid_evaluator"])
 def test_zest_transformer_tests_pass_on_gpu(name):
     exe = os.path.join(BIN, name)
     if not os.path.exists(exe):

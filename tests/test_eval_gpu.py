@@ -199,15 +199,33 @@ def test_plm_in_kernel_matches_table(device, monkeypatch):
     assert rel(b[0], oracle.zt_evaluate(c, order, lon, col, r)) < 1e-12
 
 
-def test_order_200_default_in_kernel(device):
-    """Order 200 takes the in-kernel P path and the tiled radial kernel."""
+@pytest.mark.parametrize("plm", ["table", "kernel"])
+def test_order_200(device, monkeypatch, plm):
+    """Order 200: the tiled radial kernel (beyond the streaming one's 64 n per
+    l), two or one 16-point strips in the Legendre stage, both P sources."""
     import oracle
     import zest_amd
     from zest_amd import synth
     order = 200
+    monkeypatch.setenv("ZGL_PLM", plm)
     lon, col, r = _points(8, 21, 17, 12)
     c = synth.zernike_coeffs(order, 1)
     plan = zest_amd.Plan(zest_amd.ZERNIKE, 4)
     dc, dl, dt, dr = _dev(device, c, lon, col, r)
     v = plan.evaluate(dc, order, dl, dt, dr).cpu().numpy()[0]
     assert rel(v, oracle.zt_evaluate(c, order, lon, col, r)) < 1e-12
+
+
+def test_sh_order_700_one_strip(device):
+    """L = 700: the Legendre stage's P block fits only as one 16-point strip."""
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    order = 700
+    lon, col, _ = _points(9, 19, 40, 0)
+    cs = np.stack([synth.sh_coeffs(order, i) for i in range(3)])
+    plan = zest_amd.Plan(zest_amd.SH, 4)
+    dc, dl, dt = _dev(device, cs, lon, col)
+    v = plan.evaluate(dc, order, dl, dt, batch=3).cpu().numpy()
+    for b in range(3):
+        assert rel(v[b], oracle.st_evaluate(cs[b], order, lon, col)) < 1e-12

@@ -469,3 +469,80 @@ def test_sh_evaluate_matches_compiled_zest(conv, order):
     c = synth.sh_coeffs(order, 2 + order)
     assert rel(oracle.st_evaluate(c, order, lon, col, sn, sp),
                oracle.ref_st_evaluate(c, order, lon, col, sn, sp)) < 1e-13
+
+
+# ---------------------------------------------------------------- power spectra
+@needs_ref
+@pytest.mark.parametrize("zn", [0, 1])
+@pytest.mark.parametrize("sn", [0, 1])
+@pytest.mark.parametrize("order", [1, 2, 9, 40])
+def test_zernike_power_spectrum_matches_compiled_zest(zn, sn, order):
+    """zest writes the (n, l < n) entries (its loop bound skips l = n)."""
+    c = synth.zernike_coeffs(order, order)
+    ref = oracle.ref_zt_power_spectrum(c, order, zn, sn)
+    got = oracle.zt_power_spectrum(c, order, sn)
+    written = ~np.isnan(ref)
+    assert written.sum() == (order + 1) ** 2 // 4 - order
+    if written.any():
+        assert np.abs(got[written] - ref[written]).max() <= 1e-15 * np.abs(ref[written]).max()
+
+
+@pytest.mark.parametrize("sn", [0, 1])
+def test_power_spectrum_known_answers(sn):
+    norm = 1.0 if sn == 1 else 1 / (4 * math.pi)
+    c = np.zeros((oracle.st_num_coeffs(5), 2))
+    c[3 * 4 // 2 + 2] = (0.6, 0.8)
+    c[2 * 3 // 2 + 0] = (0.5, 7.0)  # the sin part at m = 0 does not count
+    p = oracle.st_power_spectrum(c, 5, sn)
+    assert np.allclose(p, norm * np.array([0, 0, 0.25, 1.0, 0]), rtol=0, atol=1e-16)
+    z = np.zeros((oracle.zt_num_coeffs(5), 2))
+    tet = lambda n, l, m: (n + 1) * (n + 3) * (2 * n + 1) // 24 + l * l // 4 + m
+    z[tet(4, 4, 3)] = (3.0, 4.0)
+    z[tet(4, 4, 0)] = (1.0, 9.0)
+    z[tet(3, 1, 1)] = (2.0, 0.0)
+    p = oracle.zt_power_spectrum(z, 5, sn)
+    expect = np.zeros_like(p)
+    expect[(5 * 5) // 4 + 2] = 26.0
+    expect[(4 * 4) // 4 + 0] = 4.0
+    assert np.allclose(p, norm * expect, rtol=0, atol=1e-15)
+    # cross spectrum: bilinear, symmetric, and equal to the power spectrum on the diagonal
+    a, b = synth.sh_coeffs(7, 1), synth.sh_coeffs(7, 2)
+    pab = oracle.st_power_spectrum(a, 7, sn, b)
+    assert np.allclose(pab, oracle.st_power_spectrum(b, 7, sn, a), rtol=1e-15, atol=0)
+    assert np.allclose(oracle.st_power_spectrum(a + b, 7, sn),
+                       oracle.st_power_spectrum(a, 7, sn) + 2 * pab + oracle.st_power_spectrum(b, 7, sn),
+                       rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("sn", [0, 1])
+def test_sh_power_spectrum_parseval(sn):
+    """sum_l P(l) = int f^2 dOmega / (4 pi for geo: one more 4 pi from the
+    basis norm), by quadrature of the oracle's backward grid."""
+    L = 12
+    c = synth.sh_coeffs(L, 5)
+    c[[l * (l + 1) // 2 for l in range(L)], 1] = 0.0
+    g = oracle.st_backward(c, L, sh_norm=sn, sh_phase=1)
+    nlon, nlat = oracle.st_shape(L, 0)
+    _, w = oracle.gl_nodes_weights(nlat)
+    integral = float((g * g * w[None, :]).sum() * 2 * math.pi / nlon)
+    total = oracle.st_power_spectrum(c, L, sn).sum()
+    expect = integral if sn == 1 else integral / (4 * math.pi) ** 2
+    assert abs(total - expect) <= 1e-13 * expect
+
+
+@pytest.mark.parametrize("sn", [0, 1])
+def test_zernike_power_spectrum_parseval(sn):
+    """Normed Zernike functions are orthonormal in the ball (qm) or carry 4 pi
+    (geo): sum_nl P(n, l) = int f^2 dV (/ (4 pi)^2 for geo)."""
+    N = 10
+    c = synth.zernike_coeffs(N, 6)
+    g = oracle.zt_backward(c, N, zernike_norm=0, sh_norm=sn, sh_phase=-1)
+    nphi, nth, nr = oracle.zt_shape(N)
+    _, wz = oracle.gl_nodes_weights(nth)
+    x, wx = oracle.gl_nodes_weights(nr)
+    r = 0.5 * (1 + x)
+    W = (2 * math.pi / nphi) * wz[None, :, None] * (0.5 * wx * r * r)[None, None, :]
+    integral = float((g * g * W).sum())
+    total = oracle.zt_power_spectrum(c, N, sn).sum()
+    expect = integral if sn == 1 else integral / (4 * math.pi) ** 2
+    assert abs(total - expect) <= 1e-13 * expect

@@ -80,6 +80,10 @@ def _load():
     lib.zgl_backward_grid32.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_evaluate.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, sz, pd, pd, i32, vp]
     lib.zgl_evaluate.restype = i32
+    lib.zgl_power_spectrum.argtypes = [vp, vp, sz, vp, sz, pd, pd, i32, vp]
+    lib.zgl_power_spectrum.restype = i32
+    lib.zgl_cross_power_spectrum.argtypes = [vp, vp, vp, sz, vp, sz, pd, pd, i32, vp]
+    lib.zgl_cross_power_spectrum.restype = i32
     lib.zgl_backward_grid32.restype = i32
     lib.zgl_last_error_string.argtypes = []
     lib.zgl_last_error_string.restype = ctypes.c_char_p
@@ -114,6 +118,7 @@ EXPORTS = (
     "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
     "zgl_forward_ragged", "zgl_backward_ragged", "zgl_ragged_lane_costs",
     "zgl_plan_autotune", "zgl_plan_tuning", "zgl_evaluate",
+    "zgl_power_spectrum", "zgl_cross_power_spectrum",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -288,6 +293,33 @@ class Plan:
         _check(LIB.zgl_evaluate(self._h, cp, order, lp, nlon, tp, nlat, rp or None, nrad, vp, batch,
                                 coeff_stride, value_stride, vm, _stream_ptr(stream, vm)))
         return values
+
+    def power_spectrum(self, coeffs, order: int, out=None, batch: int = 1, other=None,
+                       coeff_stride: int = 0, out_stride: int = 0, stream=None):
+        """(Cross) power spectra of `batch` expansions (zgl_power_spectrum, or
+        zgl_cross_power_spectrum with `other`); returns `out`, allocated like
+        `coeffs` when not given, shaped (batch, entries)."""
+        nent = (order + 1) ** 2 // 4 if self.kind == ZERNIKE else order
+        if out is None:
+            if isinstance(coeffs, np.ndarray):
+                out = np.empty((batch, nent))
+            else:
+                import torch
+                out = torch.empty((batch, nent), dtype=torch.float64, device=coeffs.device)
+        ap, am, _ = _ptr_and_mem(coeffs)
+        op, om, _ = _ptr_and_mem(out)
+        if other is None:
+            if am != om:
+                raise ValueError("coefficients and output must live in the same memory")
+            _check(LIB.zgl_power_spectrum(self._h, ap, order, op, batch, coeff_stride, out_stride, om,
+                                          _stream_ptr(stream, om)))
+        else:
+            bp, bm, _ = _ptr_and_mem(other)
+            if len({am, bm, om}) != 1:
+                raise ValueError("coefficients and output must live in the same memory")
+            _check(LIB.zgl_cross_power_spectrum(self._h, ap, bp, order, op, batch, coeff_stride,
+                                                out_stride, om, _stream_ptr(stream, om)))
+        return out
 
     @staticmethod
     def _jobs(jobs):

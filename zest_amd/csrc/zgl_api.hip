@@ -207,9 +207,12 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
     return c;
 }
 
-enum Stage { ST_FFT_FWD, ST_LEG_FWD, ST_RAD_FWD, ST_RAD_BWD, ST_LEG_BWD, ST_FFT_BWD, ST_COUNT };
+enum Stage { ST_FFT_FWD, ST_LEG_FWD, ST_RAD_FWD, ST_RAD_BWD, ST_LEG_BWD, ST_FFT_BWD,
+             ST_EVAL_TABLES, ST_EVAL_RAD, ST_EVAL_LEG, ST_EVAL_LON, ST_COUNT };
 const char* const kStageNames[ST_COUNT] = {"fft_forward", "legendre_forward", "radial_forward",
-                                           "radial_backward", "legendre_backward", "fft_backward"};
+                                           "radial_backward", "legendre_backward", "fft_backward",
+                                           "eval_tables", "eval_radial", "eval_legendre",
+                                           "eval_longitude"};
 
 hipEvent_t pooled_event(zgl_plan* p) {
     if (!p->pool.empty()) {
@@ -1023,8 +1026,13 @@ extern "C" zgl_status zgl_evaluate(zgl_plan* plan, const double* coeffs, size_t 
                    (int)exp_order, Inst<const double>{coeffs, cs, nullptr},
                    longitudes, (int)nlon, colatitudes, (int)nlat, zt ? radii : nullptr, (int)nrad,
                    values, vs, (int)batch};
+        EvalTimer timer = nullptr;
+        if (plan->profiling)
+            timer = [&](int stage, int nb, const std::function<void()>& f) {
+                timed(plan, (Stage)(ST_EVAL_TABLES + stage), nb, s, f);
+            };
         if (mem == ZGL_MEM_DEVICE) {
-            evaluate(plan->eval, e, s);
+            evaluate(plan->eval, e, s, timer);
             check_hip(hipGetLastError(), "evaluate");
             return ZGL_SUCCESS;
         }
@@ -1049,11 +1057,73 @@ extern "C" zgl_status zgl_evaluate(zgl_plan* plan, const double* coeffs, size_t 
         e.radii = zt ? dp + nlon + nlat : nullptr;
         e.values = plan->dgrid;
         e.value_stride = npts;
-        evaluate(plan->eval, e, s);
+        evaluate(plan->eval, e, s, timer);
         check_hip(hipMemcpy2DAsync(values, vs * sizeof(double), plan->dgrid, npts * sizeof(double),
                                    npts * sizeof(double), batch, hipMemcpyDeviceToHost, s),
                   "value download");
         check_hip(hipStreamSynchronize(s), "evaluate");
         return ZGL_SUCCESS;
     });
+}
+
+// ------------------------------------------------------------ power spectra
+namespace zgl {
+namespace {
+zgl_status spectrum_entry(zgl_plan* plan, const double* a, const double* b, size_t order, double* out,
+                          size_t batch, ptrdiff_t coeff_stride, ptrdiff_t out_stride, zgl_mem mem,
+                          void* stream) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
+    if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    const long csz = 2 * (long)coeff_size(plan, order);
+    const long nent = spectrum_size(plan->kind, (int)order);
+    const long cs = coeff_stride ? (long)coeff_stride : csz;
+    const long os = out_stride ? (long)out_stride : nent;
+    if (batch > 1 && (cs < csz || os < nent)) return fail(ZGL_ERROR_INVALID_VALUE, "instance strides overlap");
+    if (batch > 65535) return fail(ZGL_ERROR_INVALID_VALUE, "batch exceeds 65535 instances per call");
+    if (order == 0 || batch == 0) return ZGL_SUCCESS;
+    if (!a || !b || !out) return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    return guarded([&]() -> zgl_status {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (mem == ZGL_MEM_DEVICE) {
+            power_spectrum(plan->kind, plan->shnorm, (int)order, Inst<const double>{a, cs, nullptr},
+                           Inst<const double>{b, cs, nullptr}, out, os, (int)batch, s);
+            return ZGL_SUCCESS;
+        }
+        const bool cross = a != b;
+        ensure(plan->dcoef, plan->dcoef_cap, (size_t)csz * batch * (cross ? 2 : 1));
+        ensure(plan->dgrid, plan->dgrid_cap, (size_t)nent * batch);
+        double* da = plan->dcoef;
+        double* db = cross ? plan->dcoef + (size_t)csz * batch : da;
+        check_hip(hipMemcpy2DAsync(da, csz * sizeof(double), a, cs * sizeof(double), csz * sizeof(double),
+                                   batch, hipMemcpyHostToDevice, s),
+                  "coefficient upload");
+        if (cross)
+            check_hip(hipMemcpy2DAsync(db, csz * sizeof(double), b, cs * sizeof(double), csz * sizeof(double),
+                                       batch, hipMemcpyHostToDevice, s),
+                      "coefficient upload");
+        power_spectrum(plan->kind, plan->shnorm, (int)order, Inst<const double>{da, csz, nullptr},
+                       Inst<const double>{db, csz, nullptr}, plan->dgrid, nent, (int)batch, s);
+        check_hip(hipMemcpy2DAsync(out, os * sizeof(double), plan->dgrid, nent * sizeof(double),
+                                   nent * sizeof(double), batch, hipMemcpyDeviceToHost, s),
+                  "spectrum download");
+        check_hip(hipStreamSynchronize(s), "power spectrum");
+        return ZGL_SUCCESS;
+    });
+}
+}  // namespace
+}  // namespace zgl
+
+extern "C" zgl_status zgl_power_spectrum(zgl_plan* plan, const double* coeffs, size_t order, double* out,
+                                         size_t batch, ptrdiff_t coeff_stride, ptrdiff_t out_stride,
+                                         zgl_mem mem, void* stream) {
+    return zgl::spectrum_entry(plan, coeffs, coeffs, order, out, batch, coeff_stride, out_stride, mem,
+                               stream);
+}
+
+extern "C" zgl_status zgl_cross_power_spectrum(zgl_plan* plan, const double* a, const double* b,
+                                               size_t order, double* out, size_t batch,
+                                               ptrdiff_t coeff_stride, ptrdiff_t out_stride, zgl_mem mem,
+                                               void* stream) {
+    return zgl::spectrum_entry(plan, a, b, order, out, batch, coeff_stride, out_stride, mem, stream);
 }

@@ -131,6 +131,18 @@ __global__ __launch_bounds__(256, 2) void eval_lon_kernel(LonArgs a) {
         colok[c] = colv[c] < a.ncols;
     }
 
+    // B values of m step 4s + g: loaded one step ahead, across chunk borders
+    auto load = [&](int mb, d2 (&b)[kLonNB]) {
+        const int m = mb + g;
+#pragma unroll
+        for (int c = 0; c < kLonNB; ++c) {
+            b[c] = d2{0.0, 0.0};
+            if (m < a.nm && colok[c])
+                b[c] = *reinterpret_cast<const d2*>(sb + (long)m * a.row_stride + 2L * colv[c]);
+        }
+    };
+    d2 bn[kLonNB];
+    load(0, bn);
     for (int kc = 0; kc < a.nm; kc += kLonKC) {
         __syncthreads();  // the previous chunk's A reads are done
         for (int e = tid; e < kLonRows * kLonKC; e += 256) {
@@ -142,33 +154,30 @@ __global__ __launch_bounds__(256, 2) void eval_lon_kernel(LonArgs a) {
         }
         __syncthreads();
         const int steps = (min(kLonKC, a.nm - kc) + 3) >> 2;
-        d2 bn[kLonNB];
-        auto load = [&](int s, d2 (&b)[kLonNB]) {
-            const int m = kc + 4 * s + g;
-#pragma unroll
-            for (int c = 0; c < kLonNB; ++c) {
-                b[c] = d2{0.0, 0.0};
-                if (m < a.nm && colok[c])
-                    b[c] = *reinterpret_cast<const d2*>(sb + (long)m * a.row_stride + 2L * colv[c]);
-            }
-        };
-        load(0, bn);
 #pragma unroll
         for (int s = 0; s < kLonKC / 4; ++s) {
             if (s < steps) {
                 d2 b[kLonNB];
 #pragma unroll
                 for (int c = 0; c < kLonNB; ++c) b[c] = bn[c];
-                if (s + 1 < steps) load(s + 1, bn);
+                const int next = kc + 4 * (s + 1);
+                if (next < a.nm) load(next, bn);
+                d2 x[kLonMB];
 #pragma unroll
-                for (int q = 0; q < kLonMB; ++q) {
-                    const d2 x = *reinterpret_cast<const d2*>(As + (16 * q + j) * kLonRS + 2 * (4 * s + g));
+                for (int q = 0; q < kLonMB; ++q)
+                    x[q] = *reinterpret_cast<const d2*>(As + (16 * q + j) * kLonRS + 2 * (4 * s + g));
+                // all cos products, then all sin products: no MFMA waits on
+                // the one just issued
 #pragma unroll
-                    for (int c = 0; c < kLonNB; ++c) {
-                        acc[q][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[0], b[c][0], acc[q][c], 0, 0, 0);
-                        acc[q][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[1], b[c][1], acc[q][c], 0, 0, 0);
-                    }
-                }
+                for (int q = 0; q < kLonMB; ++q)
+#pragma unroll
+                    for (int c = 0; c < kLonNB; ++c)
+                        acc[q][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[q][0], b[c][0], acc[q][c], 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < kLonMB; ++q)
+#pragma unroll
+                    for (int c = 0; c < kLonNB; ++c)
+                        acc[q][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[q][1], b[c][1], acc[q][c], 0, 0, 0);
             }
         }
     }
@@ -225,11 +234,14 @@ int4* upload(const std::vector<int4>& w, hipStream_t s) {
     return d;
 }
 
-bool plm_in_kernel(int order) {
+// P recomputed in the Legendre kernel only when its table would be large:
+// the evaluator's table is read once per call, while each workgroup would
+// otherwise rerun the column recursion for its points.
+bool plm_in_kernel(long table_doubles) {
     const char* e = std::getenv("ZGL_PLM");
     if (e && std::strcmp(e, "kernel") == 0) return true;
     if (e && std::strcmp(e, "table") == 0) return false;
-    return order >= 192;
+    return table_doubles * sizeof(double) > (size_t(1) << 30);
 }
 
 constexpr size_t kEvalScratchBudget = size_t(2) << 30;  // bytes of flm + S per chunk
@@ -245,7 +257,11 @@ void EvalCache::release() {
     *this = EvalCache{};
 }
 
-void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s) {
+void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s, const EvalTimer& timer) {
+    auto run = [&](int stage, int nb, const std::function<void()>& f) {
+        if (timer) timer(stage, nb, f);
+        else f();
+    };
     const bool zt = e.kind == ZGL_ZERNIKE;
     const int N = e.order;
     const long npts = (long)e.nlon * e.nlat * (zt ? e.nrad : 1);
@@ -257,7 +273,6 @@ void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s) {
         return;
     }
     const int ns = eval_legendre_strips(N);
-    if (ns == 0) throw Error{ZGL_ERROR_NOT_SUPPORTED, "order too large for the evaluator"};
     const long ntri = (long)N * (N + 1) / 2;
 
     // columns per point (Zernike: padded radii) and instances per chunk
@@ -282,25 +297,23 @@ void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s) {
     grow(ec.pts, ec.pts_cap, (size_t)e.nlat + ncol);
     double* z = ec.pts;
     double* rpad = ec.pts + e.nlat;
-    const int tp = std::max(e.nlat, ncol);
-    eval_points_kernel<<<(tp + 255) / 256, 256, 0, s>>>(e.colat, e.nlat, e.radii, e.nrad, ncol, z,
-                                                       zt ? rpad : nullptr);
-    check_hip(hipGetLastError(), "points");
-    const bool in_kernel = plm_in_kernel(N);
-    if (!in_kernel) {
-        grow(ec.ptab, ec.ptab_cap, (size_t)ntri * e.nlat);
-        eval_plm_table(N, e.nlat, e.qm, e.phase, z, ec.ptab, s);
-    }
+    const bool in_kernel = plm_in_kernel(ntri * e.nlat);
+    if (!in_kernel) grow(ec.ptab, ec.ptab_cap, (size_t)ntri * e.nlat);
     grow(ec.trig, ec.trig_cap, (size_t)2 * e.nlon * N);
-    {
+    if (zt) grow(ec.rtab, ec.rtab_cap, (size_t)((N + 1) * (N + 1) / 4) * ncol);
+    run(0, e.nb, [&] {
+        const int tp = std::max(e.nlat, ncol);
+        eval_points_kernel<<<(tp + 255) / 256, 256, 0, s>>>(e.colat, e.nlat, e.radii, e.nrad, ncol, z,
+                                                           zt ? rpad : nullptr);
+        check_hip(hipGetLastError(), "points");
+        if (!in_kernel) eval_plm_table(N, e.nlat, e.qm, e.phase, z, ec.ptab, s);
         const long n = (long)e.nlon * N;
         eval_trig_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(e.lon, e.nlon, N, ec.trig);
         check_hip(hipGetLastError(), "trig table");
-    }
+        if (zt) radial_table(N, ncol, e.znorm == ZGL_ZNORM_NORMED ? 1 : 0, rpad, ec.rtab, s);
+    });
     OrderTables et{};
     if (zt) {
-        grow(ec.rtab, ec.rtab_cap, (size_t)((N + 1) * (N + 1) / 4) * ncol);
-        radial_table(N, ncol, e.znorm == ZGL_ZNORM_NORMED ? 1 : 0, rpad, ec.rtab, s);
         auto it = ec.rad_work.find(N);
         if (it == ec.rad_work.end()) {
             std::vector<int4> rb;
@@ -329,19 +342,23 @@ void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s) {
     for (int b0 = 0; b0 < e.nb; b0 += cb) {
         const int n = std::min(cb, e.nb - b0);
         const Inst<const double> ci = e.coeffs.from(b0);
-        if (zt) {
-            ZtGemmArgs ra{&et, N, 0, nullptr, 0, ec.flm, flm_stride / 2,
-                          Inst<double>{const_cast<double*>(ci.base), ci.stride, ci.offs}, n, 0};
-            zt_radial_backward<double>(ra, s);
-        } else {
-            const long cnt = ntri * ncol;
-            eval_gather_kernel<<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(ci, n, ntri, ncol, ec.flm);
-            check_hip(hipGetLastError(), "gather");
-        }
+        run(1, n, [&] {
+            if (zt) {
+                ZtGemmArgs ra{&et, N, 0, nullptr, 0, ec.flm, flm_stride / 2,
+                              Inst<double>{const_cast<double*>(ci.base), ci.stride, ci.offs}, n, 0};
+                zt_radial_backward<double>(ra, s);
+            } else {
+                const long cnt = ntri * ncol;
+                eval_gather_kernel<<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(ci, n, ntri, ncol, ec.flm);
+                check_hip(hipGetLastError(), "gather");
+            }
+        });
         EvalLegArgs la{in_kernel ? nullptr : ec.ptab, z, e.qm, e.phase, ec.flm, flm_stride,
                        ec.spec, spec_stride, N, N, e.nlat, ncol, zt ? n : 1,
                        lw->second.first, lw->second.second, ns};
-        if (!eval_legendre(la, s)) throw Error{ZGL_ERROR_NOT_SUPPORTED, "evaluator Legendre stage"};
+        run(2, n, [&] {
+            if (!eval_legendre(la, s)) throw Error{ZGL_ERROR_NOT_SUPPORTED, "evaluator Legendre stage"};
+        });
         LonArgs lo{};
         lo.trig = ec.trig;
         lo.spec = ec.spec;
@@ -366,7 +383,7 @@ void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s) {
             lo.out_lat = 1;
             lo.out_k = e.value_stride;
         }
-        lon_sum(lo, zt ? n : 1, s);
+        run(3, n, [&] { lon_sum(lo, zt ? n : 1, s); });
     }
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <tuple>
@@ -213,8 +214,9 @@ struct EvalLegArgs {
     const int4* work; int nwork; int ns;
 };
 bool eval_legendre(const EvalLegArgs& a, hipStream_t s);
-// The 16-point strips per workgroup (4, 2 or 1) whose P block fits the LDS
-// budget at this order; 0 when none does.
+// The 16-point strips per workgroup (4 or 2) whose P block of every l fits
+// the LDS budget at this order, else 1: one strip, the l range summed in
+// passes that add to S.
 int eval_legendre_strips(int nt);
 // R_nl(r_k) table [((n+1)^2>>2)+(l>>1)][nr] at nr given radii (device).
 void radial_table(int order, int nr, int normed, const double* r, double* rtab, hipStream_t s);
@@ -250,7 +252,18 @@ struct EvalCache {
     std::map<int, std::pair<int4*, int>> rad_work;                      // order
     void release();
 };
-void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s);
+// Optional stage brackets (the plan's stage timing): timer(stage, instances,
+// launch) runs `launch`; stages 0 tables, 1 radial, 2 Legendre, 3 longitude.
+using EvalTimer = std::function<void(int, int, const std::function<void()>&)>;
+void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s, const EvalTimer& timer = nullptr);
+
+// ---- power spectra (zgl_spectra.hip) ----
+// Entries per instance: ((order+1)^2)/4 for Zernike ([rad_idx(n,l)]), order for SH.
+long spectrum_size(zgl_kind kind, int order);
+// out[b][e] = norm * sum over the m of entry e of a_b . b_b ({cos, sin} pairs,
+// the sin part of m = 0 excluded); b = a for the power spectrum.
+void power_spectrum(zgl_kind kind, int sh_norm, int order, Inst<const double> a,
+                    Inst<const double> b, double* out, long out_stride, int nb, hipStream_t s);
 
 // ---- elementwise helpers (zgl_tables.hip) ----
 // r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in elements.

@@ -27,6 +27,8 @@
 // while the MFMAs of the current one run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "zgl_internal.hpp"
 #include "zgl_plm.hpp"
 
@@ -72,6 +74,7 @@ struct LegFwd {
     int order, nt, jh, ncol, G, as;
     const double* zh; int qm; double phase;  // for P blocks recomputed in the kernel (ptab null)
     static constexpr bool kCenter = false;
+    static constexpr bool kAccum = false;
     __device__ int center_lds_row() const { return 16 * kNsLegendreFwd; }
     struct Prob {
         int M, K, N, m, par; const T* A; long a_si, a_sk; long boff;
@@ -123,6 +126,7 @@ struct LegBwd {
     int order, nt, jh, ncol, G, as, nth;
     const double* zh; int qm; double phase;  // for P blocks recomputed in the kernel (ptab null)
     static constexpr bool kCenter = true;
+    static constexpr bool kAccum = false;
     __device__ int center_lds_row() const { return 16 * kNsStreamBwd; }
     struct Prob {
         int M, K, N, m, par; const T* A; long a_si, a_sk; long boff;
@@ -180,7 +184,10 @@ struct LegEval {
     long spec_stride, flm_stride;
     int order, nt, jh, ncol, G, as;  // jh: the number of points nlat
     const double* zh; int qm; double phase;
+    int l0, kchunk;  // this pass sums l = m + l0 .. m + l0 + kchunk - 1
+    int accum;       // add to S (passes after the first)
     static constexpr bool kCenter = false;
+    static constexpr bool kAccum = true;
     __device__ int center_lds_row() const { return 0; }
     struct Prob {
         int M, K, N, m, par; const T* A; long a_si, a_sk; long boff;
@@ -189,11 +196,12 @@ struct LegEval {
     __device__ Prob problem(int prob, int m0) const {
         Prob p;
         p.m = prob; p.par = 0;
-        p.M = p.m < nt ? jh : 0;
-        p.K = nt - p.m; p.N = 2 * ncol;
+        p.K = min(kchunk, nt - p.m - l0);
+        p.M = p.K > 0 ? jh : 0;
+        p.N = 2 * ncol;
         p.hoff = 0; p.center = false; p.Ac = nullptr;
         // rows of m' < m: sum (order - m') of them, each jh long
-        const long off = ((long)p.m * order - ((long)p.m * (p.m - 1)) / 2) * jh;
+        const long off = ((long)p.m * order - ((long)p.m * (p.m - 1)) / 2 + l0) * jh;
         p.A = ptab ? ptab + off + m0 : nullptr;
         p.a_si = 1; p.a_sk = jh;
         p.boff = (long)p.m * jh * 2 * ncol;
@@ -201,17 +209,17 @@ struct LegEval {
     }
     __device__ const T* b_base(int inst) const { return flm + inst * flm_stride; }
     __device__ unsigned b_off(const Prob& p, int k) const {
-        return (unsigned)(tri_idx(p.m + k, p.m) * 2 * ncol);
+        return (unsigned)(tri_idx(p.m + l0 + k, p.m) * 2 * ncol);
     }
     __device__ T* c_base(int inst) const { return spec + inst * spec_stride; }
     __device__ unsigned c_off(const Prob& p, int r) const { return (unsigned)(p.boff + (long)r * 2 * ncol); }
-    // As[r][k] = P_{m+k,m}(z_{m0+r})
+    // As[r][k] = P_{m+l0+k,m}(z_{m0+r})
     __device__ void plm_fill(T* As, int AS, const Prob& p, int m0, int rows, int) const {
         for (int r = threadIdx.x; r < rows; r += blockDim.x) {
             T* dst = As + (long)r * AS;
             plm_column(order, p.m, zh[m0 + r], qm, phase, [&](int l, double v) {
-                const int k = l - p.m;
-                if (k < p.K) dst[k] = (T)v;
+                const int k = l - p.m - l0;
+                if (k >= 0 && k < p.K) dst[k] = (T)v;
             });
         }
     }
@@ -304,7 +312,10 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
                     V o;
 #pragma unroll
                     for (int v = 0; v < VW; ++v) o[v] = acc[q][v][r];
-                    *reinterpret_cast<V*>(cb + (st.c_off(P, m0 + i) + col)) = o;
+                    V* dst = reinterpret_cast<V*>(cb + (st.c_off(P, m0 + i) + col));
+                    if constexpr (S::kAccum)
+                        if (st.accum) o += *dst;
+                    *dst = o;
                 }
             }
 #pragma unroll
@@ -760,21 +771,31 @@ bool stream_legendre_fits(int order, int jh) {
            stream_lds(16 * kNsStreamBwd, true, plm_count(order, 0, 0), sizeof(double)) <= kMaxStreamLds;
 }
 
+// Longest l range of one pass: the P block of one 16-point strip fills the
+// LDS budget (80 KB / (16 * 8 B) = 640 values per row, less the padding).
+constexpr int kEvalKChunk = 636;
+
 int eval_legendre_strips(int nt) {
-    for (int ns : {4, 2, 1})
+    for (int ns : {4, 2})
         if (stream_lds(16 * ns, false, nt, sizeof(double)) <= kMaxStreamLds) return ns;
-    return 0;
+    return 1;
 }
 
 bool eval_legendre(const EvalLegArgs& a, hipStream_t s) {
-    LegEval<double> st{a.ptab, a.flm, a.spec, a.spec_stride, a.flm_stride, a.order, a.nt, a.nlat,
-                       a.ncol, 1, 0, a.z, a.qm, a.phase};
-    switch (a.ns) {
-        case 4: return launch_stream<4>(st, a.nt, a.work, a.nwork, a.nb, 0, s);
-        case 2: return launch_stream<2>(st, a.nt, a.work, a.nwork, a.nb, 0, s);
-        case 1: return launch_stream<1>(st, a.nt, a.work, a.nwork, a.nb, 0, s);
-        default: return false;
+    const int kchunk = a.ns == 1 ? std::min(a.nt, kEvalKChunk) : a.nt;
+    for (int l0 = 0; l0 < a.nt; l0 += kchunk) {
+        LegEval<double> st{a.ptab, a.flm, a.spec, a.spec_stride, a.flm_stride, a.order, a.nt, a.nlat,
+                           a.ncol, 1, 0, a.z, a.qm, a.phase, l0, kchunk, l0 > 0 ? 1 : 0};
+        bool ok = false;
+        switch (a.ns) {
+            case 4: ok = launch_stream<4>(st, kchunk, a.work, a.nwork, a.nb, 0, s); break;
+            case 2: ok = launch_stream<2>(st, kchunk, a.work, a.nwork, a.nb, 0, s); break;
+            case 1: ok = launch_stream<1>(st, kchunk, a.work, a.nwork, a.nb, 0, s); break;
+            default: break;
+        }
+        if (!ok) return false;
     }
+    return true;
 }
 
 template bool zt_legendre_forward_stream<double>(const ZtGemmArgs&, hipStream_t);

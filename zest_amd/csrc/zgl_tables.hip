@@ -148,48 +148,50 @@ __global__ void plm_kernel(int order, int jh, int qm, double phase, const double
     });
 }
 
-// R_nl(r_k) for one radial node: the recursion of
-// zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-288)
-// with the coefficients of src/radial_zernike_recursion.cpp:91-116. Every
-// recursion input is unnormalised there, so the normed table is the
-// unnormed one times sqrt(2n+3), applied at the end.
+// R_nl(r_k): the recursion of zt::RadialZernikeRecursion::zernike
+// (include/radial_zernike_recursion.hpp:151-288) with the coefficients of
+// src/radial_zernike_recursion.cpp:91-116, one thread per (radius, l): the
+// sequence R_ll, R_{l+2,l}, R_{l+4,l}, ... of one l depends on other l only
+// through its seeds R_ll = r^l (formed by the same repeated products as the
+// recursion's diagonal R_nn = r R_{n-1,n-1}) and R_{l+2,l+2}. Every recursion
+// input is unnormalised, so the normed table is the unnormed one times
+// sqrt(2n+3), applied as each value is stored.
 __global__ void radial_kernel(int order, int nr, int normed, const double* rk, double* rtab) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nr || order == 0) return;
+    const int l = blockIdx.y;
+    if (k >= nr || l >= order) return;
     const double r = rk[k];
-    auto R = [&](int n, int l) -> double& { return rtab[rad_idx(n, l) * nr + k]; };
-    R(0, 0) = 1.0;
-    if (order > 1) R(1, 1) = r;
-    if (order > 2) {
-        R(2, 2) = r * r;
-        R(2, 0) = 2.5 * R(2, 2) - 1.5;
+    const double r2 = r * r;
+    auto store = [&](int n, double v) {
+        rtab[rad_idx(n, l) * nr + k] = normed ? v * sqrt(2.0 * n + 3.0) : v;
+    };
+    // powers r^l and r^(l+2) as the diagonal recursion forms them
+    double dl = 1.0;
+    for (int i = 1; i <= l; ++i) dl = i == 1 ? r : (i == 2 ? r2 : dl * r);
+    store(l, dl);
+    if (l + 2 >= order) return;
+    const int n2 = l + 2;
+    const double dl2 = n2 == 2 ? r2 : dl * r * r;
+    double p2;  // R_{l+2,l}
+    if (n2 == 2) p2 = 2.5 * r2 - 1.5;
+    else if (n2 == 3) p2 = (3.5 * r2 - 2.5) * r;
+    else p2 = (n2 + 0.5) * dl2 - (n2 - 0.5) * dl;
+    store(n2, p2);
+    double p4 = dl;  // R_{n-4,l}
+    for (int n = l + 4; n < order; n += 2) {
+        const double dn = (double)n, dL = (double)l;
+        const double c0 = (dn - dL) * (dn + dL + 1.0) * (2.0 * dn - 3.0);
+        const double c1 = (2.0 * dn - 1.0) * (2.0 * dn + 1) * (2.0 * dn - 3.0);
+        const double c2 = -0.5 * (2.0 * dn - 1.0) *
+                          ((2 * dL + 1.0) * (2.0 * dL + 1.0) + (2.0 * dn + 1) * (2.0 * dn - 3.0));
+        const double c3 = -(dn - dL - 2.0) * (dn + dL - 1.0) * (2.0 * dn + 1);
+        const double inv = 1.0 / c0;
+        const double k1 = c1 * inv, k2 = c2 * inv, k3 = c3 * inv;
+        const double v = (k2 + k1 * r2) * p2 + k3 * p4;
+        store(n, v);
+        p4 = p2;
+        p2 = v;
     }
-    if (order > 3) {
-        R(3, 1) = (3.5 * R(2, 2) - 2.5) * r;
-        R(3, 3) = R(2, 2) * r;
-    }
-    const double r2 = order > 2 ? R(2, 2) : r * r;
-    for (int n = 4; n < order; ++n) {
-        const double dn = (double)n;
-        for (int l = n & 1; l + 4 <= n; l += 2) {
-            const double dl = (double)l;
-            const double c0 = (dn - dl) * (dn + dl + 1.0) * (2.0 * dn - 3.0);
-            const double c1 = (2.0 * dn - 1.0) * (2.0 * dn + 1) * (2.0 * dn - 3.0);
-            const double c2 = -0.5 * (2.0 * dn - 1.0) *
-                              ((2 * dl + 1.0) * (2.0 * dl + 1.0) + (2.0 * dn + 1) * (2.0 * dn - 3.0));
-            const double c3 = -(dn - dl - 2.0) * (dn + dl - 1.0) * (2.0 * dn + 1);
-            const double inv = 1.0 / c0;
-            const double k1 = c1 * inv, k2 = c2 * inv, k3 = c3 * inv;
-            R(n, l) = (k2 + k1 * r2) * R(n - 2, l) + k3 * R(n - 4, l);
-        }
-        R(n, n) = r * R(n - 1, n - 1);
-        R(n, n - 2) = (dn + 0.5) * R(n, n) - (dn - 0.5) * R(n - 2, n - 2);
-    }
-    if (normed)
-        for (int n = 0; n < order; ++n) {
-            const double nm = sqrt(2.0 * n + 3.0);
-            for (int l = n & 1; l <= n; l += 2) R(n, l) *= nm;
-        }
 }
 
 __global__ void to_fp32_kernel(const double* in, float* out, long n) {
@@ -284,7 +286,7 @@ __global__ void eval_plm_kernel(int order, int nlat, int qm, double phase, const
 
 void radial_table(int order, int nr, int normed, const double* r, double* rtab, hipStream_t s) {
     if (order == 0 || nr == 0) return;
-    radial_kernel<<<nr / 64 + 1, 64, 0, s>>>(order, nr, normed, r, rtab);
+    radial_kernel<<<dim3(nr / 64 + 1, order), 64, 0, s>>>(order, nr, normed, r, rtab);
     check_hip(hipGetLastError(), "radial table");
 }
 
@@ -384,8 +386,8 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
         radial_nodes_kernel<<<nr / 64 + 1, 64, 0, s>>>(nr, x, v, t.wrad);
         const long rsize = (long)((order + 1) * (order + 1) / 4) * nr;
         t.rtab = dev_alloc<double>(rsize);
-        radial_kernel<<<nr / 64 + 1, 64, 0, s>>>(order, nr, zernike_norm == ZGL_ZNORM_NORMED ? 1 : 0,
-                                                 x, t.rtab);
+        radial_kernel<<<dim3(nr / 64 + 1, order), 64, 0, s>>>(
+            order, nr, zernike_norm == ZGL_ZNORM_NORMED ? 1 : 0, x, t.rtab);
         check_hip(hipGetLastError(), "table kernels");
         check_hip(hipStreamSynchronize(s), "table generation");
         check_hip(hipFree(x), "free");
