@@ -236,6 +236,33 @@ zgl_status zgl_cross_power_spectrum(zgl_plan *plan, const double *a, const doubl
                                     ptrdiff_t coeff_stride, ptrdiff_t out_stride,
                                     zgl_mem mem, void *stream);
 
+/* Rotations, batched and in place. zgl_rotate replaces Rotor::rotate
+ * (include/rotor.hpp:124-288) with the WignerdPiHalfCollection it is given
+ * (here built on the device and kept by the plan): intrinsic ZYZ Euler
+ * angles, euler_angles[3 b .. 3 b + 2] = (alpha, beta, gamma) of instance b
+ * (in the same memory as the coefficients); rotation_type 0 rotates the
+ * object, 1 the coordinate system (zest's RotationType). zgl_polar_rotate
+ * replaces Rotor::polar_rotate (include/rotor.hpp:321-380), angles[b] per
+ * instance. Both take the plan's kind; the plan's conventions do not enter
+ * (zest converts with the expansion's own conventions, which cancel). Orders
+ * below 2 leave the coefficients unchanged; batch <= 65535. */
+zgl_status zgl_rotate(zgl_plan *plan, double *coeffs, size_t order,
+                      const double *euler_angles, int rotation_type, size_t batch,
+                      ptrdiff_t coeff_stride, zgl_mem mem, void *stream);
+zgl_status zgl_polar_rotate(zgl_plan *plan, double *coeffs, size_t order,
+                            const double *angles, int rotation_type, size_t batch,
+                            ptrdiff_t coeff_stride, zgl_mem mem, void *stream);
+/* The Wigner d-matrices at pi/2 of degrees l < order, as
+ * WignerdPiHalfCollection(order).matrices() (src/wignerd_collection.cpp:29-104):
+ * order (order + 1)(2 order + 1)/6 doubles, d^l_{m1,m2} at
+ * l(l+1)(2l+1)/6 + (l+1) m1 + m2. */
+zgl_status zgl_wignerd_pi2(zgl_plan *plan, size_t order, double *out, zgl_mem mem,
+                           void *stream);
+/* euler_angles_from_rotation_matrix (include/rotor.hpp:49-60), on the host:
+ * matrix row-major 3x3, angles = (alpha, beta, gamma). Computes beta from
+ * sqrt((1 + r22^2)(1 - r22^2)) as zest does. */
+void zgl_euler_angles_from_rotation_matrix(const double *matrix, double *angles);
+
 /* Autotuning. No counterpart in the reference (zest has nothing to tune).
  * While enabled (also: environment ZGL_AUTOTUNE=1 at plan creation), the
  * first call of each stage at a new order and arithmetic times the candidate

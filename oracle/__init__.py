@@ -52,6 +52,10 @@ def _load_oracle():
     lib.zo_st_evaluate.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr, _c_size, _ptr]
     lib.zo_st_power_spectrum.argtypes = [_c_int, _c_size, _ptr, _ptr, _ptr]
     lib.zo_zt_power_spectrum.argtypes = [_c_int, _c_size, _ptr, _ptr, _ptr]
+    lib.zo_wignerd_table.argtypes = [_c_size, _ptr]
+    _d = ctypes.c_double
+    lib.zo_rotate.argtypes = [_c_int, _c_size, _ptr, _d, _d, _d, _c_int]
+    lib.zo_polar_rotate.argtypes = [_c_int, _c_size, _ptr, _d, _c_int]
     return lib
 
 
@@ -78,6 +82,11 @@ def _load_ref():
                                      _c_size, _ptr, _c_size, _ptr]
     lib.zref_st_evaluate.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr, _c_size, _ptr, _c_size, _ptr]
     lib.zref_zt_power_spectrum.argtypes = [_c_int, _c_int, _c_size, _ptr, _ptr]
+    _d = ctypes.c_double
+    lib.zref_wignerd.argtypes = [_c_size, _ptr]
+    lib.zref_euler_angles.argtypes = [_ptr, _ptr]
+    lib.zref_rotate.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_size, _ptr, _d, _d, _d, _c_int]
+    lib.zref_polar_rotate.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_size, _ptr, _d, _c_int]
     return lib
 
 
@@ -244,6 +253,34 @@ def zt_power_spectrum(a: np.ndarray, order: int, sh_norm: int = 0, b: np.ndarray
     return out
 
 
+# ---- rotations (include/rotor.hpp, include/wignerd_collection.hpp) ----
+ZERNIKE_KIND, SH_KIND = 0, 1
+
+
+def wignerd_size(order: int) -> int:
+    return order * (order + 1) * (2 * order + 1) // 6
+
+
+def wignerd_table(order: int) -> np.ndarray:
+    """Wigner d^l_{m1,m2}(pi/2), zest's layout [l(l+1)(2l+1)/6 + (l+1) m1 + m2]."""
+    out = np.zeros(wignerd_size(order))
+    ORACLE.zo_wignerd_table(order, _p(out))
+    return out
+
+
+def rotate(coeffs: np.ndarray, order: int, kind: int, angles, rtype: int = 0) -> np.ndarray:
+    """A rotated copy (Euler angles alpha, beta, gamma; rtype 0 object, 1 coordinate)."""
+    c = np.array(coeffs, dtype=np.float64, order="C", copy=True)
+    ORACLE.zo_rotate(kind, order, _p(c), float(angles[0]), float(angles[1]), float(angles[2]), rtype)
+    return c
+
+
+def polar_rotate(coeffs: np.ndarray, order: int, kind: int, angle: float, rtype: int = 0) -> np.ndarray:
+    c = np.array(coeffs, dtype=np.float64, order="C", copy=True)
+    ORACLE.zo_polar_rotate(kind, order, _p(c), float(angle), rtype)
+    return c
+
+
 # ---- the upstream zest library (compiled reference) ----
 class ZestZernike:
     """One zest ``zt::GLQTransformer`` instance (keeps its tables between calls)."""
@@ -374,4 +411,30 @@ def ref_zt_power_spectrum(coeffs: np.ndarray, order: int, zernike_norm: int = 1,
     coeffs = _f64(coeffs)
     out = np.full((order + 1) ** 2 // 4, fill)
     REF.zref_zt_power_spectrum(zernike_norm, sh_norm, order, _p(coeffs), _p(out))
+    return out
+
+
+def ref_wignerd_table(order: int) -> np.ndarray:
+    out = np.zeros(wignerd_size(order))
+    REF.zref_wignerd(order, _p(out))
+    return out
+
+
+def ref_rotate(coeffs, order, kind, angles, rtype=0, zernike_norm=1, sh_norm=0, sh_phase=-1):
+    c = np.array(coeffs, dtype=np.float64, order="C", copy=True)
+    REF.zref_rotate(kind, zernike_norm, sh_norm, sh_phase, order, _p(c), float(angles[0]),
+                    float(angles[1]), float(angles[2]), rtype)
+    return c
+
+
+def ref_polar_rotate(coeffs, order, kind, angle, rtype=0, zernike_norm=1, sh_norm=0, sh_phase=-1):
+    c = np.array(coeffs, dtype=np.float64, order="C", copy=True)
+    REF.zref_polar_rotate(kind, zernike_norm, sh_norm, sh_phase, order, _p(c), float(angle), rtype)
+    return c
+
+
+def ref_euler_angles(matrix) -> np.ndarray:
+    m = np.ascontiguousarray(matrix, dtype=np.float64).reshape(3, 3)
+    out = np.zeros(3)
+    REF.zref_euler_angles(_p(m), _p(out))
     return out

@@ -17,6 +17,8 @@
 //   zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-154)
 //   zt::GridEvaluator::evaluate (include/grid_evaluator.hpp:227-265)
 //   st::GridEvaluator::evaluate (include/grid_evaluator.hpp:104-133)
+//   Rotor::rotate, Rotor::polar_rotate (include/rotor.hpp:124-380) and
+//   WignerdPiHalfCollection (src/wignerd_collection.cpp:29-104)
 //   zt::power_spectrum (include/power_spectra.hpp:118-141; zest's st power
 //   spectra name a member the SH expansion types lack and do not compile)
 #include <cstddef>
@@ -32,6 +34,8 @@
 #include "radial_zernike_recursion.hpp"
 #include "grid_evaluator.hpp"
 #include "power_spectra.hpp"
+#include "rotor.hpp"
+#include "wignerd_collection.hpp"
 
 namespace {
 
@@ -148,6 +152,39 @@ void zt_power(int sh_norm, std::size_t order, const double* coeffs, double* out)
     };
     if (sh_norm == 1) run.template operator()<SHNorm::qm>();
     else run.template operator()<SHNorm::geo>();
+}
+
+// Calls f with a mutable zest expansion span of the requested conventions.
+template <class F>
+void with_zt_span(int zn, int sn, int sp, double* c, std::size_t order, F&& f) {
+    auto* a = reinterpret_cast<std::array<double, 2>*>(c);
+    auto go = [&]<ZernikeNorm z, SHNorm n, SHPhase p>() {
+        f(zest::zt::RealZernikeSpan<std::array<double, 2>, z, n, p>(a, order));
+    };
+    const int key = (zn == 1 ? 4 : 0) + (sn == 1 ? 2 : 0) + (sp > 0 ? 1 : 0);
+    switch (key) {
+        case 0: go.template operator()<ZernikeNorm::normed, SHNorm::geo, SHPhase::none>(); break;
+        case 1: go.template operator()<ZernikeNorm::normed, SHNorm::geo, SHPhase::cs>(); break;
+        case 2: go.template operator()<ZernikeNorm::normed, SHNorm::qm, SHPhase::none>(); break;
+        case 3: go.template operator()<ZernikeNorm::normed, SHNorm::qm, SHPhase::cs>(); break;
+        case 4: go.template operator()<ZernikeNorm::unnormed, SHNorm::geo, SHPhase::none>(); break;
+        case 5: go.template operator()<ZernikeNorm::unnormed, SHNorm::geo, SHPhase::cs>(); break;
+        case 6: go.template operator()<ZernikeNorm::unnormed, SHNorm::qm, SHPhase::none>(); break;
+        default: go.template operator()<ZernikeNorm::unnormed, SHNorm::qm, SHPhase::cs>(); break;
+    }
+}
+
+template <class F>
+void with_st_span(int sn, int sp, double* c, std::size_t order, F&& f) {
+    auto* a = reinterpret_cast<std::array<double, 2>*>(c);
+    auto go = [&]<SHNorm n, SHPhase p>() { f(zest::st::RealSHSpan<std::array<double, 2>, n, p>(a, order)); };
+    const int key = (sn == 1 ? 2 : 0) + (sp > 0 ? 1 : 0);
+    switch (key) {
+        case 0: go.template operator()<SHNorm::geo, SHPhase::none>(); break;
+        case 1: go.template operator()<SHNorm::geo, SHPhase::cs>(); break;
+        case 2: go.template operator()<SHNorm::qm, SHPhase::none>(); break;
+        default: go.template operator()<SHNorm::qm, SHPhase::cs>(); break;
+    }
 }
 
 }  // namespace
@@ -307,6 +344,45 @@ void zref_zt_power_spectrum(int zernike_norm, int sh_norm, std::size_t order,
                             const double* coeffs, double* out) {
     if (zernike_norm == 1) zt_power<ZernikeNorm::unnormed>(sh_norm, order, coeffs, out);
     else zt_power<ZernikeNorm::normed>(sh_norm, order, coeffs, out);
+}
+
+void zref_euler_angles(const double* m, double* out) {
+    const std::array<std::array<double, 3>, 3> r{{{m[0], m[1], m[2]}, {m[3], m[4], m[5]}, {m[6], m[7], m[8]}}};
+    const auto a = zest::euler_angles_from_rotation_matrix(r);
+    out[0] = a[0];
+    out[1] = a[1];
+    out[2] = a[2];
+}
+
+void zref_wignerd(std::size_t order, double* out) {
+    zest::WignerdPiHalfCollection w(order);
+    const auto m = w.matrices();
+    std::memcpy(out, m.data(), m.size() * sizeof(double));
+}
+
+// kind 0: Zernike, 1: SH; type 0: object, 1: coordinate. In place.
+void zref_rotate(int kind, int zernike_norm, int sh_norm, int sh_phase, std::size_t order,
+                 double* coeffs, double alpha, double beta, double gamma, int type) {
+    zest::WignerdPiHalfCollection w(order);
+    zest::Rotor rotor(order);
+    const std::array<double, 3> angles{alpha, beta, gamma};
+    const auto t = type == 0 ? zest::RotationType::object : zest::RotationType::coordinate;
+    if (kind == 0)
+        with_zt_span(zernike_norm, sh_norm, sh_phase, coeffs, order,
+                     [&](auto e) { rotor.rotate(e, w, angles, t); });
+    else
+        with_st_span(sh_norm, sh_phase, coeffs, order, [&](auto e) { rotor.rotate(e, w, angles, t); });
+}
+
+void zref_polar_rotate(int kind, int zernike_norm, int sh_norm, int sh_phase, std::size_t order,
+                       double* coeffs, double angle, int type) {
+    zest::Rotor rotor(order);
+    const auto t = type == 0 ? zest::RotationType::object : zest::RotationType::coordinate;
+    if (kind == 0)
+        with_zt_span(zernike_norm, sh_norm, sh_phase, coeffs, order,
+                     [&](auto e) { rotor.polar_rotate(e, angle, t); });
+    else
+        with_st_span(sh_norm, sh_phase, coeffs, order, [&](auto e) { rotor.polar_rotate(e, angle, t); });
 }
 
 }  // extern "C"

@@ -796,3 +796,167 @@ void zo_zt_power_spectrum(int sh_norm, size_t order, const double *a, const doub
             out[rad_idx(n, l)] = norm * s;
         }
 }
+
+/* ---- rotations ------------------------------------------------------------------
+ * WignerdPiHalfCollection (src/wignerd_collection.cpp:29-104): the Wigner small-d
+ * matrices at beta = pi/2, d^l_{m1,m2} for 0 <= m1, m2 <= l, layout
+ * [l(l+1)(2l+1)/6 + (l+1) m1 + m2]. Column m2 = 0 by a two-term recursion down
+ * from d^l_{l,0}; row m1 = l by products; the rest of the lower triangle by the
+ * three-term recursion in m2 (downwards); the upper triangle by the symmetry
+ * d_{m1,m2} = (-1)^{m1+m2} d_{m2,m1}. */
+static size_t wig_idx(size_t l, size_t m1, size_t m2) {
+    return (l * (l + 1) * (2 * l + 1)) / 6 + (l + 1) * m1 + m2;
+}
+
+void zo_wignerd_table(size_t order, double *d) {
+    if (order == 0) return;
+    double *sq = (double *)malloc((2 * order + 2) * sizeof(double));
+    double *isq = (double *)malloc((2 * order + 2) * sizeof(double));
+    for (size_t i = 0; i < 2 * order + 2; ++i) {
+        sq[i] = sqrt((double)i);
+        isq[i] = 1.0 / sq[i];
+    }
+    d[wig_idx(0, 0, 0)] = 1.0;
+    if (order > 1) {
+        const double h = 1.0 / 1.41421356237309504880168872420969808;
+        d[wig_idx(1, 0, 0)] = 0.0;
+        d[wig_idx(1, 1, 0)] = -h;
+        d[wig_idx(1, 0, 1)] = h;
+        d[wig_idx(1, 1, 1)] = 0.5;
+        double dl0 = -h;
+        for (size_t l = 2; l < order; ++l) {
+            dl0 *= -sq[2 * l - 1] / sq[2 * l];
+            d[wig_idx(l, l, 0)] = dl0;
+            d[wig_idx(l, l - 1, 0)] = 0.0;
+            for (size_t i = 2; i <= l; i += 2) {
+                const size_t m2 = l - i;
+                d[wig_idx(l, m2, 0)] = -d[wig_idx(l, m2 + 2, 0)] * sq[l - m2 - 1] * sq[l + m2 + 2] *
+                                       isq[l - m2] * isq[l + m2 + 1];
+            }
+            double dlm = dl0;
+            for (size_t m1 = 1; m1 < l; ++m1) {
+                dlm *= -sq[l - m1 + 1] * isq[l + m1];
+                d[wig_idx(l, l, m1)] = dlm;
+                d[wig_idx(l, l - 1, m1)] = d[wig_idx(l, l, m1)] * 2.0 * (double)m1 * isq[2 * l];
+                for (size_t i = 2; i <= l - m1; ++i) {
+                    const size_t m2 = l - i;
+                    d[wig_idx(l, m2, m1)] =
+                        (2.0 * (double)m1 * d[wig_idx(l, m2 + 1, m1)] -
+                         sq[l - m2 - 1] * sq[l + m2 + 2] * d[wig_idx(l, m2 + 2, m1)]) *
+                        isq[l - m2] * isq[l + m2 + 1];
+                }
+            }
+            d[wig_idx(l, l, l)] = -dlm * isq[2 * l];
+            for (size_t m1 = 0; m1 <= l; ++m1)
+                for (size_t m2 = m1 + 1; m2 <= l; ++m2)
+                    d[wig_idx(l, m1, m2)] = (((m1 ^ m2) & 1) ? -1.0 : 1.0) * d[wig_idx(l, m2, m1)];
+        }
+    }
+    free(sq); free(isq);
+}
+
+/* Rotor::rotate (include/rotor.hpp:124-257) for one coefficient vector of
+ * degree l, after to_complex_expansion (include/real_sh_expansion.hpp:342-376,
+ * include/zernike_expansion.hpp:480-560: m > 0 pairs times (1/sqrt2, -1/sqrt2)):
+ * rotate_l (include/wignerd_collection.hpp:134-296), the ZXZXZ product
+ *   v <- e^{-i m gamma'} v;  v <- X(+90) v;  v <- e^{-i m beta'} v;
+ *   v <- X(-90) v;  v <- e^{-i m alpha'} v,
+ * where X(+90) sums over m' >= 0 with the conjugate symmetry of a real
+ * function folded in:
+ *   Re y_m = sum_{m'} c_{m'} d_{m,m'} Re v_{m'}   over (m + m' + l) even,
+ *   Im y_m = sum_{m'} 2 d_{m,m'} Im v_{m'}        over (m + m' + l) odd,
+ * c_0 = 1, c_{m'>0} = 2, and X(-90) the same with d_{m',m}. The converted
+ * angles (detail::convert, include/rotor.hpp:69-86): object (alpha - pi/2,
+ * beta, gamma + pi/2), coordinate the negated reverse of that. */
+static void rot_x(size_t l, const double *dl, int transpose, const double *vr, const double *vi,
+                  double *yr, double *yi) {
+    for (size_t m = 0; m <= l; ++m) {
+        double sr = 0.0, si = 0.0;
+        for (size_t mp = 0; mp <= l; ++mp) {
+            const double dv = transpose ? dl[(l + 1) * mp + m] : dl[(l + 1) * m + mp];
+            if (((m + mp + l) & 1) == 0) sr += (mp ? 2.0 : 1.0) * dv * vr[mp];
+            else si += 2.0 * dv * vi[mp];
+        }
+        yr[m] = sr;
+        yi[m] = si;
+    }
+}
+
+static void rot_phase(size_t l, double angle, double *vr, double *vi) {
+    for (size_t m = 0; m <= l; ++m) {
+        const double c = cos(-(double)m * angle), s = sin(-(double)m * angle);
+        const double a = vr[m], b = vi[m];
+        vr[m] = a * c - b * s;
+        vi[m] = a * s + b * c;
+    }
+}
+
+static void rot_angles(double alpha, double beta, double gamma, int type, double *out) {
+    const double a = alpha - 0.5 * ZO_PI, b = beta, g = gamma + 0.5 * ZO_PI;
+    if (type == 0) { out[0] = a; out[1] = b; out[2] = g; }
+    else { out[0] = -g; out[1] = -b; out[2] = -a; }
+}
+
+static void rotate_vector(size_t l, const double *dl, const double *ang, double *c, double *w) {
+    /* c: l + 1 {cos, sin} pairs; w: scratch of 4 (l + 1) */
+    const double h = 1.0 / 1.41421356237309504880168872420969808;
+    double *vr = w, *vi = w + (l + 1), *yr = w + 2 * (l + 1), *yi = w + 3 * (l + 1);
+    for (size_t m = 0; m <= l; ++m) {
+        vr[m] = m ? c[2 * m] * h : c[2 * m];
+        vi[m] = m ? c[2 * m + 1] * -h : c[2 * m + 1];
+    }
+    rot_phase(l, ang[2], vr, vi);
+    rot_x(l, dl, 0, vr, vi, yr, yi);
+    rot_phase(l, ang[1], yr, yi);
+    rot_x(l, dl, 1, yr, yi, vr, vi);
+    rot_phase(l, ang[0], vr, vi);
+    const double s2 = 1.41421356237309504880168872420969808;
+    for (size_t m = 0; m <= l; ++m) {
+        c[2 * m] = m ? vr[m] * s2 : vr[m];
+        c[2 * m + 1] = m ? vi[m] * -s2 : vi[m];
+    }
+}
+
+/* kind 0: Zernike (every (n, l >= 1)), 1: SH (every l >= 1); in place. */
+void zo_rotate(int kind, size_t order, double *coeffs, double alpha, double beta, double gamma,
+               int type) {
+    if (order < 2) return;
+    double ang[3];
+    rot_angles(alpha, beta, gamma, type, ang);
+    double *d = (double *)calloc(order * (order + 1) * (2 * order + 1) / 6 + 1, sizeof(double));
+    double *w = (double *)malloc(4 * (order + 1) * sizeof(double));
+    zo_wignerd_table(order, d);
+    for (size_t l = 1; l < order; ++l) {
+        const double *dl = d + wig_idx(l, 0, 0);
+        if (kind == 1) {
+            rotate_vector(l, dl, ang, coeffs + tri_idx(l, 0) * 2, w);
+        } else {
+            for (size_t n = l; n < order; n += 2) rotate_vector(l, dl, ang, coeffs + tet_idx(n, l, 0) * 2, w);
+        }
+    }
+    free(d); free(w);
+}
+
+/* Rotor::polar_rotate (include/rotor.hpp:321-380): v_m <- e^{-i m angle'} v_m
+ * in the complex form, angle' = angle (object) or -angle (coordinate). */
+static void polar_vector(size_t l, double a, double *c) {
+    const double h = 1.0 / 1.41421356237309504880168872420969808;
+    const double s2 = 1.41421356237309504880168872420969808;
+    for (size_t m = 1; m <= l; ++m) {
+        const double vr = c[2 * m] * h, vi = c[2 * m + 1] * -h;
+        const double co = cos(-(double)m * a), si = sin(-(double)m * a);
+        c[2 * m] = (vr * co - vi * si) * s2;
+        c[2 * m + 1] = (vr * si + vi * co) * -s2;
+    }
+}
+
+void zo_polar_rotate(int kind, size_t order, double *coeffs, double angle, int type) {
+    const double a = type == 0 ? angle : -angle;
+    for (size_t l = 1; l < order; ++l) {
+        if (kind == 1) {
+            polar_vector(l, a, coeffs + tri_idx(l, 0) * 2);
+        } else {
+            for (size_t n = l; n < order; n += 2) polar_vector(l, a, coeffs + tet_idx(n, l, 0) * 2);
+        }
+    }
+}

@@ -54,6 +54,10 @@ void zo_st_power_spectrum(int sh_norm, size_t order, const double *a, const doub
                           double *out);
 void zo_zt_power_spectrum(int sh_norm, size_t order, const double *a, const double *b,
                           double *out);
+void zo_wignerd_table(size_t order, double *d);
+void zo_rotate(int kind, size_t order, double *coeffs, double alpha, double beta, double gamma,
+               int type);
+void zo_polar_rotate(int kind, size_t order, double *coeffs, double angle, int type);
 
 #ifdef __cplusplus
 }

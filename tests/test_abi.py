@@ -89,3 +89,20 @@ def test_numpy_arrays_must_be_contiguous_float64():
     # grids may be float32 (the *_grid32 entry points); coefficients may not
     assert zest_amd._ptr_and_mem(np.zeros(4, dtype=np.float32), allow_f32=True)[2]
     assert not zest_amd._ptr_and_mem(np.zeros(4), allow_f32=True)[2]
+
+
+def test_euler_angles_from_rotation_matrix_matches_zest():
+    """Host helper of the C ABI against zest's constexpr function
+    (include/rotor.hpp:49-60), which forms sin(beta) as
+    sqrt((1 + r22^2)(1 - r22^2)); exact for beta in {0, pi/2, pi}."""
+    import oracle
+    import zest_amd
+    if oracle.REF is None:
+        pytest.skip("zest not compiled (oracle/_ref)")
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        q, _r = np.linalg.qr(rng.normal(size=(3, 3)))
+        q *= np.sign(np.linalg.det(q))
+        assert np.abs(zest_amd.euler_angles_from_rotation_matrix(q) - oracle.ref_euler_angles(q)).max() < 1e-15
+    a = zest_amd.euler_angles_from_rotation_matrix(np.eye(3))
+    assert a[1] == 0.0

@@ -546,3 +546,122 @@ def test_zernike_power_spectrum_parseval(sn):
     total = oracle.zt_power_spectrum(c, N, sn).sum()
     expect = integral if sn == 1 else integral / (4 * math.pi) ** 2
     assert abs(total - expect) <= 1e-13 * expect
+
+
+# ---------------------------------------------------------------- rotations
+def _wigner_textbook(j, mp, m, beta):
+    """Wigner's explicit sum for d^j_{m',m}(beta) at beta = pi/2, where every
+    cos^a sin^b (a + b = 2j) is 2^-j: the alternating sum is exact in rationals."""
+    from fractions import Fraction
+    assert beta == math.pi / 2
+    F = math.factorial
+    s = Fraction(0)
+    for k in range(0, 2 * j + 1):
+        a = (j + m - k, k, mp - m + k, j - mp - k)
+        if min(a) < 0:
+            continue
+        s += Fraction((-1) ** (mp - m + k), F(a[0]) * F(a[1]) * F(a[2]) * F(a[3]))
+    return float(s) * math.sqrt(F(j + mp) * F(j - mp) * F(j + m) * F(j - m)) / 2.0 ** j
+
+
+def test_wignerd_table_textbook_and_orthogonal():
+    order = 12
+    d = oracle.wignerd_table(order)
+    for l in range(order):
+        base = l * (l + 1) * (2 * l + 1) // 6
+        for m1 in range(l + 1):
+            for m2 in range(l + 1):
+                assert abs(d[base + (l + 1) * m1 + m2] - _wigner_textbook(l, m1, m2, math.pi / 2)) < 1e-14
+    # high order: the full (2l+1)^2 matrix from the stored quarter is orthogonal
+    order = 200
+    d = oracle.wignerd_table(order)
+    for l in (1, 2, 57, 199):
+        q = d[l * (l + 1) * (2 * l + 1) // 6:][:(l + 1) ** 2].reshape(l + 1, l + 1)
+        D = np.zeros((2 * l + 1, 2 * l + 1))
+        # d_{a,b} = (-1)^(b-a) d_{b,a} = d_{-b,-a};  d_{a,-b}(pi/2) = (-1)^(l+a) d_{a,b}(pi/2)
+        for a in range(-l, l + 1):
+            for b in range(-l, l + 1):
+                if a >= 0:
+                    D[a + l, b + l] = q[a, b] if b >= 0 else (-1) ** (l + a) * q[a, -b]
+                elif b <= 0:
+                    D[a + l, b + l] = q[-b, -a]
+                else:
+                    D[a + l, b + l] = (-1) ** (b - a) * (-1) ** (l + b) * q[b, -a]
+        if l <= 2:
+            for a in range(-l, l + 1):
+                for b in range(-l, l + 1):
+                    assert abs(D[a + l, b + l] - _wigner_textbook(l, a, b, math.pi / 2)) < 1e-14
+        assert np.abs(D @ D.T - np.eye(2 * l + 1)).max() < 1e-12, l
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [1, 2, 33, 128])
+def test_wignerd_table_matches_compiled_zest(order):
+    a, b = oracle.wignerd_table(order), oracle.ref_wignerd_table(order)
+    assert np.abs(a - b).max() < 1e-14
+
+
+def _rot(a, b, g):
+    Rz = lambda t: np.array([[math.cos(t), -math.sin(t), 0], [math.sin(t), math.cos(t), 0], [0, 0, 1]])
+    Ry = lambda t: np.array([[math.cos(t), 0, math.sin(t)], [0, 1, 0], [-math.sin(t), 0, math.cos(t)]])
+    return Rz(a) @ Ry(b) @ Rz(g)
+
+
+@pytest.mark.parametrize("rtype", [0, 1])
+def test_rotation_is_rotation_of_the_function(rtype):
+    """Phase convention none: the rotated expansion evaluated at x equals the
+    original at R^T x (object) or R x (coordinate), R = Rz(alpha) Ry(beta)
+    Rz(gamma), intrinsic ZYZ (include/rotor.hpp:104-122)."""
+    N = 7
+    ang = (0.3, 1.1, -2.0)
+    R = _rot(*ang)
+    M = R.T if rtype == 0 else R
+    rng = np.random.default_rng(4)
+    pts = [(rng.uniform(0, 2 * np.pi), rng.uniform(0, np.pi), rng.uniform(0.1, 1)) for _ in range(12)]
+    c = synth.sh_coeffs(N, 1)
+    z = synth.zernike_coeffs(N, 2)
+    cr = oracle.rotate(c, N, oracle.SH_KIND, ang, rtype)
+    zr = oracle.rotate(z, N, oracle.ZERNIKE_KIND, ang, rtype)
+    for lon, col, r in pts:
+        x = np.array([math.sin(col) * math.cos(lon), math.sin(col) * math.sin(lon), math.cos(col)])
+        y = M @ x
+        col2, lon2 = math.acos(max(-1.0, min(1.0, y[2]))), math.atan2(y[1], y[0])
+        a = oracle.st_evaluate(cr, N, [lon], [col], 0, -1)[0, 0]
+        b = oracle.st_evaluate(c, N, [lon2], [col2], 0, -1)[0, 0]
+        assert abs(a - b) < 1e-13
+        a = oracle.zt_evaluate(zr, N, [lon], [col], [r], 1, 0, -1)[0, 0, 0]
+        b = oracle.zt_evaluate(z, N, [lon2], [col2], [r], 1, 0, -1)[0, 0, 0]
+        assert abs(a - b) < 1e-13
+
+
+def test_rotation_invariants():
+    N = 20
+    c = synth.sh_coeffs(N, 8)
+    z = synth.zernike_coeffs(N, 9)
+    ang = (2.5, 0.4, 1.3)
+    # degree-wise power is invariant
+    for kind, x, pw in ((oracle.SH_KIND, c, oracle.st_power_spectrum), (oracle.ZERNIKE_KIND, z, oracle.zt_power_spectrum)):
+        r = oracle.rotate(x, N, kind, ang, 0)
+        assert np.allclose(pw(r, N, 1), pw(x, N, 1), rtol=1e-12, atol=1e-14)
+        # object rotation followed by the coordinate rotation of the same angles is the identity
+        back = oracle.rotate(r, N, kind, ang, 1)
+        assert np.abs(back - x).max() < 1e-12
+        # beta = 0: a z rotation by alpha + gamma, the same as polar_rotate
+        p = oracle.rotate(x, N, kind, (0.7, 0.0, 0.4), 0)
+        q = oracle.polar_rotate(x, N, kind, 1.1, 0)
+        assert np.abs(p - q).max() < 1e-12
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("order", [1, 2, 9, 64])
+@pytest.mark.parametrize("rtype", [0, 1])
+def test_rotation_matches_compiled_zest(kind, order, rtype):
+    c = synth.zernike_coeffs(order, order) if kind == 0 else synth.sh_coeffs(order, order)
+    for ang in [(0.3, 1.1, -2.0), (4.0, 2.9, 0.0)]:
+        a = oracle.rotate(c, order, kind, ang, rtype)
+        b = oracle.ref_rotate(c, order, kind, ang, rtype)
+        assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+    a = oracle.polar_rotate(c, order, kind, 0.9, rtype)
+    b = oracle.ref_polar_rotate(c, order, kind, 0.9, rtype)
+    assert np.abs(a - b).max() <= 1e-15 * np.abs(b).max()

@@ -84,6 +84,14 @@ def _load():
     lib.zgl_power_spectrum.restype = i32
     lib.zgl_cross_power_spectrum.argtypes = [vp, vp, vp, sz, vp, sz, pd, pd, i32, vp]
     lib.zgl_cross_power_spectrum.restype = i32
+    lib.zgl_rotate.argtypes = [vp, vp, sz, vp, i32, sz, pd, i32, vp]
+    lib.zgl_rotate.restype = i32
+    lib.zgl_polar_rotate.argtypes = [vp, vp, sz, vp, i32, sz, pd, i32, vp]
+    lib.zgl_polar_rotate.restype = i32
+    lib.zgl_wignerd_pi2.argtypes = [vp, sz, vp, i32, vp]
+    lib.zgl_wignerd_pi2.restype = i32
+    lib.zgl_euler_angles_from_rotation_matrix.argtypes = [vp, vp]
+    lib.zgl_euler_angles_from_rotation_matrix.restype = None
     lib.zgl_backward_grid32.restype = i32
     lib.zgl_last_error_string.argtypes = []
     lib.zgl_last_error_string.restype = ctypes.c_char_p
@@ -118,7 +126,8 @@ EXPORTS = (
     "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
     "zgl_forward_ragged", "zgl_backward_ragged", "zgl_ragged_lane_costs",
     "zgl_plan_autotune", "zgl_plan_tuning", "zgl_evaluate",
-    "zgl_power_spectrum", "zgl_cross_power_spectrum",
+    "zgl_power_spectrum", "zgl_cross_power_spectrum", "zgl_rotate", "zgl_polar_rotate",
+    "zgl_wignerd_pi2", "zgl_euler_angles_from_rotation_matrix",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
 )
 
@@ -321,6 +330,35 @@ class Plan:
                                                 out_stride, om, _stream_ptr(stream, om)))
         return out
 
+    def rotate(self, coeffs, order: int, angles, rotation_type: int = 0, batch: int = 1,
+               coeff_stride: int = 0, stream=None):
+        """In-place ZYZ rotation of `batch` expansions (zgl_rotate); `angles`
+        holds (alpha, beta, gamma) per instance, in the coefficients' memory."""
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        ap, am, _ = _ptr_and_mem(angles)
+        if am != cm:
+            raise ValueError("angles and coefficients must live in the same memory")
+        _check(LIB.zgl_rotate(self._h, cp, order, ap, rotation_type, batch, coeff_stride, cm,
+                              _stream_ptr(stream, cm)))
+        return coeffs
+
+    def polar_rotate(self, coeffs, order: int, angles, rotation_type: int = 0, batch: int = 1,
+                     coeff_stride: int = 0, stream=None):
+        """In-place rotation about z by angles[b] (zgl_polar_rotate)."""
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        ap, am, _ = _ptr_and_mem(angles)
+        if am != cm:
+            raise ValueError("angles and coefficients must live in the same memory")
+        _check(LIB.zgl_polar_rotate(self._h, cp, order, ap, rotation_type, batch, coeff_stride, cm,
+                                    _stream_ptr(stream, cm)))
+        return coeffs
+
+    def wignerd_pi2(self, order: int) -> np.ndarray:
+        """The plan's Wigner d(pi/2) table of degrees < order (zgl_wignerd_pi2), on the host."""
+        out = np.zeros(order * (order + 1) * (2 * order + 1) // 6)
+        _check(LIB.zgl_wignerd_pi2(self._h, order, out.ctypes.data, MEM_HOST, None))
+        return out
+
     @staticmethod
     def _jobs(jobs):
         if isinstance(jobs, ctypes.Array):
@@ -383,6 +421,14 @@ def make_jobs(jobs):
         arr[i].grid = grid.data_ptr() if grid is not None else None
         arr[i].coeffs = coeffs.data_ptr() if coeffs is not None else None
     return arr
+
+
+def euler_angles_from_rotation_matrix(matrix) -> np.ndarray:
+    """(alpha, beta, gamma) of a row-major 3x3 rotation matrix, as zest computes them."""
+    m = np.ascontiguousarray(matrix, dtype=np.float64).reshape(3, 3)
+    out = np.zeros(3)
+    LIB.zgl_euler_angles_from_rotation_matrix(m.ctypes.data, out.ctypes.data)
+    return out
 
 
 def _stream_ptr(stream, mem):

@@ -110,6 +110,15 @@ struct zgl_plan {
     zgl::EvalCache eval;
     double* dpts = nullptr;  // host-memory calls: the points, uploaded
     size_t dpts_cap = 0;
+    // rotations: the Wigner d(pi/2) table of degrees < wtab_order, work lists
+    // per (kind, order, batch), the intermediate and the uploaded angles
+    double* wtab = nullptr;
+    int wtab_order = 0;
+    std::map<std::tuple<int, int, int>, std::pair<int4*, int>> rot_work;
+    double* rtemp = nullptr;
+    size_t rtemp_cap = 0;
+    double* dang = nullptr;
+    size_t dang_cap = 0;
 };
 
 namespace zgl {
@@ -547,7 +556,9 @@ zgl_status zgl_plan_destroy(zgl_plan* plan) {
     plan->main.release();
     ragged_release(plan->rg);
     plan->eval.release();
-    for (void* q : {(void*)plan->dgrid, (void*)plan->dcoef, (void*)plan->dpts})
+    for (auto& kv : plan->rot_work) (void)hipFree(kv.second.first);
+    for (void* q : {(void*)plan->dgrid, (void*)plan->dcoef, (void*)plan->dpts, (void*)plan->wtab,
+                    (void*)plan->rtemp, (void*)plan->dang})
         if (q) (void)hipFree(q);
     delete plan;
     return ZGL_SUCCESS;
@@ -1126,4 +1137,131 @@ extern "C" zgl_status zgl_cross_power_spectrum(zgl_plan* plan, const double* a, 
                                                ptrdiff_t coeff_stride, ptrdiff_t out_stride, zgl_mem mem,
                                                void* stream) {
     return zgl::spectrum_entry(plan, a, b, order, out, batch, coeff_stride, out_stride, mem, stream);
+}
+
+// ------------------------------------------------------------ rotations
+namespace zgl {
+namespace {
+// The plan's d(pi/2) table covering degrees < order (grown, never shrunk).
+const double* wigner_for(zgl_plan* p, int order, hipStream_t s) {
+    if (p->wtab_order < order) {
+        if (p->wtab) {
+            check_hip(hipDeviceSynchronize(), "d table growth");
+            check_hip(hipFree(p->wtab), "free d table");
+            p->wtab = nullptr;
+            p->wtab_order = 0;
+        }
+        check_hip(hipMalloc((void**)&p->wtab, std::max<long>(wignerd_size(order), 1) * sizeof(double)),
+                  "d table allocation");
+        wignerd_table(order, p->wtab, s);
+        p->wtab_order = order;
+    }
+    return p->wtab;
+}
+
+std::pair<int4*, int> rot_work_for(zgl_plan* p, int order, int nb, hipStream_t s) {
+    const auto key = std::make_tuple((int)p->kind, order, nb);
+    auto it = p->rot_work.find(key);
+    if (it != p->rot_work.end()) return it->second;
+    const std::vector<int4> w = rotation_work(p->kind == ZGL_ZERNIKE, order, nb);
+    int4* d = nullptr;
+    check_hip(hipMalloc((void**)&d, std::max<size_t>(w.size(), 1) * sizeof(int4)), "work list");
+    if (!w.empty()) {
+        check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s), "work list");
+        check_hip(hipStreamSynchronize(s), "work list");
+    }
+    return p->rot_work.emplace(key, std::make_pair(d, (int)w.size())).first->second;
+}
+
+zgl_status rotate_entry(zgl_plan* plan, double* coeffs, size_t order, const double* angles, int rtype,
+                        size_t batch, ptrdiff_t coeff_stride, zgl_mem mem, void* stream, bool polar) {
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
+    if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    if (rtype != 0 && rtype != 1) return fail(ZGL_ERROR_INVALID_VALUE, "rotation type must be 0 (object) or 1 (coordinate)");
+    const long csz = 2 * (long)coeff_size(plan, order);
+    const long cs = coeff_stride ? (long)coeff_stride : csz;
+    if (batch > 1 && cs < csz) return fail(ZGL_ERROR_INVALID_VALUE, "instance strides overlap");
+    if (batch > 65535) return fail(ZGL_ERROR_INVALID_VALUE, "batch exceeds 65535 instances per call");
+    if (order < 2 || batch == 0) return ZGL_SUCCESS;
+    if (!coeffs || !angles) return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    const bool zt = plan->kind == ZGL_ZERNIKE;
+    const size_t nang = (polar ? 1 : 3) * batch;
+    return guarded([&]() -> zgl_status {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        double* dc = coeffs;
+        long dcs = cs;
+        const double* da = angles;
+        if (mem == ZGL_MEM_HOST) {
+            ensure(plan->dcoef, plan->dcoef_cap, (size_t)csz * batch);
+            ensure(plan->dang, plan->dang_cap, nang);
+            check_hip(hipMemcpy2DAsync(plan->dcoef, csz * sizeof(double), coeffs, cs * sizeof(double),
+                                       csz * sizeof(double), batch, hipMemcpyHostToDevice, s),
+                      "coefficient upload");
+            check_hip(hipMemcpyAsync(plan->dang, angles, nang * sizeof(double), hipMemcpyHostToDevice, s),
+                      "angle upload");
+            dc = plan->dcoef;
+            dcs = csz;
+            da = plan->dang;
+        }
+        const Inst<double> ci{dc, dcs, nullptr};
+        if (polar) {
+            polar_rotate(zt, (int)order, ci, da, rtype, (int)batch, s);
+        } else {
+            const double* w = wigner_for(plan, (int)order, s);
+            const auto work = rot_work_for(plan, (int)order, (int)batch, s);
+            ensure(plan->rtemp, plan->rtemp_cap, (size_t)csz * batch);
+            rotate(zt, (int)order, w, work.first, work.second, ci, plan->rtemp, csz, da, rtype, (int)batch, s);
+        }
+        if (mem == ZGL_MEM_HOST) {
+            check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, csz * sizeof(double),
+                                       csz * sizeof(double), batch, hipMemcpyDeviceToHost, s),
+                      "coefficient download");
+            check_hip(hipStreamSynchronize(s), "rotation");
+        }
+        return ZGL_SUCCESS;
+    });
+}
+}  // namespace
+}  // namespace zgl
+
+extern "C" zgl_status zgl_rotate(zgl_plan* plan, double* coeffs, size_t order, const double* euler_angles,
+                                 int rotation_type, size_t batch, ptrdiff_t coeff_stride, zgl_mem mem,
+                                 void* stream) {
+    return zgl::rotate_entry(plan, coeffs, order, euler_angles, rotation_type, batch, coeff_stride, mem,
+                             stream, false);
+}
+
+extern "C" zgl_status zgl_polar_rotate(zgl_plan* plan, double* coeffs, size_t order, const double* angles,
+                                       int rotation_type, size_t batch, ptrdiff_t coeff_stride, zgl_mem mem,
+                                       void* stream) {
+    return zgl::rotate_entry(plan, coeffs, order, angles, rotation_type, batch, coeff_stride, mem, stream,
+                             true);
+}
+
+extern "C" zgl_status zgl_wignerd_pi2(zgl_plan* plan, size_t order, double* out, zgl_mem mem, void* stream) {
+    using namespace zgl;
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
+    if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
+    if (order == 0) return ZGL_SUCCESS;
+    if (!out) return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    return guarded([&]() -> zgl_status {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const double* w = wigner_for(plan, (int)order, s);
+        const size_t n = (size_t)wignerd_size((int)order);
+        check_hip(hipMemcpyAsync(out, w, n * sizeof(double),
+                                 mem == ZGL_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s),
+                  "d table copy");
+        if (mem == ZGL_MEM_HOST) check_hip(hipStreamSynchronize(s), "d table copy");
+        return ZGL_SUCCESS;
+    });
+}
+
+extern "C" void zgl_euler_angles_from_rotation_matrix(const double* m, double* angles) {
+    const double r22_sq = m[8] * m[8];
+    const double beta_y = std::sqrt((1.0 + r22_sq) * (1.0 - r22_sq));
+    angles[0] = std::atan2(m[5], m[2]);
+    angles[1] = std::atan2(beta_y, m[8]);
+    angles[2] = std::atan2(m[7], -m[6]);
 }

@@ -265,6 +265,21 @@ long spectrum_size(zgl_kind kind, int order);
 void power_spectrum(zgl_kind kind, int sh_norm, int order, Inst<const double> a,
                     Inst<const double> b, double* out, long out_stride, int nb, hipStream_t s);
 
+// ---- rotations (zgl_rotate.hip) ----
+// Doubles of the Wigner d(pi/2) table of degrees l < order, zest's layout
+// [l(l+1)(2l+1)/6 + (l+1) m1 + m2].
+long wignerd_size(int order);
+void wignerd_table(int order, double* d, hipStream_t s);
+// Work items (l, row parity, row tile, column tile) of a batch of nb.
+std::vector<int4> rotation_work(bool zt, int order, int nb);
+// In-place ZYZ rotation of nb expansions; angles [nb][3] (device); temp holds
+// nb complex copies of the coefficient layout (temp_stride doubles apart).
+void rotate(bool zt, int order, const double* dtab, const int4* work, int nwork, Inst<double> coeffs,
+            double* temp, long temp_stride, const double* angles, int coordinate, int nb, hipStream_t s);
+// In-place rotation about z; angles [nb] (device).
+void polar_rotate(bool zt, int order, Inst<double> coeffs, const double* angles, int coordinate, int nb,
+                  hipStream_t s);
+
 // ---- elementwise helpers (zgl_tables.hip) ----
 // r[b][i] = g[b][i] - r[b][i] for i < n, instance strides in elements.
 template <class G>
