@@ -143,13 +143,15 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy phase (profiling runs): the mixed forms then use the "
                          "plain FP32 forward and no accuracy is reported")
-    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad", "eval"],
+    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad", "eval", "rotate"],
                     default="zernike128",
                     help="zernike128: the headline (64 instances at order 128); ragged: 10,000 "
                          "instances of orders U{8..96}; sh512: 4096 SH instances at L=512, "
                          "precision report; grad: 256 least-squares gradients at order 64; "
                          "eval: grid evaluation of 64 order-128 Zernike expansions on "
-                         "256 x 128 x 128 points (and 64 SH expansions, L=512, on 1024 x 512)")
+                         "256 x 128 x 128 points (and 64 SH expansions, L=512, on 1024 x 512); "
+                         "rotate: ZYZ rotations of 64 order-128 Zernike expansions (and 64 SH "
+                         "expansions at L=512)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the transforms are "
@@ -157,7 +159,7 @@ def main():
               "share of instances; no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
     {"zernike128": run_headline, "ragged": run_ragged, "sh512": run_sh512,
-     "grad": run_grad, "eval": run_eval}[args.config](args)
+     "grad": run_grad, "eval": run_eval, "rotate": run_rotate}[args.config](args)
 
 
 def run_headline(args):
@@ -649,6 +651,120 @@ def cpu_eval_baseline(order, c, lon, col, r, seconds):
                        f"{len(lon)}x{len(col)}x{len(r)} grid in {elapsed:.1f} s, {threads} threads, "
                        + ("zest GridEvaluator compiled from its sources" if kind == "reference"
                           else "C restatement of zest"))}
+
+
+def rotation_flops(zt: bool, order: int, lmax: int = 255) -> int:
+    """Algorithmic flops of one rotation (FP64) over the degrees 1 <= l <=
+    lmax: per block of degree l, two X steps, each (l+1)^2 multiply-adds (the
+    two parity-masked products)."""
+    total = 0
+    for l in range(1, min(order, lmax + 1)):
+        blocks = (order - l + 1) // 2 if zt else 1
+        total += blocks * 2 * 2 * (l + 1) ** 2
+    return total
+
+
+def run_rotate(args):
+    """Rotor::rotate of 64 Zernike expansions at order 128 (coefficients
+    N(0,1)/(1+n), SplitMix64 seeds 0..63), each by its own Euler angles
+    (uniform in [-pi, pi), seed 5), object rotation. One step = one
+    zgl_rotate call for all 64 instances in place, inputs resident in HBM.
+    Beside it 64 SH expansions at L = 512. The roofline is the rotation
+    kernel's (FP64 MFMA), per launch with HIP events; the CPU baseline runs
+    zest's Rotor (oracle/_ref) on the host cores over a bounded sample."""
+    import numpy as np
+    import torch
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    dev = torch.device("cuda:0")
+    N, B = 128, 64
+    cs = np.stack([synth.zernike_coeffs(N, i) for i in range(B)])
+    ang = synth.uniform(5, 3 * B).reshape(B, 3) * 2 * np.pi - np.pi
+    dc, da = torch.from_numpy(cs).to(dev), torch.from_numpy(ang).to(dev)
+    plan = zest_amd.Plan(zest_amd.ZERNIKE, 4)
+    # accuracy of one rotation against the oracle (two instances)
+    plan.rotate(dc, N, da, 0, batch=B)
+    got = dc.cpu().numpy()
+    worst = max(float(np.abs(got[b] - oracle.rotate(cs[b], N, oracle.ZERNIKE_KIND, ang[b], 0)).max()
+                      / np.abs(cs[b]).max()) for b in (0, B - 1))
+
+    def step():
+        plan.rotate(dc, N, da, 0, batch=B)
+
+    t = _timed_steps(step, args.steps, args.warmup)
+    plan.profile(True)
+    for _ in range(3):
+        step()
+    stats = plan.stage_stats()
+    plan.profile(False)
+    launches, inst, ms = stats["rotate"]
+    per_launch = rotation_flops(True, N) * inst / launches  # the fused kernel: the whole rotation
+    ach = per_launch / (ms / launches / 1e3) / 1e12
+    out = {"metric": "Zernike rotations/s (order 128, ZYZ Euler angles per instance)",
+           "value": round(B * args.steps / t, 2), "unit": "rotations/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "zernike_rotate", "order": N, "instances": B,
+                      "angles": "U[-pi, pi)^3 per instance, seed 5", "rotation_type": "object"},
+           "max_rel_err_vs_oracle": worst,
+           "roofline": {"bound": "mfma", "kernel": "rotate_kernel", "achieved": round(ach, 3),
+                        "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                        "frac": round(ach / PEAK_TFLOPS["f64"], 4), "traffic": None,
+                        "ms_per_launch": round(ms / launches, 4)}}
+    # SH, L = 512
+    L = 512
+    sc = np.stack([synth.sh_coeffs(L, i) for i in range(B)])
+    sang = synth.uniform(6, 3 * B).reshape(B, 3) * 2 * np.pi - np.pi
+    dsc, dsa = torch.from_numpy(sc).to(dev), torch.from_numpy(sang).to(dev)
+    splan = zest_amd.Plan(zest_amd.SH, 8)
+    splan.rotate(dsc, L, dsa, 0, batch=B)
+    serr = float(np.abs(dsc[0].cpu().numpy() - oracle.rotate(sc[0], L, oracle.SH_KIND, sang[0], 0)).max()
+                 / np.abs(sc[0]).max())
+    ts = _timed_steps(lambda: splan.rotate(dsc, L, dsa, 0, batch=B), args.steps, 1)
+    out["sh"] = {"value": round(B * args.steps / ts, 2), "unit": "rotations/s", "order": L, "instances": B,
+                 "max_rel_err_vs_oracle": serr}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_rotate_baseline(N, cs, ang, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_rotate_baseline(order, cs, ang, seconds):
+    """zest's Rotor (or the oracle) on the host, one Rotor per thread."""
+    import numpy as np
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    kind = "reference" if oracle.REF is not None else "port"
+    counts = [0] * threads
+    t0 = time.perf_counter()
+    deadline = t0 + seconds
+
+    per_call = 8  # rotations per call: one d table and one Rotor serve them
+
+    def work(i):
+        c = np.array(cs[:per_call], copy=True)
+        a = np.ascontiguousarray(ang[:per_call])
+        while time.perf_counter() < deadline:
+            if kind == "reference":
+                oracle.REF.zref_rotate_many(0, order, oracle._p(c), c[0].size, per_call, oracle._p(a), 0)
+            else:
+                for k in range(per_call):
+                    c[k] = oracle.rotate(c[k], order, oracle.ZERNIKE_KIND, a[k], 0)
+            counts[i] += per_call
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    elapsed = time.perf_counter() - t0
+    total = sum(counts)
+    return {"value": round(total / elapsed, 3), "unit": "rotations/s", "cores": threads, "kind": kind,
+            "sample": (f"{total} rotations of order-{order} Zernike expansions in {elapsed:.1f} s, "
+                       f"{threads} threads, "
+                       + ("zest Rotor compiled from its sources, one WignerdPiHalfCollection per 8 "
+                          "rotations" if kind == "reference" else "C restatement of zest"))}
 
 
 if __name__ == "__main__":

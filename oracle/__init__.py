@@ -85,6 +85,7 @@ def _load_ref():
     _d = ctypes.c_double
     lib.zref_wignerd.argtypes = [_c_size, _ptr]
     lib.zref_euler_angles.argtypes = [_ptr, _ptr]
+    lib.zref_rotate_many.argtypes = [_c_int, _c_size, _ptr, _c_size, _c_size, _ptr, _c_int]
     lib.zref_rotate.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_size, _ptr, _d, _d, _d, _c_int]
     lib.zref_polar_rotate.argtypes = [_c_int, _c_int, _c_int, _c_int, _c_size, _ptr, _d, _c_int]
     return lib

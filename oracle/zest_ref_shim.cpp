@@ -374,6 +374,23 @@ void zref_rotate(int kind, int zernike_norm, int sh_norm, int sh_phase, std::siz
         with_st_span(sh_norm, sh_phase, coeffs, order, [&](auto e) { rotor.rotate(e, w, angles, t); });
 }
 
+// `count` expansions of one layout, `stride` doubles apart, rotated in turn by
+// angles[3 i .. 3 i + 2] with one collection and one Rotor (the benchmark's
+// CPU baseline).
+void zref_rotate_many(int kind, std::size_t order, double* coeffs, std::size_t stride, std::size_t count,
+                      const double* angles, int type) {
+    zest::WignerdPiHalfCollection w(order);
+    zest::Rotor rotor(order);
+    const auto t = type == 0 ? zest::RotationType::object : zest::RotationType::coordinate;
+    for (std::size_t i = 0; i < count; ++i) {
+        const std::array<double, 3> a{angles[3 * i], angles[3 * i + 1], angles[3 * i + 2]};
+        if (kind == 0)
+            with_zt_span(1, 0, -1, coeffs + i * stride, order, [&](auto e) { rotor.rotate(e, w, a, t); });
+        else
+            with_st_span(0, -1, coeffs + i * stride, order, [&](auto e) { rotor.rotate(e, w, a, t); });
+    }
+}
+
 void zref_polar_rotate(int kind, int zernike_norm, int sh_norm, int sh_phase, std::size_t order,
                        double* coeffs, double angle, int type) {
     zest::Rotor rotor(order);

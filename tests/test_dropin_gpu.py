@@ -8,7 +8,9 @@ order-100 round trip at 1e-13, all for (normed, geo, none) and
 spherical harmonics over both grid layouts and all four conventions) and
 tests/test_grid_evaluator.cpp (constant, Y_10, Y_31 + Y_4,-3, Z_33,-2 + Z_531
 on grids through both poles, r = 0 and r = 1, at 1e-13, for every
-convention combination). They are
+convention combination) and tests/test_rotor.cpp (the d(pi/2) matrices to
+l = 4 against closed forms, rotations of SH, Zernike and row-skipping
+Zernike-SH spans, identity rotations at 1e-13). They are
 built by tests/dropin/Makefile where the zest sources are present; the test
 skips when the binaries were not built.
 """
@@ -23,7 +25,7 @@ BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "_bin")
 
 
 @pytest.mark.parametrize("name", ["zest_test_zernike_glq_transformer", "zest_test_sh_glq_transformer",
-                                  "zest_test_grid_evaluator"])
+                                  "zest_test_grid_evaluator", "zest_test_rotor"])
 def test_zest_transformer_tests_pass_on_gpu(name):
     exe = os.path.join(BIN, name)
     if not os.path.exists(exe):

@@ -30,12 +30,15 @@ def _coeffs(kind, order, batch, seed):
 
 
 @pytest.mark.parametrize("kind", ["zernike", "sh"])
-@pytest.mark.parametrize("order,batch", [(2, 1), (3, 2), (9, 3), (20, 40), (64, 2), (130, 2), (24, 70)])
+@pytest.mark.parametrize("order,batch", [(2, 1), (3, 2), (9, 3), (20, 40), (64, 2), (65, 3), (130, 2),
+                                         (24, 70), (300, 2)])
 @pytest.mark.parametrize("rtype", [0, 1])
 def test_rotate_matches_oracle(device, kind, order, batch, rtype):
     import torch
     import oracle
     import zest_amd
+    if kind == "zernike" and order > 200:
+        pytest.skip("Zernike order 300: covered through SH (same kernels per degree)")
     print(f"rotate {kind} order={order} batch={batch} type={rtype}", flush=True)
     zt = kind == "zernike"
     c = _coeffs(kind, order, batch, 10)

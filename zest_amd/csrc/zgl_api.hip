@@ -114,7 +114,7 @@ struct zgl_plan {
     // per (kind, order, batch), the intermediate and the uploaded angles
     double* wtab = nullptr;
     int wtab_order = 0;
-    std::map<std::tuple<int, int, int>, std::pair<int4*, int>> rot_work;
+    std::map<std::tuple<int, int, int, int>, std::pair<int4*, int>> rot_work;  // (fused?, kind, order, batch)
     double* rtemp = nullptr;
     size_t rtemp_cap = 0;
     double* dang = nullptr;
@@ -217,11 +217,12 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
 }
 
 enum Stage { ST_FFT_FWD, ST_LEG_FWD, ST_RAD_FWD, ST_RAD_BWD, ST_LEG_BWD, ST_FFT_BWD,
-             ST_EVAL_TABLES, ST_EVAL_RAD, ST_EVAL_LEG, ST_EVAL_LON, ST_COUNT };
+             ST_EVAL_TABLES, ST_EVAL_RAD, ST_EVAL_LEG, ST_EVAL_LON, ST_ROTATE, ST_ROTATE_WIDE, ST_POLAR,
+             ST_COUNT };
 const char* const kStageNames[ST_COUNT] = {"fft_forward", "legendre_forward", "radial_forward",
                                            "radial_backward", "legendre_backward", "fft_backward",
                                            "eval_tables", "eval_radial", "eval_legendre",
-                                           "eval_longitude"};
+                                           "eval_longitude", "rotate", "rotate_wide", "polar_rotate"};
 
 hipEvent_t pooled_event(zgl_plan* p) {
     if (!p->pool.empty()) {
@@ -1159,11 +1160,12 @@ const double* wigner_for(zgl_plan* p, int order, hipStream_t s) {
     return p->wtab;
 }
 
-std::pair<int4*, int> rot_work_for(zgl_plan* p, int order, int nb, hipStream_t s) {
-    const auto key = std::make_tuple((int)p->kind, order, nb);
+std::pair<int4*, int> rot_work_for(zgl_plan* p, bool fused, int order, int nb, hipStream_t s) {
+    const auto key = std::make_tuple(fused ? 1 : 0, (int)p->kind, order, nb);
     auto it = p->rot_work.find(key);
     if (it != p->rot_work.end()) return it->second;
-    const std::vector<int4> w = rotation_work(p->kind == ZGL_ZERNIKE, order, nb);
+    const std::vector<int4> w = fused ? rotation_fused_work(p->kind == ZGL_ZERNIKE, order, nb)
+                                      : rotation_work(p->kind == ZGL_ZERNIKE, order, nb);
     int4* d = nullptr;
     check_hip(hipMalloc((void**)&d, std::max<size_t>(w.size(), 1) * sizeof(int4)), "work list");
     if (!w.empty()) {
@@ -1206,12 +1208,23 @@ zgl_status rotate_entry(zgl_plan* plan, double* coeffs, size_t order, const doub
         }
         const Inst<double> ci{dc, dcs, nullptr};
         if (polar) {
-            polar_rotate(zt, (int)order, ci, da, rtype, (int)batch, s);
+            timed(plan, ST_POLAR, (int)batch, s, [&] { polar_rotate(zt, (int)order, ci, da, rtype, (int)batch, s); });
         } else {
             const double* w = wigner_for(plan, (int)order, s);
-            const auto work = rot_work_for(plan, (int)order, (int)batch, s);
-            ensure(plan->rtemp, plan->rtemp_cap, (size_t)csz * batch);
-            rotate(zt, (int)order, w, work.first, work.second, ci, plan->rtemp, csz, da, rtype, (int)batch, s);
+            const auto fw = rot_work_for(plan, true, (int)order, (int)batch, s);
+            timed(plan, ST_ROTATE, (int)batch, s, [&] {
+                rotate_fused(zt, (int)order, w, fw.first, fw.second, ci, da, rtype, (int)batch, s);
+            });
+            const auto work = rot_work_for(plan, false, (int)order, (int)batch, s);
+            if (work.second > 0) {
+                // degrees above 255: two passes through an intermediate
+                ensure(plan->rtemp, plan->rtemp_cap, (size_t)csz * batch);
+                for (int pass = 0; pass < 2; ++pass)
+                    timed(plan, ST_ROTATE_WIDE, (int)batch, s, [&] {
+                        rotate_pass(pass, zt, (int)order, w, work.first, work.second, ci, plan->rtemp, csz,
+                                    da, rtype, (int)batch, s);
+                    });
+            }
         }
         if (mem == ZGL_MEM_HOST) {
             check_hip(hipMemcpy2DAsync(coeffs, cs * sizeof(double), plan->dcoef, csz * sizeof(double),

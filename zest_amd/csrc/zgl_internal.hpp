@@ -270,12 +270,21 @@ void power_spectrum(zgl_kind kind, int sh_norm, int order, Inst<const double> a,
 // [l(l+1)(2l+1)/6 + (l+1) m1 + m2].
 long wignerd_size(int order);
 void wignerd_table(int order, double* d, hipStream_t s);
-// Work items (l, row parity, row tile, column tile) of a batch of nb.
+// Work items of a batch of nb: (l, column tile) for the fused kernel (l <=
+// 255), (l, row parity, row tile, column tile) for the two passes (l > 255).
+std::vector<int4> rotation_fused_work(bool zt, int order, int nb);
 std::vector<int4> rotation_work(bool zt, int order, int nb);
-// In-place ZYZ rotation of nb expansions; angles [nb][3] (device); temp holds
-// nb complex copies of the coefficient layout (temp_stride doubles apart).
-void rotate(bool zt, int order, const double* dtab, const int4* work, int nwork, Inst<double> coeffs,
-            double* temp, long temp_stride, const double* angles, int coordinate, int nb, hipStream_t s);
+// In-place ZYZ rotation of the degrees l <= 255 of nb expansions; angles
+// [nb][3] (device).
+void rotate_fused(bool zt, int order, const double* dtab, const int4* work, int nwork, Inst<double> coeffs,
+                  const double* angles, int coordinate, int nb, hipStream_t s);
+// One of the two passes of the in-place ZYZ rotation of the degrees l > 255 (pass 0:
+// coefficients -> temp through X(+90); pass 1: temp -> coefficients through
+// X(-90)); angles [nb][3] (device); temp holds nb complex copies of the
+// coefficient layout (temp_stride doubles apart).
+void rotate_pass(int pass, bool zt, int order, const double* dtab, const int4* work, int nwork,
+                 Inst<double> coeffs, double* temp, long temp_stride, const double* angles, int coordinate,
+                 int nb, hipStream_t s);
 // In-place rotation about z; angles [nb] (device).
 void polar_rotate(bool zt, int order, Inst<double> coeffs, const double* angles, int coordinate, int nb,
                   hipStream_t s);

@@ -129,7 +129,7 @@ constexpr int kRotRows = 64;   // rows of one parity per workgroup (16 per wave)
 constexpr int kRotCols = 64;   // columns per workgroup (4 blocks of 16, every wave)
 constexpr int kRotK = 64;      // m' per LDS chunk (32 of each parity)
 
-__global__ __launch_bounds__(256) void rotate_kernel(RotArgs a) {
+__global__ __launch_bounds__(256, 2) void rotate_kernel(RotArgs a) {
     __shared__ d2 Xs[kRotK * kRotCols];
     const int4 w = a.work[blockIdx.x];
     const int l = w.x, p = w.y;
@@ -143,63 +143,97 @@ __global__ __launch_bounds__(256) void rotate_kernel(RotArgs a) {
     const double* dl = a.dtab + wig_base(l);
     const int mrow = 2 * (i0 + 16 * wave + j) + p;  // A-operand row of this lane
     const bool rowok = mrow <= l;
+    const bool wave_active = 2 * (i0 + 16 * wave) + p <= l;  // wave-uniform
 
-    // pair index of (column, m')
+    // pair index of a column's m' = 0 element, and its instance
     auto locate = [&](int col, int& inst) -> long {
-        int q = 0;
         inst = col;
         if (a.zt) {
             inst = col / cnt;
-            q = col - inst * cnt;
-            return tet_idx(l + 2 * q, l, 0);
+            return tet_idx(l + 2 * (col - inst * cnt), l, 0);
         }
         return tri_idx(l, 0);
     };
+
+    // Staging: thread tid fills column tid % 64 of the chunk at m' = k0 +
+    // tid / 64 + 4u, u < 16; its column and instance are fixed.
+    const int sc = tid & (kRotCols - 1), sk = tid / kRotCols;
+    const int scol = c0 + sc;
+    const bool scolok = scol < ncols;
+    const double* sp = nullptr;
+    d2 step = {1.0, 0.0};  // e^{-4 i gamma'} (pass 0)
+    double sga = 0.0;
+    if (scolok) {
+        int inst;
+        const long base = locate(scol, inst);
+        sp = a.src.at(inst) + 2 * base;
+        double al, be;
+        converted(a, inst, al, be, sga);
+        double s4, c4;
+        sincos(-4.0 * sga, &s4, &c4);
+        step = d2{c4, s4};
+    }
 
     d4 are[4], aim[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) are[c] = aim[c] = d4{0, 0, 0, 0};
 
     for (int k0 = 0; k0 <= l; k0 += kRotK) {
-        __syncthreads();
-        for (int e = tid; e < kRotK * kRotCols; e += 256) {
-            const int kl = e / kRotCols, c = e - kl * kRotCols;
-            const int col = c0 + c, mp = k0 + kl;
-            d2 v = {0.0, 0.0};
-            if (col < ncols && mp <= l) {
-                int inst;
-                const long base = locate(col, inst);
-                if (a.stage == 0) {
-                    const double* x = a.src.at(inst) + 2 * (base + mp);
-                    v = mp ? d2{x[0] * kInvSqrt2, x[1] * -kInvSqrt2} : d2{x[0], x[1]};
-                    double al, be, ga;
-                    converted(a, inst, al, be, ga);
-                    v = phase(v, mp, ga);
-                } else {
-                    v = *reinterpret_cast<const d2*>(a.src.at(inst) + 2 * (base + mp));
-                }
-            }
-            Xs[e] = v;
-        }
-        __syncthreads();
+        // the d values of this chunk's MFMA steps (independent of the LDS)
+        double xa[kRotK / 8], xb[kRotK / 8];
 #pragma unroll
         for (int s = 0; s < kRotK / 8; ++s) {
             const int kk = 4 * s + g;
             const int mre = k0 + 2 * kk + pa, mim = k0 + 2 * kk + pb;
-            double xa = 0.0, xb = 0.0;
-            if (rowok && mre <= l)
-                xa = (mre ? 2.0 : 1.0) * (a.stage == 0 ? dl[(long)(l + 1) * mrow + mre] : dl[(long)(l + 1) * mre + mrow]);
-            if (rowok && mim <= l)
-                xb = 2.0 * (a.stage == 0 ? dl[(long)(l + 1) * mrow + mim] : dl[(long)(l + 1) * mim + mrow]);
+            xa[s] = xb[s] = 0.0;
+            if (wave_active && rowok && mre <= l)
+                xa[s] = (mre ? 2.0 : 1.0) *
+                        (a.stage == 0 ? dl[(long)(l + 1) * mrow + mre] : dl[(long)(l + 1) * mre + mrow]);
+            if (wave_active && rowok && mim <= l)
+                xb[s] = 2.0 * (a.stage == 0 ? dl[(long)(l + 1) * mrow + mim] : dl[(long)(l + 1) * mim + mrow]);
+        }
+        __syncthreads();  // the previous chunk's LDS reads are done
+        // two halves of 8 independent loads each
+        double s0 = 0.0, c0v = 1.0;
+        if (a.stage == 0 && scolok) sincos(-(double)(k0 + sk) * sga, &s0, &c0v);
+        d2 ph = {c0v, s0};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const double bre = Xs[(2 * kk + pa) * kRotCols + 16 * c + j][0];
-                const double bim = Xs[(2 * kk + pb) * kRotCols + 16 * c + j][1];
-                are[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, bre, are[c], 0, 0, 0);
-                aim[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb, bim, aim[c], 0, 0, 0);
+        for (int h = 0; h < 2; ++h) {
+            d2 v[kRotK / 8];
+#pragma unroll
+            for (int u = 0; u < kRotK / 8; ++u) {
+                const int mp = k0 + sk + 4 * (8 * h + u);
+                v[u] = (scolok && mp <= l) ? *reinterpret_cast<const d2*>(sp + 2 * mp) : d2{0.0, 0.0};
+            }
+            if (a.stage == 0) {
+                // v_m' scaled to the complex form, times e^{-i m' gamma'}
+#pragma unroll
+                for (int u = 0; u < kRotK / 8; ++u) {
+                    const int mp = k0 + sk + 4 * (8 * h + u);
+                    const d2 x = mp ? d2{v[u][0] * kInvSqrt2, v[u][1] * -kInvSqrt2} : v[u];
+                    v[u] = d2{x[0] * ph[0] - x[1] * ph[1], x[0] * ph[1] + x[1] * ph[0]};
+                    ph = d2{ph[0] * step[0] - ph[1] * step[1], ph[0] * step[1] + ph[1] * step[0]};
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kRotK / 8; ++u) Xs[(sk + 4 * (8 * h + u)) * kRotCols + sc] = v[u];
+        }
+        __syncthreads();
+        if (wave_active) {
+#pragma unroll
+            for (int s = 0; s < kRotK / 8; ++s) {
+                const int kk = 4 * s + g;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double bre = Xs[(2 * kk + pa) * kRotCols + 16 * c + j][0];
+                    const double bim = Xs[(2 * kk + pb) * kRotCols + 16 * c + j][1];
+                    are[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], bre, are[c], 0, 0, 0);
+                    aim[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[s], bim, aim[c], 0, 0, 0);
+                }
             }
         }
     }
+    if (!wave_active) return;
 
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -223,6 +257,231 @@ __global__ __launch_bounds__(256) void rotate_kernel(RotArgs a) {
             }
             *reinterpret_cast<d2*>(out + 2 * m) = y;
         }
+    }
+}
+
+// ---------------------------------------------------------------- fused
+// Degrees l <= 255: one workgroup holds every m of CT columns in LDS
+// ((l+1) CT <= 4096 complex values) and runs the whole rotation there:
+// stage (scale, gamma phase) -> X(+90) into registers -> beta phase back into
+// LDS -> X(-90) -> alpha phase, rescale, store. The coefficients cross HBM
+// once each way. The MFMA tiles are (row parity, 16-row block, 16-column
+// block) tasks, at most four per wave, advanced in lockstep so that the d
+// values of the next step (read from L2) are in flight during the current
+// step's MFMAs.
+constexpr int kFusedMaxL = 255;
+constexpr int kFusedElems = 4096;
+
+__host__ __device__ inline int fused_cols(int l) { return l < 64 ? 64 : (l < 128 ? 32 : 16); }
+
+// LDS: the (l+1) x CT complex block, then per column its coefficient block
+// offset and the converted angles
+constexpr size_t kFusedLds = kFusedElems * sizeof(d2) + 64 * (sizeof(long) + 3 * sizeof(double));
+
+__global__ __launch_bounds__(256, 2) void rotate_fused_kernel(RotArgs a) {
+    extern __shared__ __align__(16) unsigned char fused_smem[];
+    d2* Xs = reinterpret_cast<d2*>(fused_smem);
+    long* coff = reinterpret_cast<long*>(Xs + kFusedElems);     // [CT] doubles from coefficient base
+    double* cang = reinterpret_cast<double*>(coff + 64);        // [CT][3] alpha', beta', gamma'
+    const int4 w = a.work[blockIdx.x];
+    const int l = w.x;
+    const int CT = fused_cols(l), CB = CT / 16;
+    const int c0 = w.y * CT;
+    const int cnt = a.zt ? (a.order - l + 1) / 2 : 1;
+    const int ncols = a.nb * cnt;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int j = lane & 15, g = lane >> 4;
+    const double* dl = a.dtab + wig_base(l);
+    const int L1 = l + 1;
+    const int nel = L1 * CT;
+
+    // ---- per column: where its block starts (a.src and a.dst share the layout), its angles
+    if (tid < CT) {
+        const int col = c0 + tid;
+        long off = -1;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        if (col < ncols) {
+            int inst = col;
+            long base = tri_idx(l, 0);
+            if (a.zt) {
+                inst = col / cnt;
+                base = tet_idx(l + 2 * (col - inst * cnt), l, 0);
+            }
+            off = (a.src.at(inst) - a.src.base) + 2 * base;
+            converted(a, inst, al, be, ga);
+        }
+        coff[tid] = off;
+        cang[3 * tid] = al;
+        cang[3 * tid + 1] = be;
+        cang[3 * tid + 2] = ga;
+    }
+    __syncthreads();
+
+    // ---- stage: element e = (column e / L1, m' e % L1): consecutive lanes read
+    // consecutive pairs of one block
+    for (int e0 = 0; e0 < nel; e0 += 256 * 8) {
+        d2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + 256 * u + tid;
+            v[u] = d2{0.0, 0.0};
+            if (e < nel) {
+                const int c = e / L1, mp = e - c * L1;
+                const long off = coff[c];
+                if (off >= 0) v[u] = *reinterpret_cast<const d2*>(a.src.base + off + 2 * mp);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + 256 * u + tid;
+            if (e < nel) {
+                const int c = e / L1, mp = e - c * L1;
+                Xs[mp * CT + c] = v[u];
+            }
+        }
+    }
+    // In place on Xs: thread tid takes column tid % CT at rows tid / CT +
+    // (256 / CT) u and multiplies by e^{-i m x_c}, x_c angle `which` of its
+    // column, the power stepped by a complex product (two sincos per thread).
+    // mode 0: complex form first (m > 0 pairs times (1/sqrt2, -1/sqrt2));
+    // mode 2: back to {cos, sin} pairs after (times (sqrt2, -sqrt2)).
+    auto phase_pass = [&](int which, int mode) {
+        const int c = tid % CT, r0 = tid / CT, stride = 256 / CT;
+        const double x = cang[3 * c + which];
+        double ss, cs;
+        sincos(-(double)stride * x, &ss, &cs);
+        const d2 step = {cs, ss};
+        sincos(-(double)r0 * x, &ss, &cs);
+        d2 ph = {cs, ss};
+        for (int m = r0; m <= l; m += stride) {
+            d2 v = Xs[m * CT + c];
+            if (mode == 0 && m) v = d2{v[0] * kInvSqrt2, v[1] * -kInvSqrt2};
+            v = d2{v[0] * ph[0] - v[1] * ph[1], v[0] * ph[1] + v[1] * ph[0]};
+            if (mode == 2 && m) v = d2{v[0] * kSqrt2, v[1] * -kSqrt2};
+            Xs[m * CT + c] = v;
+            ph = d2{ph[0] * step[0] - ph[1] * step[1], ph[0] * step[1] + ph[1] * step[0]};
+        }
+    };
+    __syncthreads();
+    phase_pass(2, 0);  // gamma'
+
+    // ---- tasks of this wave: row tasks (row parity, 16-row block) w, w + 4,
+    // .., each with all CB column blocks; four (row task, column block) slots
+    // per wave, slot k = (row task k / CB, column block k % CB)
+    const int rb0 = (l / 2 + 1 + 15) / 16;        // parity 0: rows m = 0, 2, .., <= l
+    const int rb1 = ((l - 1) / 2 + 1 + 15) / 16;  // parity 1 (l >= 1)
+    const int NR = rb0 + rb1;                     // row tasks
+    const int nrt = 4 / CB;                       // row tasks per wave
+    int rp[4], rrow[4];
+    bool rok[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int t = wave + 4 * r;
+        rok[r] = r < nrt && t < NR;
+        rp[r] = t < rb0 ? 0 : 1;
+        rrow[r] = 2 * (16 * (t - (rp[r] ? rb0 : 0))) + rp[r];  // m of row 0; row i is rrow + 2i
+    }
+    int tr[4], tcol[4];
+    bool tok[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        tr[k] = k / CB;
+        tcol[k] = 16 * (k % CB);
+        tok[k] = rok[tr[k]];
+    }
+    int tp[4], trow[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        tp[k] = rp[tr[k]];
+        trow[k] = rrow[tr[k]];
+    }
+    // steps of 4 values of m' of one parity: the longer of the two parity runs
+    const int steps = (l / 2 + 1 + 3) / 4;
+
+    d4 are[4], aim[4];
+    auto product = [&](int pass) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) are[k] = aim[k] = d4{0, 0, 0, 0};
+        auto dval = [&](int m, int mp) -> double {
+            return pass == 0 ? dl[(long)(l + 1) * m + mp] : dl[(long)(l + 1) * mp + m];
+        };
+        // d values of step s for each row task
+        auto load = [&](int s, double (&xa)[4], double (&xb)[4]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int pa = (l + rp[r]) & 1;
+                const int m = rrow[r] + 2 * j;
+                const int mre = 2 * (4 * s + g) + pa, mim = 2 * (4 * s + g) + (pa ^ 1);
+                xa[r] = (rok[r] && m <= l && mre <= l) ? (mre ? 2.0 : 1.0) * dval(m, mre) : 0.0;
+                xb[r] = (rok[r] && m <= l && mim <= l) ? 2.0 * dval(m, mim) : 0.0;
+            }
+        };
+        // two steps in flight ahead of the MFMAs
+        double xa0[4], xb0[4], xa1[4], xb1[4];
+        load(0, xa0, xb0);
+        load(1, xa1, xb1);
+        for (int s = 0; s < steps; ++s) {
+            double na[4] = {0, 0, 0, 0}, nb[4] = {0, 0, 0, 0};
+            if (s + 2 < steps) load(s + 2, na, nb);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!tok[k]) continue;
+                const int pa = (l + tp[k]) & 1;
+                const int mre = 2 * (4 * s + g) + pa, mim = 2 * (4 * s + g) + (pa ^ 1);
+                const double bre = mre <= l ? Xs[mre * CT + tcol[k] + j][0] : 0.0;
+                const double bim = mim <= l ? Xs[mim * CT + tcol[k] + j][1] : 0.0;
+                are[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0[tr[k]], bre, are[k], 0, 0, 0);
+                aim[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb0[tr[k]], bim, aim[k], 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                xa0[r] = xa1[r];
+                xb0[r] = xb1[r];
+                xa1[r] = na[r];
+                xb1[r] = nb[r];
+            }
+        }
+    };
+
+    __syncthreads();
+    product(0);
+    __syncthreads();  // every X(+90) read of Xs is done
+    // y back into Xs, times e^{-i m beta'}
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!tok[k]) continue;
+        const int c = tcol[k] + j;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = trow[k] + 2 * (g + 4 * r);
+            if (m <= l) Xs[m * CT + c] = d2{are[k][r], aim[k][r]};
+        }
+    }
+    __syncthreads();
+    phase_pass(1, 1);
+    __syncthreads();
+    product(1);
+    __syncthreads();  // every X(-90) read of Xs is done
+    // z into Xs, times e^{-i m alpha'} as {cos, sin} pairs, then out with the
+    // staging's mapping
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!tok[k]) continue;
+        const int c = tcol[k] + j;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = trow[k] + 2 * (g + 4 * r);
+            if (m <= l) Xs[m * CT + c] = d2{are[k][r], aim[k][r]};
+        }
+    }
+    __syncthreads();
+    phase_pass(0, 2);
+    __syncthreads();
+    for (int e = tid; e < nel; e += 256) {
+        const int c = e / L1, mp = e - c * L1;
+        const long off = coff[c];
+        if (off >= 0) *reinterpret_cast<d2*>(a.dst.base + off + 2 * mp) = Xs[mp * CT + c];
     }
 }
 
@@ -270,9 +529,20 @@ void wignerd_table(int order, double* d, hipStream_t s) {
     check_hip(hipGetLastError(), "d table launch");
 }
 
+std::vector<int4> rotation_fused_work(bool zt, int order, int nb) {
+    std::vector<int4> w;
+    for (int l = 1; l < order && l <= kFusedMaxL; ++l) {
+        const int cnt = zt ? (order - l + 1) / 2 : 1;
+        const long ncols = (long)nb * cnt;
+        const int CT = fused_cols(l);
+        for (int c = 0; c < (int)((ncols + CT - 1) / CT); ++c) w.push_back(make_int4(l, c, 0, 0));
+    }
+    return w;
+}
+
 std::vector<int4> rotation_work(bool zt, int order, int nb) {
     std::vector<int4> w;
-    for (int l = 1; l < order; ++l) {
+    for (int l = kFusedMaxL + 1; l < order; ++l) {
         const int cnt = zt ? (order - l + 1) / 2 : 1;
         const long ncols = (long)nb * cnt;
         const int ct = (int)((ncols + kRotCols - 1) / kRotCols);
@@ -287,16 +557,25 @@ std::vector<int4> rotation_work(bool zt, int order, int nb) {
     return w;
 }
 
-void rotate(bool zt, int order, const double* dtab, const int4* work, int nwork, Inst<double> coeffs,
-            double* temp, long temp_stride, const double* angles, int coordinate, int nb, hipStream_t s) {
+void rotate_fused(bool zt, int order, const double* dtab, const int4* work, int nwork, Inst<double> coeffs,
+                  const double* angles, int coordinate, int nb, hipStream_t s) {
     if (nwork == 0 || nb == 0) return;
-    RotArgs a{dtab, work, zt ? 1 : 0, order, nb, Inst<const double>(coeffs),
-              Inst<double>{temp, temp_stride, nullptr}, angles, coordinate, 0};
-    rotate_kernel<<<nwork, 256, 0, s>>>(a);
+    RotArgs a{dtab, work, zt ? 1 : 0, order, nb, Inst<const double>(coeffs), coeffs, angles, coordinate, 0};
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&rotate_fused_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFusedLds),
+              "LDS attribute");
+    rotate_fused_kernel<<<nwork, 256, kFusedLds, s>>>(a);
     check_hip(hipGetLastError(), "rotation launch");
-    a.src = Inst<const double>{temp, temp_stride, nullptr};
-    a.dst = coeffs;
-    a.stage = 1;
+}
+
+void rotate_pass(int pass, bool zt, int order, const double* dtab, const int4* work, int nwork,
+                 Inst<double> coeffs, double* temp, long temp_stride, const double* angles, int coordinate,
+                 int nb, hipStream_t s) {
+    if (nwork == 0 || nb == 0) return;
+    const Inst<double> tmp{temp, temp_stride, nullptr};
+    RotArgs a{dtab, work, zt ? 1 : 0, order, nb,
+              pass == 0 ? Inst<const double>(coeffs) : Inst<const double>(tmp),
+              pass == 0 ? tmp : coeffs, angles, coordinate, pass};
     rotate_kernel<<<nwork, 256, 0, s>>>(a);
     check_hip(hipGetLastError(), "rotation launch");
 }
