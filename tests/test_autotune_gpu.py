@@ -41,4 +41,38 @@ def test_tuned_results_identical(device, precision):
             assert (order, st) in stages
     for o, rb, gb, st, choice in tuning:
         assert rb == (8 if precision == 0 else 4) and gb == 8
-        assert choice in ((16, 8, 4, 2, 1) if st.startswith("fft") else (1, 2, 4, 8))
+        assert choice in ((32, 16, 8, 4, 2, 1) if st.startswith("fft") else (1, 2, 4, 8))
+
+
+@pytest.mark.parametrize("kind,layout", [("zernike", 0), ("sh", 0), ("sh", 1)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fft_tile_widths_agree(device, monkeypatch, kind, layout, precision):
+    """Every FFT tile width (ZGL_FFT_TILE forces one at table build) runs the
+    same per-column arithmetic: bit-identical grids and coefficients, and the
+    FP64 form matches the oracle."""
+    import torch
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    zt = kind == "zernike"
+    order, batch = (40, 3) if zt else (100, 3)
+    gen = synth.zernike_coeffs if zt else synth.sh_coeffs
+    c = np.stack([gen(order, 7 + b) for b in range(batch)])
+    shape = zest_amd.zt_shape(order) if zt else zest_amd.st_shape(order, layout)
+    res = {}
+    for tile in ("32", "16", "8"):
+        monkeypatch.setenv("ZGL_FFT_TILE", tile)
+        plan = zest_amd.Plan(zest_amd.ZERNIKE if zt else zest_amd.SH, order, layout=layout,
+                             precision=precision)
+        dc = torch.from_numpy(c).to(device)
+        dg = torch.empty((batch,) + shape, dtype=torch.float64, device=device)
+        co = torch.zeros_like(dc)
+        plan.backward(dc, dg, order, batch=batch)
+        plan.forward(dg, co, order, batch=batch)
+        torch.cuda.synchronize()
+        res[tile] = (dg.cpu().numpy(), co.cpu().numpy())
+    for tile in ("16", "8"):
+        assert np.array_equal(res[tile][0], res["32"][0]) and np.array_equal(res[tile][1], res["32"][1]), tile
+    if precision == 0:
+        ref = oracle.zt_backward(c[0], order) if zt else oracle.st_backward(c[0], order, layout=layout)
+        assert np.abs(res["32"][0][0] - ref).max() <= 1e-12 * np.abs(ref).max()

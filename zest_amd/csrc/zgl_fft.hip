@@ -27,6 +27,8 @@
 // variant that computes two outputs per work item.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -588,6 +590,7 @@ template <class R, class G>
 void dispatch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss, int nb,
                       hipStream_t s) {
     switch (L.tile ? L.tile : L.t->fft.tile[sizeof(R) == 4]) {
+        case 32: launch_forward<32, R>(L, grid, spec, ss, nb, s); break;
         case 16: launch_forward<16, R>(L, grid, spec, ss, nb, s); break;
         case 8: launch_forward<8, R>(L, grid, spec, ss, nb, s); break;
         case 4: launch_forward<4, R>(L, grid, spec, ss, nb, s); break;
@@ -600,6 +603,7 @@ template <class R, class G>
 void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid, int nb,
                        hipStream_t s) {
     switch (L.tile ? L.tile : L.t->fft.tile[sizeof(R) == 4]) {
+        case 32: launch_backward<32, R>(L, spec, ss, grid, nb, s); break;
         case 16: launch_backward<16, R>(L, spec, ss, grid, nb, s); break;
         case 8: launch_backward<8, R>(L, spec, ss, grid, nb, s); break;
         case 4: launch_backward<4, R>(L, spec, ss, grid, nb, s); break;
@@ -608,20 +612,25 @@ void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> gr
     }
 }
 
+// Whether T column pairs per workgroup fit the LDS and the per-thread item
+// budget of the LDS-staged large-prime pass.
+bool tile_fits(int n, const int* rad, int npass, int T, size_t csize) {
+    const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
+    if (lds_bytes(n, npass, T, csize) > limit) return false;
+    for (int k = 0; k < npass; ++k)
+        if (rad[k] > kMaxRegRadix) {
+            const long items = (long)(n / rad[k]) * T * ((rad[k] - 1) / 2 + 1);
+            if ((items + kThreads - 1) / kThreads > kMaxBigItems) return false;
+        }
+    return true;
+}
+
 int choose_tile(int n, const std::vector<int>& rad, size_t csize) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     const int npass = (int)rad.size();
     int T = 16;
-    for (; T > 1; T /= 2) {
-        if (lds_bytes(n, npass, T, csize) > limit) continue;
-        bool ok = true;
-        for (int r : rad)
-            if (r > kMaxRegRadix) {
-                const long items = (long)(n / r) * T * ((r - 1) / 2 + 1);
-                if ((items + kThreads - 1) / kThreads > kMaxBigItems) ok = false;
-            }
-        if (ok) break;
-    }
+    for (; T > 1; T /= 2)
+        if (tile_fits(n, rad.data(), npass, T, csize)) break;
     if (lds_bytes(n, npass, T, csize) > limit)
         throw Error{ZGL_ERROR_NOT_SUPPORTED, "longitude count too large for the LDS"};
     return T;
@@ -638,11 +647,14 @@ C* upload(const std::vector<C>& v, hipStream_t s, const char* what) {
 }  // namespace
 
 std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
+    // the plan's width, half of it, and the double width where it fits
+    // (longer contiguous segments per longitude row, fewer workgroups per CU)
     std::vector<int> out;
-    const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
-    for (int T = p.tile[fp32 ? 1 : 0]; T >= 1 && out.size() < 2; T /= 2)
-        if (lds_bytes(p.desc.n, p.desc.npass, T, fp32 ? sizeof(float2) : sizeof(double2)) <= limit)
-            out.push_back(T);
+    const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
+    const int T0 = p.tile[fp32 ? 1 : 0];
+    if (T0 == 16 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 32, cs)) out.push_back(32);
+    for (int T = T0; T >= 1 && out.size() < 3; T /= 2)
+        if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs)) out.push_back(T);
     return out;
 }
 
@@ -671,6 +683,14 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
     }
     p.tile[0] = choose_tile(n, rad, sizeof(double2));
     p.tile[1] = choose_tile(n, rad, sizeof(float2));
+    // ZGL_FFT_TILE=T forces a width where it fits (comparison and test runs)
+    if (const char* e = std::getenv("ZGL_FFT_TILE")) {
+        const int T = std::atoi(e);
+        if (T == 1 || T == 2 || T == 4 || T == 8 || T == 16 || T == 32) {
+            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(double2))) p.tile[0] = T;
+            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(float2))) p.tile[1] = T;
+        }
+    }
 
     // Input slot of sample i for the DIT order: the last pass combines
     // sub-transforms of samples i = r (mod p_last), stored in block r.
