@@ -1164,6 +1164,12 @@ std::pair<int4*, int> rot_work_for(zgl_plan* p, bool fused, int order, int nb, h
     const auto key = std::make_tuple(fused ? 1 : 0, (int)p->kind, order, nb);
     auto it = p->rot_work.find(key);
     if (it != p->rot_work.end()) return it->second;
+    if (p->rot_work.size() >= 64) {
+        // many distinct batch sizes: start the cache over
+        check_hip(hipDeviceSynchronize(), "work list cache");
+        for (auto& kv : p->rot_work) (void)hipFree(kv.second.first);
+        p->rot_work.clear();
+    }
     const std::vector<int4> w = fused ? rotation_fused_work(p->kind == ZGL_ZERNIKE, order, nb)
                                       : rotation_work(p->kind == ZGL_ZERNIKE, order, nb);
     int4* d = nullptr;
@@ -1272,6 +1278,7 @@ extern "C" zgl_status zgl_wignerd_pi2(zgl_plan* plan, size_t order, double* out,
 }
 
 extern "C" void zgl_euler_angles_from_rotation_matrix(const double* m, double* angles) {
+    if (!m || !angles) return;
     const double r22_sq = m[8] * m[8];
     const double beta_y = std::sqrt((1.0 + r22_sq) * (1.0 - r22_sq));
     angles[0] = std::atan2(m[5], m[2]);

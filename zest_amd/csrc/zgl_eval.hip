@@ -329,6 +329,13 @@ void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s, const EvalTimer& 
     }
     const auto key = std::make_tuple(N, e.nlat, ns);
     auto lw = ec.leg_work.find(key);
+    if (lw == ec.leg_work.end() && ec.leg_work.size() >= 64) {
+        // many distinct point counts: start the cache over
+        check_hip(hipDeviceSynchronize(), "work list cache");
+        for (auto& kv : ec.leg_work) (void)hipFree(kv.second.first);
+        ec.leg_work.clear();
+        lw = ec.leg_work.end();
+    }
     if (lw == ec.leg_work.end()) {
         std::vector<int4> w;
         for (int m = 0; m < N; ++m)
