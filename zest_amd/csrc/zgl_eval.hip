@@ -79,7 +79,7 @@ __global__ void eval_gather_kernel(Inst<const double> c, int n, long ntri, int n
 // accumulator.
 constexpr int kLonMB = 8;                 // 128 longitudes per workgroup
 constexpr int kLonNB = 2;                 // 32 columns per wave, 128 per workgroup
-constexpr int kLonKC = 32;                // m per LDS chunk
+constexpr int kLonKC = 8;                 // m per LDS chunk
 constexpr int kLonRS = 2 * kLonKC + 2;    // LDS row stride (doubles): 16 rows hit distinct banks
 constexpr int kLonRows = 16 * kLonMB;
 constexpr int kLonCols = 4 * 16 * kLonNB;
@@ -141,18 +141,33 @@ __global__ __launch_bounds__(256, 2) void eval_lon_kernel(LonArgs a) {
                 b[c] = *reinterpret_cast<const d2*>(sb + (long)m * a.row_stride + 2L * colv[c]);
         }
     };
-    d2 bn[kLonNB];
-    load(0, bn);
-    for (int kc = 0; kc < a.nm; kc += kLonKC) {
-        __syncthreads();  // the previous chunk's A reads are done
-        for (int e = tid; e < kLonRows * kLonKC; e += 256) {
+    // the (cos, sin) block of a chunk: loaded into registers one chunk ahead,
+    // parked in LDS between two barriers
+    constexpr int kStage = kLonRows * kLonKC / 256;
+    d2 an[kStage];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = tid + 256 * u;
             const int r = e / kLonKC, mm = e - r * kLonKC;
             const int i = i0 + r, m = kc + mm;
-            d2 v = {0.0, 0.0};
-            if (i < a.nlon && m < a.nm) v = *reinterpret_cast<const d2*>(a.trig + ((long)i * a.nm + m) * 2);
-            *reinterpret_cast<d2*>(As + r * kLonRS + 2 * mm) = v;
+            an[u] = d2{0.0, 0.0};
+            if (i < a.nlon && m < a.nm) an[u] = *reinterpret_cast<const d2*>(a.trig + ((long)i * a.nm + m) * 2);
+        }
+    };
+    d2 bn[kLonNB];
+    load(0, bn);
+    fetch(0);
+    for (int kc = 0; kc < a.nm; kc += kLonKC) {
+        __syncthreads();  // the previous chunk's A reads are done
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = tid + 256 * u;
+            const int r = e / kLonKC, mm = e - r * kLonKC;
+            *reinterpret_cast<d2*>(As + r * kLonRS + 2 * mm) = an[u];
         }
         __syncthreads();
+        if (kc + kLonKC < a.nm) fetch(kc + kLonKC);
         const int steps = (min(kLonKC, a.nm - kc) + 3) >> 2;
 #pragma unroll
         for (int s = 0; s < kLonKC / 4; ++s) {
