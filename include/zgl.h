@@ -72,9 +72,12 @@ enum { ZGL_LAYOUT_LONLAT = 0, ZGL_LAYOUT_LATLON = 1 };
 typedef enum zgl_precision {
     ZGL_FP64 = 0,        /* FP64 throughout: the drop-in form, matches zest */
     ZGL_FP32 = 1,        /* FP32 FFT and FP32-MFMA contractions, FP32 accumulation */
-    ZGL_FP32_REFINE = 2  /* ZGL_FP32, plus one step of iterative refinement on the
+    ZGL_FP32_REFINE = 2, /* ZGL_FP32, plus one step of iterative refinement on the
                             forward transform: c += F32(g - B64(c)), the residual
                             formed by an FP64 backward transform */
+    ZGL_BF16X3 = 3       /* ZGL_FP32 with the Zernike Legendre products formed on the
+                            BF16 MFMA from operands split into three BF16 values
+                            (six partial products, FP32 accumulation) */
 } zgl_precision;
 
 typedef enum zgl_mem {
@@ -267,11 +270,14 @@ void zgl_euler_angles_from_rotation_matrix(const double *matrix, double *angles)
  * While enabled (also: environment ZGL_AUTOTUNE=1 at plan creation), the
  * first call of each stage at a new order and arithmetic times the candidate
  * tile shapes on that call's own operands -- the FFT's column pairs per
- * workgroup, the streaming contractions' instances per workgroup -- and keeps
- * the fastest for later calls. Results do not depend on the choice.
+ * workgroup, the streaming contractions' instances per workgroup and, for the
+ * FP32 Legendre stages, FP32 MFMA products against split-BF16x3 ones -- and
+ * keeps the fastest for later calls. Results depend on the choice only
+ * within the FP32 rounding of the mixed-precision transforms.
  * zgl_plan_tuning lists the choices made: 5 ints per entry (order, bytes per
  * real of the arithmetic, bytes per grid value, stage index as in
- * zgl_profile_stats, choice). */
+ * zgl_profile_stats, choice); a Legendre choice of 16 + g is the BF16x3
+ * variant with g instances per workgroup. */
 zgl_status zgl_plan_autotune(zgl_plan *plan, int enable);
 zgl_status zgl_plan_tuning(const zgl_plan *plan, int *entries, int max_entries,
                            int *num_entries);

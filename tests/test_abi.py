@@ -38,6 +38,7 @@ def test_invalid_arguments_fail_before_device_work():
     assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, 0, 0, 4, 0) == 1
     assert lib.zgl_plan_create(ctypes.byref(h), 1, 1, 0, -1, 3, 4, 0) == 1
     assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, -1, 0, 4, 9) == 4
+    assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, -1, 0, 4, 4) == 4
     assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, -1, 0, 1 << 20, 0) == 1
     assert not h.value
     assert lib.zgl_forward(None, None, 4, None, 4, 1, 0, 0, 0, None) == 1

@@ -1,7 +1,10 @@
 """The autotuner (zgl_plan_autotune): the first call at an order times the
-candidate FFT tile widths and instances per workgroup and keeps the fastest.
-The choice must not change any result: each instance's arithmetic is the
-same whatever the grouping, so tuned and untuned plans agree bit for bit."""
+candidate FFT tile widths and instances per workgroup (and, for the FP32
+Legendre stages, FP32 against split-BF16x3 MFMA products) and keeps the
+fastest. Tile widths and groupings must not change any result: each
+instance's arithmetic is the same whatever the grouping, so tuned and untuned
+plans agree bit for bit unless a BF16x3 product was chosen, which rounds the
+same sums differently (FP32-level agreement)."""
 import numpy as np
 import pytest
 
@@ -29,11 +32,16 @@ def test_tuned_results_identical(device, precision):
             torch.cuda.synchronize()
             res.append((dg.cpu().numpy(), co.cpu().numpy()))
         out[tuned] = (res, plan.tuning())
-    for (g0, c0), (g1, c1) in zip(out[False][0], out[True][0]):
-        assert np.array_equal(g0, g1) and np.array_equal(c0, c1)
     assert out[False][1] == []
     tuning = out[True][1]
     print("tuning:", tuning)
+    x3 = any(st.startswith("legendre") and choice >= 16 for _, _, _, st, choice in tuning)
+    for (g0, c0), (g1, c1) in zip(out[False][0], out[True][0]):
+        if x3:
+            assert np.abs(g0 - g1).max() <= 5e-6 * np.abs(g0).max()
+            assert np.abs(c0 - c1).max() <= 5e-6 * np.abs(c0).max()
+        else:
+            assert np.array_equal(g0, g1) and np.array_equal(c0, c1)
     stages = {(o, st) for o, _, _, st, _ in tuning}
     for order in (24, 64):
         for st in ("fft_forward", "legendre_forward", "radial_forward", "radial_backward",
@@ -41,7 +49,12 @@ def test_tuned_results_identical(device, precision):
             assert (order, st) in stages
     for o, rb, gb, st, choice in tuning:
         assert rb == (8 if precision == 0 else 4) and gb == 8
-        assert choice in ((32, 16, 8, 4, 2, 1) if st.startswith("fft") else (1, 2, 4, 8))
+        if st.startswith("fft"):
+            assert choice in (32, 16, 8, 4, 2, 1)
+        elif st.startswith("legendre") and precision == 1:
+            assert choice in (1, 2, 4, 8, 17, 18, 20, 24)
+        else:
+            assert choice in (1, 2, 4, 8)
 
 
 @pytest.mark.parametrize("kind,layout", [("zernike", 0), ("sh", 0), ("sh", 1)])

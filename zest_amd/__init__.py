@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libzgl.so")
 
 ZERNIKE, SH = 0, 1
-FP64, FP32, FP32_REFINE = 0, 1, 2
+FP64, FP32, FP32_REFINE, BF16X3 = 0, 1, 2, 3
 MEM_DEVICE, MEM_HOST = 0, 1
 
 _STATUS = {0: "success", 1: "invalid value", 2: "out of memory", 3: "HIP error", 4: "not supported"}
@@ -187,12 +187,15 @@ class Plan:
         self.kind = kind
         self.layout = layout if kind == SH else 0
         self._h = ctypes.c_void_p()
+        # held by the plan: module globals may already be cleared when a plan
+        # is collected at interpreter exit
+        self._lib = LIB
         _check(LIB.zgl_plan_create(ctypes.byref(self._h), kind, zernike_norm, sh_norm,
                                    sh_phase, layout, order, precision))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            LIB.zgl_plan_destroy(self._h)
+            self._lib.zgl_plan_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -308,6 +308,22 @@ int tuned_choice(zgl_plan* p, size_t order, int rbytes, int gbytes, int stage, h
 }
 
 const std::vector<int> kGroupCandidates = {1, 2, 4, 8};
+// FP32 Legendre stages: the FP32 MFMA products against the BF16x3 ones
+// (kGroupX3), each at every group size.
+const std::vector<int> kLegendreCandidates32 = {1, 2, 4, 8, 1 | kGroupX3, 2 | kGroupX3, 4 | kGroupX3, 8 | kGroupX3};
+const std::vector<int> kLegendreCandidatesX3 = {1 | kGroupX3, 2 | kGroupX3, 4 | kGroupX3, 8 | kGroupX3};
+
+// ZGL_BF16X3 plans keep the BF16x3 products (tuning only the grouping)
+template <class T>
+const std::vector<int>& legendre_candidates(const zgl_plan* p) {
+    if (sizeof(T) == 8) return kGroupCandidates;
+    return p->precision == ZGL_BF16X3 ? kLegendreCandidatesX3 : kLegendreCandidates32;
+}
+
+template <class T>
+int legendre_math(const zgl_plan* p) {
+    return sizeof(T) == 4 && p->precision == ZGL_BF16X3 ? kGroupX3 : 0;
+}
 
 // T: arithmetic (double or float); G: grid element type in memory.
 // adjoint: the transpose of the backward transform (no quadrature weights,
@@ -342,11 +358,12 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
         if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, kappa, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride,
                          coeffs.from(b0), nb};
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_FWD, s, kGroupCandidates, [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_FWD, s, legendre_candidates<T>(p), [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_legendre_forward<T>(b, s);
             });
+            a.group |= legendre_math<T>(p);
             timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
             a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_FWD, s, kGroupCandidates, [&](int c) {
                 ZtGemmArgs b = a;
@@ -402,11 +419,12 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
                 zt_radial_backward<T>(b, s);
             });
             timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward<T>(a, s); });
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_BWD, s, kGroupCandidates, [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_BWD, s, legendre_candidates<T>(p), [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_legendre_backward<T>(b, s);
             });
+            a.group |= legendre_math<T>(p);
             timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward<T>(a, s); });
         } else {
             StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, cin, nb};
@@ -431,7 +449,7 @@ void forward_any(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
         if constexpr (std::is_same_v<G, double>) run_forward<double, G>(p, sc, grid, N, coeffs, nt, batch, s);
         return;
     }
-    if (p->precision == ZGL_FP32) {
+    if (p->precision == ZGL_FP32 || p->precision == ZGL_BF16X3) {
         run_forward<float, G>(p, sc, grid, N, coeffs, nt, batch, s);
         return;
     }
@@ -518,7 +536,8 @@ zgl_status zgl_plan_create(zgl_plan** plan, zgl_kind kind, int zernike_norm, int
     *plan = nullptr;
     zgl_status st = check_conventions(kind, zernike_norm, sh_norm, sh_phase, layout);
     if (st != ZGL_SUCCESS) return st;
-    if (precision != ZGL_FP64 && precision != ZGL_FP32 && precision != ZGL_FP32_REFINE)
+    if (precision != ZGL_FP64 && precision != ZGL_FP32 && precision != ZGL_FP32_REFINE &&
+        precision != ZGL_BF16X3)
         return fail(ZGL_ERROR_NOT_SUPPORTED, "unsupported precision");
     if (order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "order exceeds the supported maximum");
     return guarded([&]() -> zgl_status {
@@ -746,7 +765,7 @@ bool reserve(zgl_plan* p, Scratch& sc, size_t N, size_t count) {
     const Geom& g = tables_for(p, N).g;
     const Scratch before = sc;
     for (size_t csize : {sizeof(double2), sizeof(float2)}) {
-        if (csize == sizeof(double2) && p->precision == ZGL_FP32) continue;
+        if (csize == sizeof(double2) && (p->precision == ZGL_FP32 || p->precision == ZGL_BF16X3)) continue;
         if (csize == sizeof(float2) && p->precision == ZGL_FP64) continue;
         const Chunking ch = chunking(p, g, count, csize);
         ensure_bytes(sc.spec, sc.spec_cap, (size_t)ch.spec_stride * ch.nb * csize);

@@ -169,8 +169,10 @@ struct ZtGemmArgs {
     void* flm; long flm_stride;
     Inst<double> coeffs;    // strides in doubles
     int nb;
-    int group = 0;          // instances per workgroup of the streaming kernels (0: default)
+    int group = 0;          // instances per workgroup of the streaming kernels (0: default);
+                            // Legendre stages: | kGroupX3 for the BF16x3 products (FP32)
 };
+constexpr int kGroupX3 = 16;
 template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);

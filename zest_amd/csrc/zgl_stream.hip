@@ -28,6 +28,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
 
 #include "zgl_internal.hpp"
 #include "zgl_plm.hpp"
@@ -347,8 +350,183 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
     }
 }
 
+// ---------------------------------------------------------------- BF16x3
+// The FP32 tile loop with the products formed on the BF16 MFMA: each FP32
+// operand is split as x = hi + mid + lo (three BF16 values, round to nearest
+// even, ~24 bits together) and a*b is accumulated in FP32 from the six
+// leading partial products (lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi,
+// smallest first), dropping terms below 2^-24 relative. MFMA
+// v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + i] and
+// B[4 (l / 16) + i][l % 16], i < 4, and D like the FP32 16x16x4 MFMA, so a
+// lane streams the same 16-byte column loads for four consecutive k (a step
+// covers 16 rows) and stores exactly as the FP32 loop does.
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_rne(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_value(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ inline void split3(float x, short& h, short& m, short& l) {
+    const unsigned short a = bf16_rne(x);
+    const float r1 = x - bf16_value(a);
+    const unsigned short b = bf16_rne(r1);
+    const float r2 = r1 - bf16_value(b);
+    h = (short)a;
+    m = (short)b;
+    l = (short)bf16_rne(r2);
+}
+
+constexpr int kKC3 = 1;  // 16-row steps per register chunk
+
+template <int NSA, bool CENTER, class S>
+__device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::Prob& P, const float* As, int AS,
+                                                int m0, int rows, int g0, int gn, int wave, int lane) {
+    using M = SM<float>;
+    using V = f4;
+    constexpr int VW = 4;
+    constexpr int TNW = 16 * VW;
+    const int j = lane & 15, g = lane >> 4;
+    const int K = P.K, steps = (K + 15) >> 4;
+    const int N = P.N;
+    const int ntile = (N + TNW - 1) / TNW;
+    const int tiles = gn * ntile;
+    const int ntw = tiles > wave ? (tiles - wave + 3) >> 2 : 0;
+    const int nchunks = max(1, (steps + kKC3 - 1) / kKC3);
+    const int U = ntw * nchunks;
+
+    f4 acc[NSA][VW];
+#pragma unroll
+    for (int q = 0; q < NSA; ++q)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) acc[q][v] = f4{0, 0, 0, 0};
+    V cpart = {};
+    const float* Ac = As + (long)st.center_lds_row() * AS;
+
+    auto unit = [&](int u, int& inst, int& col, int& ch) {
+        const int tw = u / nchunks;
+        ch = u - tw * nchunks;
+        const int t = wave + 4 * tw;
+        const int gi = t / ntile;
+        inst = g0 + gi;
+        col = (t - gi * ntile) * TNW + VW * j;
+    };
+    // rows k = 16 (chunk step) + 4 g + r, r < 4
+    auto load = [&](V (&buf)[kKC3][4], int u) {
+        int inst, col, ch;
+        unit(u, inst, col, ch);
+        const bool colok = col < N;
+        const float* bb = st.b_base(inst);
+#pragma unroll
+        for (int s = 0; s < kKC3; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * (ch * kKC3 + s) + 4 * g + r;
+                V x = {};
+                if (colok && k < K) x = *reinterpret_cast<const V*>(bb + (st.b_off(P, k) + col));
+                buf[s][r] = x;
+            }
+    };
+    auto compute = [&](const V (&buf)[kKC3][4], int u) {
+        int inst, col, ch;
+        unit(u, inst, col, ch);
+#pragma unroll
+        for (int s = 0; s < kKC3; ++s) {
+            const int ks = ch * kKC3 + s;
+            if (ks < steps) {
+                s4 bh[VW], bm[VW], bl[VW];
+#pragma unroll
+                for (int v = 0; v < VW; ++v)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        short h, m, l;
+                        split3(buf[s][r][v], h, m, l);
+                        bh[v][r] = h;
+                        bm[v][r] = m;
+                        bl[v][r] = l;
+                    }
+#pragma unroll
+                for (int q = 0; q < NSA; ++q) {
+                    s4 ah, am, al;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        short h, m, l;
+                        split3(As[(16 * q + j) * AS + 16 * ks + 4 * g + r], h, m, l);
+                        ah[r] = h;
+                        am[r] = m;
+                        al[r] = l;
+                    }
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) {
+                        f4 c = acc[q][v];
+                        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[v], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm[v], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[v], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh[v], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm[v], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[v], c, 0, 0, 0);
+                        acc[q][v] = c;
+                    }
+                }
+                if constexpr (CENTER) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float a = Ac[16 * ks + 4 * g + r];
+#pragma unroll
+                        for (int v = 0; v < VW; ++v) cpart[v] += a * buf[s][r][v];
+                    }
+                }
+            }
+        }
+        if (ch != nchunks - 1) return;
+        const bool colok = col < N;
+        float* cb = st.c_base(inst);
+#pragma unroll
+        for (int q = 0; q < NSA; ++q) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * q + M::row(lane, r);
+                if (colok && i < rows) {
+                    V o;
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) o[v] = acc[q][v][r];
+                    V* dst = reinterpret_cast<V*>(cb + (st.c_off(P, m0 + i) + col));
+                    if constexpr (S::kAccum)
+                        if (st.accum) o += *dst;
+                    *dst = o;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < VW; ++v) acc[q][v] = f4{0, 0, 0, 0};
+        }
+        if constexpr (CENTER) {
+#pragma unroll
+            for (int v = 0; v < VW; ++v) {
+                float x = cpart[v];
+                x += __shfl_xor(x, 16);
+                x += __shfl_xor(x, 32);
+                cpart[v] = x;
+            }
+            if (colok && g == 0) *reinterpret_cast<V*>(cb + (st.c_off(P, -P.hoff) + col)) = cpart;
+            cpart = V{};
+        }
+    };
+
+    V b0[kKC3][4], b1[kKC3][4];
+    if (U > 0) load(b0, 0);
+    for (int u = 0; u < U; u += 2) {
+        if (u + 1 < U) load(b1, u + 1);
+        compute(b0, u);
+        if (u + 1 >= U) break;
+        if (u + 2 < U) load(b0, u + 2);
+        compute(b1, u + 1);
+    }
+}
+
 // PLM: the P block is recomputed in the workgroup (no table)
-template <class S, int NS, bool PLM>
+// X3: the BF16x3 tile loop (FP32 stages only)
+template <class S, int NS, bool PLM, bool X3 = false>
 __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb) {
     using T = typename S::T;
     constexpr int TM = 16 * NS;
@@ -366,7 +544,7 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int K = P.K, KP = (K + 3) & ~3;
+    const int K = P.K, KP = X3 ? (K + 15) & ~15 : (K + 3) & ~3;
     const int AS = st.as;
 
     // A tile: TM rows x KP, zero padded, As[i * AS + k]; the equator row
@@ -395,9 +573,14 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
 
     const int nstrips = (rows + 15) >> 4;
     static_assert(NS >= 1 && NS <= 4, "strip count");
-#define ZGL_TILES(n)                                                                          \
-    (center ? stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane)      \
-            : stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane))
+#define ZGL_TILES(n)                                                                                  \
+    if constexpr (X3) {                                                                              \
+        if (center) stream_tiles_x3<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane);   \
+        else stream_tiles_x3<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);               \
+    } else {                                                                                         \
+        if (center) stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane);      \
+        else stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);                  \
+    }
     switch (nstrips) {
         case 1: ZGL_TILES(1); break;
         case 2: if constexpr (NS >= 2) ZGL_TILES(2); break;
@@ -425,22 +608,31 @@ int group_of(int requested, int nb) {
 }
 
 template <int NS, class S>
-bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, int group, hipStream_t s) {
+bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, int group, hipStream_t s,
+                   bool x3 = false) {
     using T = typename S::T;
-    const int kp = (kmax + 3) & ~3;
+    if constexpr (!std::is_same_v<T, float>) x3 = false;
+    const int kp = x3 ? (kmax + 15) & ~15 : (kmax + 3) & ~3;
     st.as = kp + 1;  // odd stride: the 16 rows of an A fragment fall in distinct banks
     const size_t lds = (size_t)(16 * NS + (S::kCenter ? 1 : 0)) * st.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (nwork == 0 || nb <= 0) return true;
     st.G = group_of(group, nb);
     dim3 grid(nwork, (nb + st.G - 1) / st.G);
-    if (st.ptab) {
-        allow_lds(&stream_kernel<S, NS, false>, lds);
-        stream_kernel<S, NS, false><<<grid, kThreads, lds, s>>>(st, work, nb);
-    } else {
-        allow_lds(&stream_kernel<S, NS, true>, lds);
-        stream_kernel<S, NS, true><<<grid, kThreads, lds, s>>>(st, work, nb);
+    auto go = [&](auto kernel) {
+        allow_lds(kernel, lds);
+        kernel<<<grid, kThreads, lds, s>>>(st, work, nb);
+    };
+    if constexpr (std::is_same_v<T, float>) {
+        if (x3) {
+            if (st.ptab) go(&stream_kernel<S, NS, false, true>);
+            else go(&stream_kernel<S, NS, true, true>);
+            check_hip(hipGetLastError(), "stream launch");
+            return true;
+        }
     }
+    if (st.ptab) go(&stream_kernel<S, NS, false>);
+    else go(&stream_kernel<S, NS, true>);
     check_hip(hipGetLastError(), "stream launch");
     return true;
 }
@@ -747,13 +939,24 @@ bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     return true;
 }
 
+// ZtGemmArgs::group of the Legendre stages: instances per workgroup in the
+// low four bits, kGroupX3 set for the BF16x3 products (FP32 stages);
+// ZGL_LEGENDRE_MATH=bf16x3 selects them when the caller leaves it unset.
+static bool legendre_x3(int group) {
+    if (group & kGroupX3) return true;
+    if (group != 0) return false;
+    const char* e = std::getenv("ZGL_LEGENDRE_MATH");
+    return e && std::strcmp(e, "bf16x3") == 0;
+}
+
 template <class T>
 bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
                  a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0,
                  a.t->zh, a.t->plm_qm, a.t->plm_phase};
-    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, a.group, s);
+    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, a.group & 15, s,
+                                         legendre_x3(a.group));
 }
 
 template <class T>
@@ -763,7 +966,7 @@ bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
                  a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0, g.nth,
                  a.t->zh, a.t->plm_qm, a.t->plm_phase};
     return launch_stream<kNsStreamBwd>(st, plm_count(g.order, 0, 0), a.t->work_sb, a.t->n_sb,
-                                       a.nb, a.group, s);
+                                       a.nb, a.group & 15, s, legendre_x3(a.group));
 }
 
 bool stream_legendre_fits(int order, int jh) {
