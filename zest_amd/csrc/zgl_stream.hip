@@ -758,12 +758,16 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdA
     constexpr int KI = 4 * RadCfg<T>::KV;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* As = reinterpret_cast<T*>(smem_raw);
-    const int4 wk = work[blockIdx.x];
+    // one-dimensional grid, instance group fastest: every group starts the
+    // costliest items (the list is sorted by cost) first
+    const int ng = (nb + a.G - 1) / a.G;
+    const int wi = blockIdx.x / ng;
+    const int4 wk = work[wi];
     const int l = wk.x, i0 = wk.y;
     const int M = l < a.nt ? (a.nt - l + 1) / 2 : 0;
     const int rows = min(TM, M - i0);
     if (rows <= 0) return;
-    const int g0 = blockIdx.y * a.G;
+    const int g0 = (blockIdx.x - wi * ng) * a.G;
     const int gn = min(a.G, nb - g0);
     if (gn <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -804,11 +808,16 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdA
     constexpr int VW = M::VW, TNW = 16 * VW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* Rs = reinterpret_cast<T*>(smem_raw);
-    const int l = blockIdx.x;
-    if (l >= a.nt) return;
+    // one-dimensional grid, instance group fastest; the rank r enumerates l
+    // from the middle outwards (c, c-1, c+1, ..., c = nt/2), roughly by
+    // decreasing cost (l+1)(nt-l+1)/2, and covers [0, nt) exactly once
+    const int ng = (nb + a.G - 1) / a.G;
+    const int r = blockIdx.x / ng, d = (r + 1) >> 1;
+    const int l = (r & 1) ? a.nt / 2 - d : a.nt / 2 + d;
+    if (l < 0 || l >= a.nt) return;
     const int K = (a.nt - l + 1) / 2;
     const int steps = (K + 3) >> 2;
-    const int g0 = blockIdx.y * a.G;
+    const int g0 = (blockIdx.x - r * ng) * a.G;
     const int gn = min(a.G, nb - g0);
     if (gn <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -911,7 +920,7 @@ bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     if (lds > kMaxStreamLds) return false;
     if (a.t->n_rf == 0 || a.nb <= 0) return true;
     r.G = group_of(a.group, a.nb);
-    dim3 grid(a.t->n_rf, (a.nb + r.G - 1) / r.G);
+    dim3 grid((unsigned)a.t->n_rf * (unsigned)((a.nb + r.G - 1) / r.G));
     allow_lds(&rad_fwd_kernel<T, kNsRadial>, lds);
     rad_fwd_kernel<T, kNsRadial><<<grid, kThreads, lds, s>>>(r, a.t->work_rf, a.nb);
     check_hip(hipGetLastError(), "radial forward launch");
@@ -932,7 +941,7 @@ bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     if (a.nt == 0 || a.nb <= 0) return true;
     RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
                     a.nt, g.ncol, group_of(a.group, a.nb), rs};
-    dim3 grid(a.nt, (a.nb + r.G - 1) / r.G);
+    dim3 grid((unsigned)a.nt * (unsigned)((a.nb + r.G - 1) / r.G));
     allow_lds(&rad_bwd_kernel<T>, lds);
     rad_bwd_kernel<T><<<grid, kThreads, lds, s>>>(r, a.nb);
     check_hip(hipGetLastError(), "radial backward launch");

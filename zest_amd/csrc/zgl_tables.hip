@@ -422,6 +422,13 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
             for (int m0 = 0; m0 < ncount; m0 += tm_r) rf.push_back(make_int4(l, m0, 0, 0));
             for (int m0 = 0; m0 < 2 * (l + 1); m0 += tm_r) rb.push_back(make_int4(l, m0, 0, 0));
         }
+        // radial forward items, costliest first (rows of the tile x (l + 1)
+        // values of m): the short items fill in behind the long ones
+        auto cost = [&](const int4& w) {
+            const int ncount = (order - w.x + 1) / 2;
+            return (long)std::min(tm_r, ncount - w.y) * (w.x + 1);
+        };
+        std::stable_sort(rf.begin(), rf.end(), [&](const int4& a, const int4& b) { return cost(a) > cost(b); });
     }
     t.work_lf = upload_work(lf, s); t.n_lf = (int)lf.size();
     t.work_lb = upload_work(lb, s); t.n_lb = (int)lb.size();
