@@ -194,9 +194,10 @@ class Plan:
                                    sh_phase, layout, order, precision))
 
     def close(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
-            self._lib.zgl_plan_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.zgl_plan_destroy(h)
+        self._h = None
 
     def __del__(self):
         self.close()

@@ -58,7 +58,7 @@ template <class C> __device__ inline C cmul(C a, C b) {
 // In-register p-point DFT, y_s = sum_r x_r e^{DIR 2 pi i r s / p}; `twl` holds
 // e^{-2 pi i t / n} and p divides n.
 template <int P, int DIR, class C>
-__device__ inline void dft_small(C (&x)[P], const C* twl, int n) {
+__device__ __forceinline__ void dft_small(C (&x)[P], const C* twl, int n) {
     using R = decltype(C{}.x);
     if constexpr (P == 2) {
         const C a = x[0], b = x[1];
@@ -139,7 +139,7 @@ __device__ inline C twiddled(C v, const C* tw, int pos) {
 }
 
 template <int P, int T, int DIR, class C>
-__device__ void pass_reg(C* data, const C* twl, const C* tw, int n, int L) {
+__device__ __forceinline__ void pass_reg(C* data, const C* twl, const C* tw, int n, int L) {
     const int nbf = n / P;
     for (int it = threadIdx.x; it < nbf * T; it += kThreads) {
         const int t = it % T;
@@ -163,7 +163,7 @@ __device__ void pass_reg(C* data, const C* twl, const C* tw, int n, int L) {
 // y_s and y_{P-s}; results are held in registers across a barrier because the
 // DFT reads every input slot of its butterfly.
 template <int T, int DIR, class C>
-__device__ void pass_big(C* data, const C* twl, const C* tw, int n, int P, int L) {
+__device__ __forceinline__ void pass_big(C* data, const C* twl, const C* tw, int n, int P, int L) {
     using R = decltype(C{}.x);
     const int H = (P - 1) / 2;
     const int nbf = n / P;
@@ -233,8 +233,148 @@ __device__ void pass_big(C* data, const C* twl, const C* tw, int n, int P, int L
     }
 }
 
+// Prime radix 17 < P <= kMaxMfmaRadix on the matrix cores. With H = (P-1)/2,
+// u_r = x_r + x_{P-r} and v_r = x_r - x_{P-r} (inputs twiddled, u_0 = x_0,
+// v_0 = 0), the pass is two real products over r = 0..H,
+//   a_s = sum_r cos(2 pi r s / P) u_r,   b_s = sum_r sin(2 pi r s / P) v_r,
+// for s = 0..H, and y_s = a_s -/+ i b_s, y_{P-s} = a_s +/- i b_s (the same
+// combination as pass_big). Rows s and the contraction index r go to 16x16x4
+// MFMA tiles (cos and sin from the LDS twiddle table, index r*s mod P kept
+// incrementally); the columns are the (butterfly, sequence) pairs of the
+// pass, NC = (n / P) * T. A round computes at most 8 tiles (two per wave) --
+// every M block of up to 8 / MB column blocks -- holds them in registers
+// across a barrier (the pass is in place) and writes them back. Each column's
+// arithmetic is the same whatever T, so results do not depend on the tile
+// width. The pass runs in its own kernel variants (its accumulators would
+// otherwise cost every FFT kernel occupancy).
+constexpr int kMfmaTasks = 2;       // tiles per wave and round
+constexpr int kMaxMfmaRadix = 255;  // MB = ceil((H + 1) / 16) <= 4 * kMfmaTasks
+
+__host__ __device__ inline bool mfma_radix(int P) { return P > kMaxRegRadix && P <= kMaxMfmaRadix; }
+
+template <class R> struct PassMfma;
+template <> struct PassMfma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    __device__ static acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // lane holds D[lane/16 + 4r][lane%16]
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct PassMfma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    __device__ static acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // lane holds D[4(lane/16) + r][lane%16]
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
 template <int T, int DIR, class C>
-__device__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d) {
+__device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, int n, int P, int L) {
+    using R = decltype(C{}.x);
+    using M = PassMfma<R>;
+    using A = typename M::acc;
+    const int H = (P - 1) / 2;
+    const int NC = (n / P) * T;
+    const int MB = (H + 16) >> 4, NB = (NC + 15) >> 4;
+    const int NBR = max(1, 4 * kMfmaTasks / MB);  // column blocks per round
+    const int KS = (H + 4) >> 2;       // k-steps over r = 0..H
+    const int stride = n / P;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+
+    for (int nb0 = 0; nb0 < NB; nb0 += NBR) {
+        const int ntask = MB * min(NBR, NB - nb0);
+        A ar[kMfmaTasks], ai[kMfmaTasks], br[kMfmaTasks], bi[kMfmaTasks];
+        int sA[kMfmaTasks], idx[kMfmaTasks], step[kMfmaTasks], base[kMfmaTasks], tcol[kMfmaTasks];
+        bool cok[kMfmaTasks];
+#pragma unroll
+        for (int q = 0; q < kMfmaTasks; ++q) {
+            ar[q] = ai[q] = br[q] = bi[q] = A{0, 0, 0, 0};
+            const int task = wave + 4 * q;
+            const int mb = task % MB, nb = nb0 + task / MB;
+            // A operand of this lane: row s, contraction index r = 4 kk + g
+            sA[q] = 16 * mb + i;
+            step[q] = (4 * sA[q]) % P;
+            idx[q] = (g * sA[q]) % P;
+            // B operand of this lane: row r = 4 kk + g, column c
+            const int c = 16 * nb + i;
+            cok[q] = task < ntask && c < NC;
+            const int bf = c / T;
+            tcol[q] = c - bf * T;
+            const int bq = bf / L;
+            base[q] = bq * P * L + (bf - bq * L);
+        }
+        for (int kk = 0; kk < KS; ++kk) {
+            const int r = 4 * kk + g;
+#pragma unroll
+            for (int q = 0; q < kMfmaTasks; ++q) {
+                if (wave + 4 * q >= ntask) break;  // uniform in the wave
+                R ca = 0, sa = 0;
+                if (sA[q] <= H && r <= H) {
+                    const C w = twl[idx[q] * stride];
+                    ca = w.x;
+                    sa = -w.y;
+                }
+                idx[q] += step[q];
+                if (idx[q] >= P) idx[q] -= P;
+                C u = mk<C>(0, 0), v = mk<C>(0, 0);
+                if (cok[q] && r <= H) {
+                    const int t = tcol[q];
+                    if (r == 0) {
+                        u = data[base[q] * T + t];
+                    } else {
+                        const int pr = base[q] + r * L, pm = base[q] + (P - r) * L;
+                        const C xr = twiddled<DIR>(data[pr * T + t], tw, pr);
+                        const C xm = twiddled<DIR>(data[pm * T + t], tw, pm);
+                        u = cadd(xr, xm);
+                        v = csub(xr, xm);
+                    }
+                }
+                ar[q] = M::mma(ca, u.x, ar[q]);
+                ai[q] = M::mma(ca, u.y, ai[q]);
+                br[q] = M::mma(sa, v.x, br[q]);
+                bi[q] = M::mma(sa, v.y, bi[q]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kMfmaTasks; ++q) {
+            const int task = wave + 4 * q;
+            if (task >= ntask) break;
+            const int mb = task % MB, nb = nb0 + task / MB;
+            const int c = 16 * nb + i;  // output column of this lane
+            if (c >= NC) continue;
+            const int bf = c / T, t = c - bf * T;
+            const int bq = bf / L;
+            const int b0 = bq * P * L + (bf - bq * L);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int s = 16 * mb + M::row(lane, rr);
+                if (s > H) continue;
+                const C a = mk<C>(ar[q][rr], ai[q][rr]);
+                const C mib = mk<C>(bi[q][rr], -br[q][rr]);  // -i b
+                C ya, yb;
+                if (DIR < 0) {
+                    ya = cadd(a, mib);
+                    yb = csub(a, mib);
+                } else {
+                    ya = csub(a, mib);
+                    yb = cadd(a, mib);
+                }
+                data[(b0 + s * L) * T + t] = ya;
+                if (s > 0) data[(b0 + (P - s) * L) * T + t] = yb;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// MF: the kernel variant with the matrix-core pass for prime radices in
+// (kMaxRegRadix, kMaxMfmaRadix]; without it those radices take pass_big.
+template <int T, int DIR, bool MF, class C>
+__device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d) {
     const int n = d.n;
     for (int ps = 0; ps < d.npass; ++ps) {
         __syncthreads();
@@ -249,7 +389,15 @@ __device__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d
             case 11: pass_reg<11, T, DIR>(data, twl, tw, n, L); break;
             case 13: pass_reg<13, T, DIR>(data, twl, tw, n, L); break;
             case 17: pass_reg<17, T, DIR>(data, twl, tw, n, L); break;
-            default: pass_big<T, DIR>(data, twl, tw, n, P, L); break;
+            default:
+                if constexpr (MF) {
+                    if (mfma_radix(P)) {
+                        pass_mfma<T, DIR>(data, twl, tw, n, P, L);
+                        break;
+                    }
+                }
+                pass_big<T, DIR>(data, twl, tw, n, P, L);
+                break;
         }
     }
     __syncthreads();
@@ -305,7 +453,7 @@ __device__ inline void stage_tables(C*& data, C*& twl, C*& twp, unsigned short*&
     __syncthreads();
 }
 
-template <int T, int LK, class R, class G>
+template <int T, int LK, class R, class G, bool MF>
 __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
@@ -377,7 +525,7 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
         }
     }
 
-    run_passes<T, -1>(data, twl, twp, d);
+    run_passes<T, -1, MF>(data, twl, twp, d);
 
     for (int e = threadIdx.x; e < nm * T; e += kThreads) {
         const int m = e / T, t = e % T;
@@ -408,7 +556,7 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     }
 }
 
-template <int T, int LK, class R, class G>
+template <int T, int LK, class R, class G, bool MF>
 __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
@@ -469,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void fft_backward_kernel(
         }
     }
 
-    run_passes<T, +1>(data, twl, twp, d);
+    run_passes<T, +1, MF>(data, twl, twp, d);
 
     if (LK == GRID_COLS) {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
@@ -556,8 +704,11 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(grid_tiles(pm), nb);
-    auto k = L.layout_kind == GRID_COLS ? fft_forward_kernel<T, GRID_COLS, R, G>
-                                        : fft_forward_kernel<T, GRID_ROWS, R, G>;
+    const bool mf = f.mfma_pass;
+    auto k = L.layout_kind == GRID_COLS ? (mf ? fft_forward_kernel<T, GRID_COLS, R, G, true>
+                                              : fft_forward_kernel<T, GRID_COLS, R, G, false>)
+                                        : (mf ? fft_forward_kernel<T, GRID_ROWS, R, G, true>
+                                              : fft_forward_kernel<T, GRID_ROWS, R, G, false>);
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -575,8 +726,11 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(grid_tiles(pm), nb);
-    auto k = L.layout_kind == GRID_COLS ? fft_backward_kernel<T, GRID_COLS, R, G>
-                                        : fft_backward_kernel<T, GRID_ROWS, R, G>;
+    const bool mf = f.mfma_pass;
+    auto k = L.layout_kind == GRID_COLS ? (mf ? fft_backward_kernel<T, GRID_COLS, R, G, true>
+                                              : fft_backward_kernel<T, GRID_COLS, R, G, false>)
+                                        : (mf ? fft_backward_kernel<T, GRID_ROWS, R, G, true>
+                                              : fft_backward_kernel<T, GRID_ROWS, R, G, false>);
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -614,23 +768,23 @@ void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> gr
 
 // Whether T column pairs per workgroup fit the LDS and the per-thread item
 // budget of the LDS-staged large-prime pass.
-bool tile_fits(int n, const int* rad, int npass, int T, size_t csize) {
+bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     if (lds_bytes(n, npass, T, csize) > limit) return false;
     for (int k = 0; k < npass; ++k)
-        if (rad[k] > kMaxRegRadix) {
+        if (rad[k] > kMaxRegRadix && !(mf && mfma_radix(rad[k]))) {
             const long items = (long)(n / rad[k]) * T * ((rad[k] - 1) / 2 + 1);
             if ((items + kThreads - 1) / kThreads > kMaxBigItems) return false;
         }
     return true;
 }
 
-int choose_tile(int n, const std::vector<int>& rad, size_t csize) {
+int choose_tile(int n, const std::vector<int>& rad, size_t csize, bool mf) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     const int npass = (int)rad.size();
     int T = 16;
     for (; T > 1; T /= 2)
-        if (tile_fits(n, rad.data(), npass, T, csize)) break;
+        if (tile_fits(n, rad.data(), npass, T, csize, mf)) break;
     if (lds_bytes(n, npass, T, csize) > limit)
         throw Error{ZGL_ERROR_NOT_SUPPORTED, "longitude count too large for the LDS"};
     return T;
@@ -652,9 +806,9 @@ std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
     std::vector<int> out;
     const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
     const int T0 = p.tile[fp32 ? 1 : 0];
-    if (T0 == 16 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 32, cs)) out.push_back(32);
+    if (T0 == 16 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 32, cs, p.mfma_pass)) out.push_back(32);
     for (int T = T0; T >= 1 && out.size() < 3; T /= 2)
-        if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs)) out.push_back(T);
+        if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs, p.mfma_pass)) out.push_back(T);
     return out;
 }
 
@@ -681,14 +835,20 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
         p.desc.L[i] = L;
         L *= rad[i];
     }
-    p.tile[0] = choose_tile(n, rad, sizeof(double2));
-    p.tile[1] = choose_tile(n, rad, sizeof(float2));
+    // prime radices in (17, kMaxMfmaRadix] run on the matrix cores unless
+    // ZGL_FFT_MFMA=0 (comparison runs)
+    for (int r : rad) p.mfma_pass |= mfma_radix(r);
+    if (const char* e = std::getenv("ZGL_FFT_MFMA"))
+        if (e[0] == '0') p.mfma_pass = false;
+    const bool mf = p.mfma_pass;
+    p.tile[0] = choose_tile(n, rad, sizeof(double2), mf);
+    p.tile[1] = choose_tile(n, rad, sizeof(float2), mf);
     // ZGL_FFT_TILE=T forces a width where it fits (comparison and test runs)
     if (const char* e = std::getenv("ZGL_FFT_TILE")) {
         const int T = std::atoi(e);
         if (T == 1 || T == 2 || T == 4 || T == 8 || T == 16 || T == 32) {
-            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(double2))) p.tile[0] = T;
-            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(float2))) p.tile[1] = T;
+            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(double2), mf)) p.tile[0] = T;
+            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(float2), mf)) p.tile[1] = T;
         }
     }
 

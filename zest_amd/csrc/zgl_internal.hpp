@@ -42,6 +42,7 @@ struct FftPlan {
     float2* twp32 = nullptr;
     unsigned short* perm = nullptr;  // LDS slot of input sample i
     int tile[2] = {16, 16};          // column pairs per workgroup: FP64, FP32
+    bool mfma_pass = false;          // a prime radix runs on the matrix cores
 };
 
 // Geometry of one order for a plan, both transform kinds.
