@@ -371,15 +371,30 @@ __device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, in
     }
 }
 
-// MF: the kernel variant with the matrix-core pass for prime radices in
-// (kMaxRegRadix, kMaxMfmaRadix]; without it those radices take pass_big.
-template <int T, int DIR, bool MF, class C>
+// MF: 0, the kernel variant without the matrix-core pass (prime radices
+// above kMaxRegRadix take pass_big); 1, with it; 2, the lean variant for
+// plans whose other radices are at most kLeanRadix (no 11-, 13- or 17-point
+// register DFT compiled in, so fewer registers and more waves per SIMD).
+constexpr int kLeanRadix = 7;
+
+template <int T, int DIR, int MF, class C>
 __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d) {
     const int n = d.n;
     for (int ps = 0; ps < d.npass; ++ps) {
         __syncthreads();
         const int P = d.radix[ps], L = d.L[ps];
         const C* tw = ps ? twp + (ps - 1) * n : nullptr;
+        if constexpr (MF == 2) {
+            switch (P) {
+                case 2: pass_reg<2, T, DIR>(data, twl, tw, n, L); break;
+                case 3: pass_reg<3, T, DIR>(data, twl, tw, n, L); break;
+                case 4: pass_reg<4, T, DIR>(data, twl, tw, n, L); break;
+                case 5: pass_reg<5, T, DIR>(data, twl, tw, n, L); break;
+                case 7: pass_reg<7, T, DIR>(data, twl, tw, n, L); break;
+                default: pass_mfma<T, DIR>(data, twl, tw, n, P, L); break;
+            }
+            continue;
+        }
         switch (P) {
             case 2: pass_reg<2, T, DIR>(data, twl, tw, n, L); break;
             case 3: pass_reg<3, T, DIR>(data, twl, tw, n, L); break;
@@ -453,8 +468,14 @@ __device__ inline void stage_tables(C*& data, C*& twl, C*& twp, unsigned short*&
     __syncthreads();
 }
 
-template <int T, int LK, class R, class G, bool MF>
-__global__ __launch_bounds__(kThreads) void fft_forward_kernel(
+// Waves per SIMD asked of the lean variant (3 in FP64, 4 in FP32); none of
+// the others.
+template <class R, int MF>
+constexpr int lean_waves() { return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : 1; }
+
+template <int T, int LK, class R, class G, int MF>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(lean_waves<R, MF>())))
+void fft_forward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
     Inst<const G> grid, typename Cplx<R>::type* __restrict__ spec,
@@ -556,8 +577,9 @@ __global__ __launch_bounds__(kThreads) void fft_forward_kernel(
     }
 }
 
-template <int T, int LK, class R, class G, bool MF>
-__global__ __launch_bounds__(kThreads) void fft_backward_kernel(
+template <int T, int LK, class R, class G, int MF>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(lean_waves<R, MF>())))
+void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
     const typename Cplx<R>::type* __restrict__ spec, long spec_stride, Inst<G> grid, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
@@ -696,6 +718,17 @@ template <class R> const typename Cplx<R>::type* twp_of(const FftPlan& f);
 template <> const double2* twp_of<double>(const FftPlan& f) { return f.twp; }
 template <> const float2* twp_of<float>(const FftPlan& f) { return f.twp32; }
 
+// The lean variant is instantiated for the widths prime lengths use (others
+// fall back to the general one, which computes the same values).
+template <int T>
+constexpr int lean_mf() { return (T == 8 || T == 16) ? 2 : 1; }
+
+template <class K>
+K pick_variant(const FftPlan& f, K none, K general, K lean) {
+    if (!f.mfma_pass) return none;
+    return f.lean ? lean : general;
+}
+
 template <int T, class R, class G>
 void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss, int nb,
                     hipStream_t s) {
@@ -704,11 +737,12 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(grid_tiles(pm), nb);
-    const bool mf = f.mfma_pass;
-    auto k = L.layout_kind == GRID_COLS ? (mf ? fft_forward_kernel<T, GRID_COLS, R, G, true>
-                                              : fft_forward_kernel<T, GRID_COLS, R, G, false>)
-                                        : (mf ? fft_forward_kernel<T, GRID_ROWS, R, G, true>
-                                              : fft_forward_kernel<T, GRID_ROWS, R, G, false>);
+    auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_forward_kernel<T, GRID_COLS, R, G, 0>,
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, 1>,
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>)
+                                        : pick_variant(f, fft_forward_kernel<T, GRID_ROWS, R, G, 0>,
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, 1>,
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>);
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -726,11 +760,12 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(grid_tiles(pm), nb);
-    const bool mf = f.mfma_pass;
-    auto k = L.layout_kind == GRID_COLS ? (mf ? fft_backward_kernel<T, GRID_COLS, R, G, true>
-                                              : fft_backward_kernel<T, GRID_COLS, R, G, false>)
-                                        : (mf ? fft_backward_kernel<T, GRID_ROWS, R, G, true>
-                                              : fft_backward_kernel<T, GRID_ROWS, R, G, false>);
+    auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_backward_kernel<T, GRID_COLS, R, G, 0>,
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, 1>,
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>)
+                                        : pick_variant(f, fft_backward_kernel<T, GRID_ROWS, R, G, 0>,
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, 1>,
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>);
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -840,6 +875,8 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
     for (int r : rad) p.mfma_pass |= mfma_radix(r);
     if (const char* e = std::getenv("ZGL_FFT_MFMA"))
         if (e[0] == '0') p.mfma_pass = false;
+    p.lean = p.mfma_pass;
+    for (int r : rad) p.lean &= r <= kLeanRadix || mfma_radix(r);
     const bool mf = p.mfma_pass;
     p.tile[0] = choose_tile(n, rad, sizeof(double2), mf);
     p.tile[1] = choose_tile(n, rad, sizeof(float2), mf);

@@ -43,6 +43,7 @@ struct FftPlan {
     unsigned short* perm = nullptr;  // LDS slot of input sample i
     int tile[2] = {16, 16};          // column pairs per workgroup: FP64, FP32
     bool mfma_pass = false;          // a prime radix runs on the matrix cores
+    bool lean = false;               // ... and every other radix is at most 7
 };
 
 // Geometry of one order for a plan, both transform kinds.
