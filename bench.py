@@ -446,8 +446,15 @@ def run_sh512(args):
             plan.forward(gw, cw, L, batch=B)
 
         t = _timed_steps(step, args.steps, args.warmup)
+        # one more step with HIP events around every stage launch (outside the timing)
+        plan.profile(True)
+        plan.stage_stats()
+        step()
+        stages = {k: round(v[2] * 1e3, 1) for k, v in plan.stage_stats().items() if v[0]}
+        plan.profile(False)
         out[name] = {"value": round(B * args.steps / t, 2), "ms_per_step": round(t / args.steps * 1e3, 3),
-                     "fwd_max_rel_err_vs_fp64": maxrel, "fwd_rms_rel_err_vs_fp64": rms}
+                     "fwd_max_rel_err_vs_fp64": maxrel, "fwd_rms_rel_err_vs_fp64": rms,
+                     "stage_us_per_step": stages}
     out["value"] = out["fp32+refine"]["value"]
     out["ms_per_step"] = out["fp32+refine"]["ms_per_step"]
     out["dtype"] = "f32"
