@@ -39,6 +39,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of a stage kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (in separate runs, scaled by what the counters read for a 512 MiB device
+# copy: x2.0 and x1.0) on the workload named by the key: (stage, order, instances, form)
+# -> (fetch bytes, write bytes). Reported as roofline.traffic only for that workload;
+# the commands and the full table are in profiles/zernike_n128_b64_kernels.md.
+MEASURED_TRAFFIC = {
+    ("fft_forward", 128, 64, "mixed_grid32"): (1189e6, 1164e6),
+    ("fft_forward", 128, 64, "fp64"): (2323e6, 2317e6),
+    ("fft_backward", 128, 64, "mixed_grid32"): (1168e6, 1221e6),
+    ("fft_backward", 128, 64, "fp64"): (2318e6, 2303e6),
+}
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 
@@ -57,6 +68,12 @@ def stage_work(order: int, real_bytes: int, grid_bytes: int = 8):
         "legendre_forward": ("mfma", legendre_flops), "legendre_backward": ("mfma", legendre_flops),
         "radial_forward": ("mfma", radial_flops), "radial_backward": ("mfma", radial_flops),
     }
+
+
+def _traffic(stage, order, instances, form):
+    """Measured HBM bytes per launch (fetch + write) for the workload, or None."""
+    t = MEASURED_TRAFFIC.get((stage, order, instances, form))
+    return None if t is None else round(t[0] + t[1])
 
 
 def stage_table(stats, order, dtype, grid_bytes=8):
@@ -280,7 +297,8 @@ def run_headline(args):
                    "grid_storage": "f32" if head == "mixed_grid32" else "f64",
                    "forward": forms[head][2] if head in forms else "fp64"},
         "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
-                     "unit": dom["unit"], "frac": dom["frac"], "traffic": None,
+                     "unit": dom["unit"], "frac": dom["frac"],
+                     "traffic": _traffic(dom_name, N, B, head),
                      "kernel": dom_name},
         "accuracy": acc,
         "stages": stages,
