@@ -196,7 +196,8 @@ struct Chunking {
 Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) {
     Chunking c;
     c.spec_stride = (long)g.nm * 2 * g.jh * g.ncol;
-    c.flm_stride = p->kind == ZGL_ZERNIKE ? (long)g.order * (g.order + 1) / 2 * g.ncol : 0;
+    // Zernike: flm; SH: the coefficient-major staging of the Legendre stage
+    c.flm_stride = (long)g.order * (g.order + 1) / 2 * (p->kind == ZGL_ZERNIKE ? g.ncol : 1);
     const size_t per = (size_t)(c.spec_stride + c.flm_stride) * csize;
     size_t budget, cap;
     if (p->kind == ZGL_ZERNIKE) {
@@ -207,8 +208,8 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
         cap = 64;
     } else {
         // Instances form the GEMM column axis: larger chunks reuse the P table more.
-        budget = env_mb("ZGL_SH_SCRATCH_MB", 1024) << 20;
-        cap = 512;
+        budget = env_mb("ZGL_SH_SCRATCH_MB", 4096) << 20;
+        cap = 1024;
     }
     size_t nb = per ? budget / per : cap;
     nb = std::max<size_t>(1, std::min({nb, cap, batch}));
@@ -372,7 +373,7 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
             });
             timed(p, ST_RAD_FWD, nb, s, [&] { zt_radial_forward<T>(a, s); });
         } else {
-            StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, coeffs.from(b0), nb};
+            StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, coeffs.from(b0), nb, sc.flm};
             timed(p, ST_LEG_FWD, nb, s, [&] { st_legendre_forward<T>(a, s); });
         }
     }
@@ -427,7 +428,7 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
             a.group |= legendre_math<T>(p);
             timed(p, ST_LEG_BWD, nb, s, [&] { zt_legendre_backward<T>(a, s); });
         } else {
-            StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, cin, nb};
+            StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, cin, nb, sc.flm};
             timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward<T>(a, s); });
         }
         L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_BWD, s, fft_tiles(t.fft, sizeof(T) == 4),

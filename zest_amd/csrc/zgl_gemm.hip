@@ -62,13 +62,21 @@ constexpr int kThreads = 256;
 // Operand kinds (see the header comment).
 enum { ROWS_OUT = 0, ROWS_K = 1 };
 
+// Workgroup b runs on XCD b % 8 (each XCD has its own L2). The grid deals the
+// column tiles of one work item to one XCD, consecutively: the A block of the
+// item (a P_lm or R_nl table block, re-staged for every column tile) is then
+// read from HBM once per item instead of once per tile.
 template <class S, int NS>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __restrict__ work) {
+__global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __restrict__ work, int nwork,
+                                                        int nct) {
     using T = typename S::T;
     using M = Mfma<T>;
     constexpr int TM = 16 * NS;
-    const int4 wk = work[blockIdx.x];
-    const int n0 = blockIdx.y * TN;
+    const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
+    const int ct = q % nct, wi = (q / nct) * 8 + xcd;
+    if (wi >= nwork) return;
+    const int4 wk = work[wi];
+    const int n0 = ct * TN;
     const typename S::Prob P = st.problem(wk.x, blockIdx.z);
     const int m0 = wk.y;
     if (!P.active || m0 >= P.M || n0 >= P.N) return;
@@ -331,12 +339,16 @@ struct ZtRadialBwd {
 
 // ------------------------------------------------------ spherical harmonics
 // The instances of a chunk form the column axis: columns (instance, re/im).
+// The coefficients go through a coefficient-major staging array
+// stage[tri_idx(l, m)][instance][re/im] (contiguous along the column axis);
+// st_stage_coeffs moves them to and from the per-instance arrays with
+// coalesced accesses on both sides.
 template <class TT>
 struct StLegendreFwd {
     using T = TT;
     static constexpr int A_KIND = ROWS_OUT, B_KIND = ROWS_OUT;
     static constexpr bool B_PAIR = true;
-    const T* ptab; const T* spec; Inst<double> coeffs;
+    const T* ptab; const T* spec; T* stage;
     long spec_stride;  // elements of T
     int order, nt, jh, nb;
     struct Prob { int active, M, N, K, m, par; const T* A; long boff; };
@@ -353,18 +365,18 @@ struct StLegendreFwd {
     __device__ const T* b_row(const Prob& p, int b) const { return spec + (long)b * spec_stride + p.boff; }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         const int l = p.m + p.par + 2 * i;
-        coeffs.at(j >> 1)[tri_idx(l, p.m) * 2 + (j & 1)] = (double)v;
+        stage[(long)tri_idx(l, p.m) * (2 * nb) + j] = v;
     }
 };
 
-// Backward: the coefficient operand is gathered per (instance, re/im) column,
-// so it is staged as rows over the contraction axis with a strided column.
+// Backward: the coefficient operand is the staging array (rows l of the
+// problem's m, columns (instance, re/im)).
 template <class TT>
 struct StLegendreBwd {
     using T = TT;
-    static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_OUT;
-    static constexpr bool B_PAIR = true;
-    const T* ptab; Inst<const double> coeffs; T* spec;
+    static constexpr int A_KIND = ROWS_K, B_KIND = ROWS_K;
+    static constexpr bool B_PAIR = false;
+    const T* ptab; const T* stage; T* spec;
     long spec_stride;
     int order, nt, jh, nb;
     struct Prob { int active, M, N, K, m, par; const T* A; long soff; };
@@ -378,29 +390,77 @@ struct StLegendreBwd {
         return p;
     }
     __device__ const T* a_row(const Prob& p, int k) const { return p.A + (long)k * jh; }
-    // Row b of the B operand: element 2k + ri is c[b][l(k), m][ri]. The rows
-    // are not contiguous in memory, so b_row returns a proxy.
-    struct Gather {
-        const double* base;
-        int m, par;
-        __device__ double operator[](int e) const {
-            const int k = e >> 1;
-            return base[tri_idx(m + par + 2 * k, m) * 2 + (e & 1)];
-        }
-    };
-    __device__ Gather b_row(const Prob& p, int b) const {
-        return Gather{coeffs.at(b), p.m, p.par};
+    __device__ const T* b_row(const Prob& p, int k) const {
+        return stage + (long)tri_idx(p.m + p.par + 2 * k, p.m) * (2 * nb);
     }
     __device__ void c(const Prob& p, int i, int j, T v) const {
         spec[(long)(j >> 1) * spec_stride + p.soff + 2 * i + (j & 1)] = v;
     }
 };
 
+// Coefficients <-> staging: a 32 x 32 tile of (coefficient, instance) pairs
+// through LDS; each instance's array is read or written as 32 consecutive
+// pairs, each staging row likewise. TO_COEFF: stage -> coefficients.
+template <class T, bool TO_COEFF>
+__global__ __launch_bounds__(kThreads) void stage_kernel(T* stage, Inst<double> coeffs, int ntri, int nb) {
+    __shared__ double2 tile[32][33];
+    const int t0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const long row = 2L * nb;
+    if constexpr (TO_COEFF) {
+        for (int i = ty; i < 32; i += kThreads / 32) {
+            const int t = t0 + i, b = b0 + tx;
+            if (t < ntri && b < nb) {
+                const T* sp = stage + (long)t * row + 2 * b;
+                tile[i][tx] = make_double2((double)sp[0], (double)sp[1]);
+            }
+        }
+        __syncthreads();
+        for (int i = ty; i < 32; i += kThreads / 32) {
+            const int b = b0 + i, t = t0 + tx;
+            if (t < ntri && b < nb) {
+                const double2 v = tile[tx][i];
+                double* c = coeffs.at(b) + 2L * t;
+                c[0] = v.x;
+                c[1] = v.y;
+            }
+        }
+    } else {
+        for (int i = ty; i < 32; i += kThreads / 32) {
+            const int b = b0 + i, t = t0 + tx;
+            if (t < ntri && b < nb) {
+                const double* c = coeffs.at(b) + 2L * t;
+                tile[tx][i] = make_double2(c[0], c[1]);
+            }
+        }
+        __syncthreads();
+        for (int i = ty; i < 32; i += kThreads / 32) {
+            const int t = t0 + i, b = b0 + tx;
+            if (t < ntri && b < nb) {
+                const double2 v = tile[i][tx];
+                T* sp = stage + (long)t * row + 2 * b;
+                sp[0] = (T)v.x;
+                sp[1] = (T)v.y;
+            }
+        }
+    }
+}
+
+template <class T, bool TO_COEFF>
+void st_stage_coeffs(T* stage, Inst<double> coeffs, int nt, int nb, hipStream_t s) {
+    const int ntri = nt * (nt + 1) / 2;
+    if (ntri == 0 || nb <= 0) return;
+    dim3 grid((ntri + 31) / 32, (nb + 31) / 32);
+    stage_kernel<T, TO_COEFF><<<grid, kThreads, 0, s>>>(stage, coeffs, ntri, nb);
+    check_hip(hipGetLastError(), "coefficient staging launch");
+}
+
 template <int NS, class S>
 void launch(const S& st, const int4* work, int nwork, int ncols, int nz, hipStream_t s) {
     if (nwork == 0 || ncols <= 0 || nz <= 0) return;
-    dim3 grid(nwork, (ncols + TN - 1) / TN, nz);
-    gemm_kernel<S, NS><<<grid, kThreads, 0, s>>>(st, work);
+    const int nct = (ncols + TN - 1) / TN;
+    dim3 grid((unsigned)(8 * nct) * (unsigned)((nwork + 7) / 8), 1, nz);
+    gemm_kernel<S, NS><<<grid, kThreads, 0, s>>>(st, work, nwork, nct);
     check_hip(hipGetLastError(), "gemm launch");
 }
 
@@ -461,15 +521,17 @@ void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s) {
 template <class T>
 void st_legendre_forward(const StGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
-    StLegendreFwd<T> st{ptab_of<T>(*a.t), (const T*)a.spec, a.coeffs, a.spec_stride * 2,
+    StLegendreFwd<T> st{ptab_of<T>(*a.t), (const T*)a.spec, (T*)a.stage, a.spec_stride * 2,
                         g.order, a.nt, g.jh, a.nb};
     launch<kNsLegendreFwd>(st, a.t->work_lf, a.t->n_lf, 2 * a.nb, 1, s);
+    st_stage_coeffs<T, true>((T*)a.stage, a.coeffs, a.nt, a.nb, s);
 }
 
 template <class T>
 void st_legendre_backward(const StGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
-    StLegendreBwd<T> st{ptab_of<T>(*a.t), a.coeffs, (T*)a.spec, a.spec_stride * 2,
+    st_stage_coeffs<T, false>((T*)a.stage, a.coeffs, a.nt, a.nb, s);
+    StLegendreBwd<T> st{ptab_of<T>(*a.t), (const T*)a.stage, (T*)a.spec, a.spec_stride * 2,
                         g.order, a.nt, g.jh, a.nb};
     launch<kNsLegendreBwdSt>(st, a.t->work_lb, a.t->n_lb, 2 * a.nb, 1, s);
 }

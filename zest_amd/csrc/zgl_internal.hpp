@@ -199,6 +199,7 @@ struct StGemmArgs {
     void* spec; long spec_stride;
     Inst<double> coeffs;    // strides in doubles
     int nb;
+    void* stage;            // coefficient-major staging, nt(nt+1)/2 x nb complex
 };
 template <class T> void st_legendre_forward(const StGemmArgs& a, hipStream_t s);
 template <class T> void st_legendre_backward(const StGemmArgs& a, hipStream_t s);
