@@ -160,7 +160,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy phase (profiling runs): the mixed forms then use the "
                          "plain FP32 forward and no accuracy is reported")
-    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad", "eval", "rotate"],
+    ap.add_argument("--config", choices=["zernike128", "ragged", "sh512", "grad", "eval", "rotate", "spectra"],
                     default="zernike128",
                     help="zernike128: the headline (64 instances at order 128); ragged: 10,000 "
                          "instances of orders U{8..96}; sh512: 4096 SH instances at L=512, "
@@ -168,7 +168,8 @@ def main():
                          "eval: grid evaluation of 64 order-128 Zernike expansions on "
                          "256 x 128 x 128 points (and 64 SH expansions, L=512, on 1024 x 512); "
                          "rotate: ZYZ rotations of 64 order-128 Zernike expansions (and 64 SH "
-                         "expansions at L=512)")
+                         "expansions at L=512); spectra: power spectra of 512 order-128 Zernike "
+                         "expansions (and 512 SH expansions at L=512)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the transforms are "
@@ -176,7 +177,7 @@ def main():
               "share of instances; no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
     {"zernike128": run_headline, "ragged": run_ragged, "sh512": run_sh512,
-     "grad": run_grad, "eval": run_eval, "rotate": run_rotate}[args.config](args)
+     "grad": run_grad, "eval": run_eval, "rotate": run_rotate, "spectra": run_spectra}[args.config](args)
 
 
 def run_headline(args):
@@ -687,6 +688,113 @@ def rotation_flops(zt: bool, order: int, lmax: int = 255) -> int:
         blocks = (order - l + 1) // 2 if zt else 1
         total += blocks * 2 * 2 * (l + 1) ** 2
     return total
+
+
+def run_spectra(args):
+    """zt::power_spectrum of 512 Zernike expansions at order 128 (coefficients
+    N(0,1)/(1+n), torch generator seed 7; 2.9 MB each), one zgl_power_spectrum
+    call for all of them per step, inputs resident in HBM; beside it the SH
+    power spectra of 512 expansions at L = 512. The kernel reads every
+    coefficient once and writes one value per (n, l): HBM-bound; the roofline
+    is its algorithmic bytes per launch over the launch time (HIP events on
+    the launch stream). The CPU baseline runs zest's zt::power_spectrum
+    (oracle/_ref) on the host cores over a bounded sample."""
+    import numpy as np
+    import torch
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    dev = torch.device("cuda:0")
+    N, B = 128, 512
+    n, _, m = synth.zernike_nlm(N)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    c = torch.randn((B, n.size, 2), generator=gen, dtype=torch.float64, device=dev)
+    c *= torch.from_numpy(1.0 / (1.0 + n)).to(dev)[None, :, None]
+    c[:, torch.from_numpy(m == 0).to(dev), 1] = 0.0
+    plan = zest_amd.Plan(zest_amd.ZERNIKE, 4)
+    nent = (N + 1) ** 2 // 4
+    out = torch.empty((B, nent), dtype=torch.float64, device=dev)
+    plan.power_spectrum(c, N, out, batch=B)
+    torch.cuda.synchronize()
+    host = c[:2].cpu().numpy()
+    got = out[:2].cpu().numpy()
+    worst = max(float(np.abs(got[b] - oracle.zt_power_spectrum(host[b], N)).max()
+                      / np.abs(oracle.zt_power_spectrum(host[b], N)).max()) for b in range(2))
+
+    def step():
+        plan.power_spectrum(c, N, out, batch=B)
+
+    t = _timed_steps(step, args.steps, args.warmup)
+    plan.profile(True)
+    plan.stage_stats()
+    for _ in range(3):
+        step()
+    launches, inst, ms = plan.stage_stats()["power_spectrum"]
+    plan.profile(False)
+    per_inst = 16 * n.size + 8 * nent  # coefficient pairs read, one value written per (n, l)
+    per_launch_ms = ms / launches
+    ach = per_inst * inst / launches / (per_launch_ms / 1e3) / 1e9
+    res = {"metric": "Zernike power spectra/s (order 128)",
+           "value": round(B * args.steps / t, 2), "unit": "spectra/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "zernike_power_spectrum", "order": N, "instances": B,
+                      "coefficients": "N(0,1)/(1+n), torch seed 7"},
+           "max_rel_err_vs_oracle": worst,
+           "roofline": {"bound": "hbm", "kernel": "spectrum_kernel", "achieved": round(ach, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": None, "ms_per_launch": round(per_launch_ms, 4),
+                        "bytes_per_launch": per_inst * B}}
+    # SH, L = 512
+    L = 512
+    ls, ms_ = synth.sh_lm(L)
+    sc = torch.randn((B, ls.size, 2), generator=gen, dtype=torch.float64, device=dev)
+    splan = zest_amd.Plan(zest_amd.SH, 8)
+    sout = torch.empty((B, L), dtype=torch.float64, device=dev)
+    ts = _timed_steps(lambda: splan.power_spectrum(sc, L, sout, batch=B), args.steps, 1)
+    shost = sc[0].cpu().numpy()
+    serr = float(np.abs(sout[0].cpu().numpy() - oracle.st_power_spectrum(shost, L)).max()
+                 / np.abs(oracle.st_power_spectrum(shost, L)).max())
+    res["sh"] = {"value": round(B * args.steps / ts, 2), "unit": "spectra/s", "order": L, "instances": B,
+                 "max_rel_err_vs_oracle": serr}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_spectrum_baseline(N, c[:8].cpu().numpy(), args.cpu_seconds)
+    print(json.dumps(res), flush=True)
+
+
+def cpu_spectrum_baseline(order, cs, seconds):
+    """zest's zt::power_spectrum (or the oracle) on the host, one thread per core."""
+    import numpy as np
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    kind = "reference" if oracle.REF is not None else "port"
+    counts = [0] * threads
+    nent = (order + 1) ** 2 // 4
+    t0 = time.perf_counter()
+    deadline = t0 + seconds
+
+    def work(i):
+        c = np.ascontiguousarray(cs[i % len(cs)])
+        o = np.empty(nent)
+        while time.perf_counter() < deadline:
+            if kind == "reference":
+                oracle.REF.zref_zt_power_spectrum(1, 0, order, oracle._p(c), oracle._p(o))
+            else:
+                o = oracle.zt_power_spectrum(c, order)
+            counts[i] += 1
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    elapsed = time.perf_counter() - t0
+    total = sum(counts)
+    return {"value": round(total / elapsed, 2), "unit": "spectra/s", "cores": threads, "kind": kind,
+            "sample": f"{total} zt::power_spectrum calls at order {order} in {elapsed:.1f} s, "
+                      f"{threads} threads"}
 
 
 def run_rotate(args):

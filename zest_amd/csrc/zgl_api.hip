@@ -219,11 +219,12 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
 
 enum Stage { ST_FFT_FWD, ST_LEG_FWD, ST_RAD_FWD, ST_RAD_BWD, ST_LEG_BWD, ST_FFT_BWD,
              ST_EVAL_TABLES, ST_EVAL_RAD, ST_EVAL_LEG, ST_EVAL_LON, ST_ROTATE, ST_ROTATE_WIDE, ST_POLAR,
-             ST_COUNT };
+             ST_SPECTRUM, ST_COUNT };
 const char* const kStageNames[ST_COUNT] = {"fft_forward", "legendre_forward", "radial_forward",
                                            "radial_backward", "legendre_backward", "fft_backward",
                                            "eval_tables", "eval_radial", "eval_legendre",
-                                           "eval_longitude", "rotate", "rotate_wide", "polar_rotate"};
+                                           "eval_longitude", "rotate", "rotate_wide", "polar_rotate",
+                                           "power_spectrum"};
 
 hipEvent_t pooled_event(zgl_plan* p) {
     if (!p->pool.empty()) {
@@ -1118,8 +1119,10 @@ zgl_status spectrum_entry(zgl_plan* plan, const double* a, const double* b, size
     return guarded([&]() -> zgl_status {
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
-            power_spectrum(plan->kind, plan->shnorm, (int)order, Inst<const double>{a, cs, nullptr},
-                           Inst<const double>{b, cs, nullptr}, out, os, (int)batch, s);
+            timed(plan, ST_SPECTRUM, (int)batch, s, [&] {
+                power_spectrum(plan->kind, plan->shnorm, (int)order, Inst<const double>{a, cs, nullptr},
+                               Inst<const double>{b, cs, nullptr}, out, os, (int)batch, s);
+            });
             return ZGL_SUCCESS;
         }
         const bool cross = a != b;

@@ -2,10 +2,10 @@
 // st::cross_power_spectrum and zt::power_spectrum
 // (include/power_spectra.hpp:43-141), batched.
 //
-// One thread per output entry (l for SH, (n, l) for Zernike) sums its m run
-// of {cos, sin} pairs in zest's order (m = 0 cos part first, then m = 1..l);
-// the pairs of one entry are contiguous, and the entries of one instance are
-// contiguous runs in the same order, so a wave reads one span of the array.
+// Sixteen lanes per output entry (l for SH, (n, l) for Zernike): the {cos, sin}
+// pairs of one entry are contiguous, so the group reads 16 consecutive pairs
+// (256 bytes) per step, each lane summing m = k, k + 16, ... (the m = 0 sin
+// part excluded, as zest excludes it), and the group reduces with shuffles.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -16,10 +16,17 @@ namespace zgl {
 
 namespace {
 
-__global__ void spectrum_kernel(int zernike, int order, long nent, Inst<const double> a,
-                                Inst<const double> b, double norm, double* out, long os) {
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e >= nent) return;
+constexpr int kGroup = 8;                // lanes per entry
+constexpr int kSpecThreads = 256;
+constexpr int kEntriesPerBlock = kSpecThreads / kGroup;
+
+template <bool CROSS>
+__global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(int zernike, long nent, Inst<const double> a,
+                                                                Inst<const double> b, double norm, double* out,
+                                                                long os) {
+    const long e = blockIdx.x * (long)kEntriesPerBlock + (threadIdx.x / kGroup);
+    const int k = threadIdx.x % kGroup;
+    if (e >= nent) return;  // whole groups leave together
     const int inst = blockIdx.y;
     long first;  // pair index of m = 0
     int l;
@@ -33,11 +40,20 @@ __global__ void spectrum_kernel(int zernike, int order, long nent, Inst<const do
         l = (int)e;
         first = tri_idx(l, 0);
     }
+    // 8-byte loads: instance strides are in doubles and need not keep pairs 16-byte aligned
     const double* x = a.at(inst) + 2 * first;
     const double* y = b.at(inst) + 2 * first;
-    double s = x[0] * y[0];
-    for (int m = 1; m <= l; ++m) s += x[2 * m] * y[2 * m] + x[2 * m + 1] * y[2 * m + 1];
-    out[inst * os + e] = norm * s;
+    double s = 0;
+#pragma unroll 4
+    for (int m = k; m <= l; m += kGroup) {
+        const double ur = x[2 * m], ui = x[2 * m + 1];
+        const double vr = CROSS ? y[2 * m] : ur, vi = CROSS ? y[2 * m + 1] : ui;
+        s += ur * vr;
+        if (m > 0) s += ui * vi;
+    }
+#pragma unroll
+    for (int d = kGroup / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kGroup);
+    if (k == 0) out[inst * os + e] = norm * s;
 }
 
 }  // namespace
@@ -51,8 +67,14 @@ void power_spectrum(zgl_kind kind, int sh_norm, int order, Inst<const double> a,
     const long nent = spectrum_size(kind, order);
     if (nent == 0 || nb == 0) return;
     const double norm = sh_norm == ZGL_SHNORM_QM ? 1.0 : 1.0 / (4.0 * M_PI);
-    spectrum_kernel<<<dim3((unsigned)((nent + 255) / 256), nb), 256, 0, s>>>(
-        kind == ZGL_ZERNIKE ? 1 : 0, order, nent, a, b, norm, out, out_stride);
+    const dim3 grid((unsigned)((nent + kEntriesPerBlock - 1) / kEntriesPerBlock), nb);
+    const bool cross = a.base != b.base || a.stride != b.stride || a.offs != b.offs;
+    if (cross)
+        spectrum_kernel<true><<<grid, kSpecThreads, 0, s>>>(kind == ZGL_ZERNIKE ? 1 : 0, nent, a, b, norm, out,
+                                                           out_stride);
+    else
+        spectrum_kernel<false><<<grid, kSpecThreads, 0, s>>>(kind == ZGL_ZERNIKE ? 1 : 0, nent, a, b, norm, out,
+                                                            out_stride);
     check_hip(hipGetLastError(), "power spectrum launch");
 }
 
