@@ -23,8 +23,9 @@
 // order during the load, and each pass of radix p multiplies its inputs by a
 // precomputed per-slot twiddle table (kept in LDS, so no index arithmetic
 // beyond the butterfly base) and runs p-point DFTs in registers (the
-// symmetric cos/sin form for odd p). Prime radices above 17 use an LDS-staged
-// variant that computes two outputs per work item.
+// symmetric cos/sin form for odd p). Prime radices 19..255 run on the matrix
+// cores (pass_mfma, in kernel variants of their own); larger primes use an
+// LDS-staged variant that computes two outputs per work item.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
