@@ -4,7 +4,7 @@ products over the symmetric sums and differences of the inputs. Checked
 against the FP64 oracle at 1e-12, against the VALU pass (ZGL_FFT_MFMA=0)
 to rounding level, in the FP32 mode against the oracle at the FP32 level,
 and for independence from the tile width (bit-identical; widths 16 and 8 run
-the lean kernel variant, width 4 the general one)."""
+the lean kernel variant, widths 32 and 4 the general one)."""
 import numpy as np
 import pytest
 
@@ -81,9 +81,31 @@ def test_prime_radix_tile_widths(device, monkeypatch, kind, order, layout):
     gen = synth.zernike_coeffs if zt else synth.sh_coeffs
     c = np.stack([gen(order, 500 + b) for b in range(2)])
     res = {}
-    for tile in ("16", "8", "4"):
+    for tile in ("32", "16", "8", "4"):
         monkeypatch.setenv("ZGL_FFT_TILE", tile)
         res[tile] = _roundtrip(device, kind, order, c, layout=layout)
-    for tile in ("8", "4"):
+    for tile in ("32", "8", "4"):
         assert np.array_equal(res[tile][0], res["16"][0]), tile
         assert np.array_equal(res[tile][1], res["16"][1]), tile
+
+
+@pytest.mark.parametrize("order,layout", [(512, 0), (512, 1), (300, 0)])
+def test_long_lengths_global_twiddles(device, monkeypatch, order, layout):
+    """FFT lengths from 384 on keep their twiddle tables in global memory
+    (2L-1 = 1023 = 3 * 11 * 31; 599, a prime above the matrix-core range, on
+    the VALU pass): SH against the oracle at 1e-12, and tile widths 8 and 4
+    bit-identical."""
+    import oracle
+    from zest_amd import synth
+    c = np.stack([synth.sh_coeffs(order, 600 + b) for b in range(2)])
+    res = {}
+    for tile in ("8", "4"):
+        monkeypatch.setenv("ZGL_FFT_TILE", tile)
+        res[tile] = _roundtrip(device, "sh", order, c, layout=layout)
+    assert np.array_equal(res["8"][0], res["4"][0]) and np.array_equal(res["8"][1], res["4"][1])
+    g, co = res["4"]
+    for b in range(2):
+        g_ref = oracle.st_backward(c[b], order, layout=layout)
+        c_ref = oracle.st_forward(g_ref, order, layout=layout)
+        print(f"sh {order} layout {layout}: backward {_rel(g[b], g_ref):.2e}, forward {_rel(co[b], c_ref):.2e}")
+        assert _rel(g[b], g_ref) <= 1e-12 and _rel(co[b], c_ref) <= 1e-12

@@ -449,23 +449,37 @@ __device__ inline void tile_origin(const PairMap& pm, int tile, int& h0, int& c0
     }
 }
 
-// Shared-memory layout: data [n*T], e^{-2 pi i t/n} [n], pass twiddles
-// [(npass-1)*n], then the DIT permutation [n] as unsigned shorts.
+// Tile widths up to kTwiddleGlobalMaxT keep the twiddle tables in global
+// memory (read through L1/L2, where every workgroup finds the same few KB);
+// wider tiles stage them in LDS. Only long sequences take such narrow tiles,
+// and beside them the tables would take a third of the LDS or more and cut
+// the workgroups per CU (n = 1023, FP64, T = 4: 50 KB of tables beside 65 KB
+// of data). Fixed per width, so the compiler knows each table's address
+// space (LDS or global loads, never flat ones).
+constexpr int kTwiddleGlobalMaxT = 4;
+__host__ __device__ constexpr bool twiddles_in_lds(int T) { return T > kTwiddleGlobalMaxT; }
+
 template <int T, class C>
-__device__ inline void stage_tables(C*& data, C*& twl, C*& twp, unsigned short*& perm,
+__device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsigned short*& perm,
                                     const FftDesc& d, const C* twg, const C* twpg,
                                     const unsigned short* permg) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = d.n;
     data = reinterpret_cast<C*>(smem_raw);
-    twl = data + n * T;
-    twp = twl + n;
-    perm = reinterpret_cast<unsigned short*>(twp + (d.npass > 0 ? d.npass - 1 : 0) * n);
-    for (int i = threadIdx.x; i < n; i += kThreads) {
-        twl[i] = twg[i];
-        perm[i] = permg[i];
+    if constexpr (twiddles_in_lds(T)) {
+        C* tl = data + n * T;
+        C* tp = tl + n;
+        perm = reinterpret_cast<unsigned short*>(tp + (d.npass > 0 ? d.npass - 1 : 0) * n);
+        for (int i = threadIdx.x; i < n; i += kThreads) tl[i] = twg[i];
+        for (int i = threadIdx.x; i < (d.npass - 1) * n; i += kThreads) tp[i] = twpg[i];
+        twl = tl;
+        twp = tp;
+    } else {
+        perm = reinterpret_cast<unsigned short*>(data + n * T);
+        twl = twg;
+        twp = twpg;
     }
-    for (int i = threadIdx.x; i < (d.npass - 1) * n; i += kThreads) twp[i] = twpg[i];
+    for (int i = threadIdx.x; i < n; i += kThreads) perm[i] = permg[i];
     __syncthreads();
 }
 
@@ -485,7 +499,8 @@ void fft_forward_kernel(
     using C = typename Cplx<R>::type;
     const int tile = xcd_tile(pm);
     if (tile >= pm.ntiles) return;
-    C *data, *twl, *twp;
+    C* data;
+    const C *twl, *twp;
     unsigned short* perm;
     stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
@@ -588,7 +603,8 @@ void fft_backward_kernel(
     using C = typename Cplx<R>::type;
     const int tile = xcd_tile(pm);
     if (tile >= pm.ntiles) return;
-    C *data, *twl, *twp;
+    C* data;
+    const C *twl, *twp;
     unsigned short* perm;
     stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
@@ -694,7 +710,8 @@ int lds_limit() {
 }
 
 size_t lds_bytes(int n, int npass, int T, size_t csize) {
-    return (size_t)(n * T + n + (npass > 0 ? npass - 1 : 0) * n) * csize + 2 * (size_t)n;
+    const size_t tables = twiddles_in_lds(T) ? (size_t)(n + (npass > 0 ? npass - 1 : 0) * n) * csize : 0;
+    return (size_t)n * T * csize + tables + 2 * (size_t)n;
 }
 
 PairMap pair_map(const Geom& g, int T) {
@@ -815,9 +832,14 @@ bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf) {
     return true;
 }
 
+// The widest tile (at most 16) that leaves room for two workgroups per CU
+// (80 KB of LDS each), else the widest that fits at all.
 int choose_tile(int n, const std::vector<int>& rad, size_t csize, bool mf) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     const int npass = (int)rad.size();
+    for (int T = 16; T >= 1; T /= 2)
+        if (lds_bytes(n, npass, T, csize) <= 80 * 1024 && tile_fits(n, rad.data(), npass, T, csize, mf))
+            return T;
     int T = 16;
     for (; T > 1; T /= 2)
         if (tile_fits(n, rad.data(), npass, T, csize, mf)) break;
@@ -842,7 +864,7 @@ std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
     std::vector<int> out;
     const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
     const int T0 = p.tile[fp32 ? 1 : 0];
-    if (T0 == 16 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 32, cs, p.mfma_pass)) out.push_back(32);
+    if (T0 < 32 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 2 * T0, cs, p.mfma_pass)) out.push_back(2 * T0);
     for (int T = T0; T >= 1 && out.size() < 3; T /= 2)
         if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs, p.mfma_pass)) out.push_back(T);
     return out;
