@@ -319,25 +319,38 @@ __global__ __launch_bounds__(256, 2) void rotate_fused_kernel(RotArgs a) {
     __syncthreads();
 
     // ---- stage: element e = (column e / L1, m' e % L1): consecutive lanes read
-    // consecutive pairs of one block
-    for (int e0 = 0; e0 < nel; e0 += 256 * 8) {
-        d2 v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + 256 * u + tid;
-            v[u] = d2{0.0, 0.0};
-            if (e < nel) {
-                const int c = e / L1, mp = e - c * L1;
-                const long off = coff[c];
-                if (off >= 0) v[u] = *reinterpret_cast<const d2*>(a.src.base + off + 2 * mp);
-            }
+    // consecutive pairs of one block. A thread's elements step by 256, so its
+    // (column, m') pair is stepped by (256 / L1, 256 % L1) with one carry.
+    const int cq = 256 / L1, cr = 256 - cq * L1;
+    auto advance = [&](int& c, int& mp) {
+        c += cq;
+        mp += cr;
+        if (mp >= L1) {
+            mp -= L1;
+            ++c;
         }
+    };
+    {
+        int c = tid / L1, mp = tid - (tid / L1) * L1;
+        for (int e0 = 0; e0 < nel; e0 += 256 * 8) {
+            d2 v[8];
+            int cu[8], mu[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + 256 * u + tid;
-            if (e < nel) {
-                const int c = e / L1, mp = e - c * L1;
-                Xs[mp * CT + c] = v[u];
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 256 * u + tid;
+                cu[u] = c;
+                mu[u] = mp;
+                v[u] = d2{0.0, 0.0};
+                if (e < nel) {
+                    const long off = coff[c];
+                    if (off >= 0) v[u] = *reinterpret_cast<const d2*>(a.src.base + off + 2 * mp);
+                }
+                advance(c, mp);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 256 * u + tid;
+                if (e < nel) Xs[mu[u] * CT + cu[u]] = v[u];
             }
         }
     }
@@ -400,21 +413,49 @@ __global__ __launch_bounds__(256, 2) void rotate_fused_kernel(RotArgs a) {
     const int steps = (l / 2 + 1 + 3) / 4;
 
     d4 are[4], aim[4];
+    // Per lane and row task: pointers to the d values of step 0 (the row m of
+    // the lane, columns m' = 2g + parity and its opposite parity), advanced by
+    // 8 values of m' per step, and the last step each is inside the degree;
+    // per slot the same for the B operand in LDS. No index arithmetic per step.
+    const int ld = l + 1;
+    const int bstep = 8 * CT;
+    const d2 *bre_p[4], *bim_p[4];
+    int blre[4], blim[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int pa = (l + tp[k]) & 1;
+        const int cre = 2 * g + pa, cim = 2 * g + (pa ^ 1);
+        bre_p[k] = Xs + cre * CT + tcol[k] + j;
+        bim_p[k] = Xs + cim * CT + tcol[k] + j;
+        blre[k] = cre <= l ? (l - cre) >> 3 : -1;
+        blim[k] = cim <= l ? (l - cim) >> 3 : -1;
+    }
     auto product = [&](int pass) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) are[k] = aim[k] = d4{0, 0, 0, 0};
-        auto dval = [&](int m, int mp) -> double {
-            return pass == 0 ? dl[(long)(l + 1) * m + mp] : dl[(long)(l + 1) * mp + m];
-        };
+        const long dstep = pass == 0 ? 8 : 8L * ld;
+        const double *pre[4], *pim[4];
+        int lre[4], lim[4];
+        bool z0[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int pa = (l + rp[r]) & 1;
+            const int m = rrow[r] + 2 * j;
+            const bool mok = rok[r] && m <= l;
+            const int cre = 2 * g + pa, cim = 2 * g + (pa ^ 1);
+            pre[r] = pass == 0 ? dl + (long)ld * m + cre : dl + (long)ld * cre + m;
+            pim[r] = pass == 0 ? dl + (long)ld * m + cim : dl + (long)ld * cim + m;
+            lre[r] = mok && cre <= l ? (l - cre) >> 3 : -1;
+            lim[r] = mok && cim <= l ? (l - cim) >> 3 : -1;
+            z0[r] = cre == 0;  // m' = 0 at step 0: the weight 1 instead of 2
+        }
         // d values of step s for each row task
         auto load = [&](int s, double (&xa)[4], double (&xb)[4]) {
+            const long o = s * dstep;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int pa = (l + rp[r]) & 1;
-                const int m = rrow[r] + 2 * j;
-                const int mre = 2 * (4 * s + g) + pa, mim = 2 * (4 * s + g) + (pa ^ 1);
-                xa[r] = (rok[r] && m <= l && mre <= l) ? (mre ? 2.0 : 1.0) * dval(m, mre) : 0.0;
-                xb[r] = (rok[r] && m <= l && mim <= l) ? 2.0 * dval(m, mim) : 0.0;
+                xa[r] = s <= lre[r] ? ((s == 0 && z0[r]) ? 1.0 : 2.0) * pre[r][o] : 0.0;
+                xb[r] = s <= lim[r] ? 2.0 * pim[r][o] : 0.0;
             }
         };
         // two steps in flight ahead of the MFMAs
@@ -424,13 +465,12 @@ __global__ __launch_bounds__(256, 2) void rotate_fused_kernel(RotArgs a) {
         for (int s = 0; s < steps; ++s) {
             double na[4] = {0, 0, 0, 0}, nb[4] = {0, 0, 0, 0};
             if (s + 2 < steps) load(s + 2, na, nb);
+            const int bo = s * bstep;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (!tok[k]) continue;
-                const int pa = (l + tp[k]) & 1;
-                const int mre = 2 * (4 * s + g) + pa, mim = 2 * (4 * s + g) + (pa ^ 1);
-                const double bre = mre <= l ? Xs[mre * CT + tcol[k] + j][0] : 0.0;
-                const double bim = mim <= l ? Xs[mim * CT + tcol[k] + j][1] : 0.0;
+                const double bre = s <= blre[k] ? bre_p[k][bo][0] : 0.0;
+                const double bim = s <= blim[k] ? bim_p[k][bo][1] : 0.0;
                 are[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa0[tr[k]], bre, are[k], 0, 0, 0);
                 aim[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb0[tr[k]], bim, aim[k], 0, 0, 0);
             }
@@ -478,10 +518,13 @@ __global__ __launch_bounds__(256, 2) void rotate_fused_kernel(RotArgs a) {
     __syncthreads();
     phase_pass(0, 2);
     __syncthreads();
-    for (int e = tid; e < nel; e += 256) {
-        const int c = e / L1, mp = e - c * L1;
-        const long off = coff[c];
-        if (off >= 0) *reinterpret_cast<d2*>(a.dst.base + off + 2 * mp) = Xs[mp * CT + c];
+    {
+        int c = tid / L1, mp = tid - (tid / L1) * L1;
+        for (int e = tid; e < nel; e += 256) {
+            const long off = coff[c];
+            if (off >= 0) *reinterpret_cast<d2*>(a.dst.base + off + 2 * mp) = Xs[mp * CT + c];
+            advance(c, mp);
+        }
     }
 }
 
