@@ -36,6 +36,15 @@
 
 #include "zgl_internal.hpp"
 
+// The kernels are compiled in three translation units, one per (arithmetic,
+// grid) pair, so that they build in parallel: ZGL_FFT_PART 0 (this file) holds
+// the plan functions and the FP64 entry points, zgl_fft_f32.hip (part 1) the
+// FP32-arithmetic entry points on FP64 grids, zgl_fft_f32g.hip (part 2) those
+// on FP32 grids.
+#ifndef ZGL_FFT_PART
+#define ZGL_FFT_PART 0
+#endif
+
 namespace zgl {
 
 namespace {
@@ -729,12 +738,16 @@ PairMap pair_map(const Geom& g, int T) {
 // Workgroups per instance: a multiple of 8 (see xcd_tile).
 int grid_tiles(const PairMap& pm) { return (pm.ntiles + 7) / 8 * 8; }
 
-template <class R> const typename Cplx<R>::type* tw_of(const FftPlan& f);
-template <> const double2* tw_of<double>(const FftPlan& f) { return f.tw; }
-template <> const float2* tw_of<float>(const FftPlan& f) { return f.tw32; }
-template <class R> const typename Cplx<R>::type* twp_of(const FftPlan& f);
-template <> const double2* twp_of<double>(const FftPlan& f) { return f.twp; }
-template <> const float2* twp_of<float>(const FftPlan& f) { return f.twp32; }
+template <class R>
+const typename Cplx<R>::type* tw_of(const FftPlan& f) {
+    if constexpr (sizeof(R) == 8) return f.tw;
+    else return f.tw32;
+}
+template <class R>
+const typename Cplx<R>::type* twp_of(const FftPlan& f) {
+    if constexpr (sizeof(R) == 8) return f.twp;
+    else return f.twp32;
+}
 
 // The lean variant is instantiated for the widths prime lengths use (others
 // fall back to the general one, which computes the same values).
@@ -834,7 +847,7 @@ bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf) {
 
 // The widest tile (at most 16) that leaves room for two workgroups per CU
 // (80 KB of LDS each), else the widest that fits at all.
-int choose_tile(int n, const std::vector<int>& rad, size_t csize, bool mf) {
+[[maybe_unused]] int choose_tile(int n, const std::vector<int>& rad, size_t csize, bool mf) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     const int npass = (int)rad.size();
     for (int T = 16; T >= 1; T /= 2)
@@ -858,6 +871,7 @@ C* upload(const std::vector<C>& v, hipStream_t s, const char* what) {
 
 }  // namespace
 
+#if ZGL_FFT_PART == 0
 std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
     // the plan's width, half of it, and the double width where it fits
     // (longer contiguous segments per longitude row, fewer workgroups per CU)
@@ -964,6 +978,8 @@ void fft_plan_destroy(FftPlan& p) {
     p = FftPlan{};
 }
 
+#endif  // ZGL_FFT_PART == 0
+
 template <class R, class G>
 void fft_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long spec_stride, int nb,
                  hipStream_t s) {
@@ -977,11 +993,15 @@ void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, Inst<G
 }
 
 // FP64 grids with FP64 or FP32 arithmetic; FP32 grids with FP32 arithmetic.
+#if ZGL_FFT_PART == 0
 template void fft_forward<double, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
-template void fft_forward<float, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
-template void fft_forward<float, float>(const FftLaunch&, Inst<const float>, void*, long, int, hipStream_t);
 template void fft_backward<double, double>(const FftLaunch&, const void*, long, Inst<double>, int, hipStream_t);
+#elif ZGL_FFT_PART == 1
+template void fft_forward<float, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
 template void fft_backward<float, double>(const FftLaunch&, const void*, long, Inst<double>, int, hipStream_t);
+#else
+template void fft_forward<float, float>(const FftLaunch&, Inst<const float>, void*, long, int, hipStream_t);
 template void fft_backward<float, float>(const FftLaunch&, const void*, long, Inst<float>, int, hipStream_t);
+#endif
 
 }  // namespace zgl

@@ -1,0 +1,4 @@
+// Part 1 of the longitude FFT: the FP32-arithmetic entry points on FP64 grids
+// (the kernels and their description are in zgl_fft.hip).
+#define ZGL_FFT_PART 1
+#include "zgl_fft.hip"

@@ -574,12 +574,14 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
     const int nstrips = (rows + 15) >> 4;
     static_assert(NS >= 1 && NS <= 4, "strip count");
 #define ZGL_TILES(n)                                                                                  \
-    if constexpr (X3) {                                                                              \
-        if (center) stream_tiles_x3<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane);   \
-        else stream_tiles_x3<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);               \
-    } else {                                                                                         \
-        if (center) stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane);      \
-        else stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);                  \
+    {                                                                                                \
+        if constexpr (X3) {                                                                          \
+            if (center) stream_tiles_x3<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane); \
+            else stream_tiles_x3<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);             \
+        } else {                                                                                     \
+            if (center) stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane);    \
+            else stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);                \
+        }                                                                                            \
     }
     switch (nstrips) {
         case 1: ZGL_TILES(1); break;
