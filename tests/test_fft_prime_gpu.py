@@ -33,9 +33,11 @@ def _roundtrip(device, kind, order, c, layout=0, precision=0):
 
 
 # 2N-1: 19, 31, 127, 3 * 59, 5 * 37, 191 and 11 * 19 (the general kernel
-# variant; the others run the lean one, whose other radices are at most 7)
+# variant; the others run the lean one, whose other radices are at most 7);
+# SH 2L-1 = 31, 199 and 437 = 23 * 19 (two matrix-core passes, the second
+# with input twiddles)
 CASES = [("zernike", 10), ("zernike", 16), ("zernike", 64), ("zernike", 89), ("zernike", 93),
-         ("zernike", 96), ("zernike", 105), ("sh", 16), ("sh", 100)]
+         ("zernike", 96), ("zernike", 105), ("sh", 16), ("sh", 100), ("sh", 219)]
 
 
 @pytest.mark.parametrize("kind,order", CASES)
