@@ -58,6 +58,9 @@ template <> struct Mfma<float> {
 constexpr int TN = 64;     // columns per tile
 constexpr int TK = 32;     // contraction chunk
 constexpr int kThreads = 256;
+// the operand loaders map 32 consecutive threads onto one chunk row (and the
+// pair loader 64 threads onto 32 (k, re/im) pairs)
+static_assert(TK == 32 && TN == 64 && kThreads == 256, "loader thread mapping");
 
 // Operand kinds (see the header comment).
 enum { ROWS_OUT = 0, ROWS_K = 1 };
