@@ -204,8 +204,12 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
         // Large chunks keep every CU busy through each stage and cut launches;
         // measured faster than chunks sized to stay in the 256 MB Infinity
         // Cache (order 128: 64 instances of 27 MB FP32 / 54 MB FP64 scratch).
+        // The cap keeps the work per chunk about that of 64 instances at order
+        // 128 (memory ~ N^3): small orders take far more instances per chunk
+        // (order 16: 608k -> 1.74M transforms/s with 1024 instead of 64).
         budget = env_mb("ZGL_SCRATCH_MB", 4096) << 20;
-        cap = 64;
+        const double scale = 128.0 / (double)std::max<size_t>(1, g.order);
+        cap = (size_t)std::min(8192.0, std::max(64.0, 64.0 * scale * scale * scale));
     } else {
         // Instances form the GEMM column axis: larger chunks reuse the P table more.
         budget = env_mb("ZGL_SH_SCRATCH_MB", 4096) << 20;
