@@ -440,6 +440,7 @@ struct PairMap {
 // walks a contiguous range of tiles: the column tiles of one latitude pair,
 // which share cache lines at their edges, meet in the same L2.
 __device__ inline int xcd_tile(const PairMap& pm) {
+    if (pm.ntiles < 8) return blockIdx.x;  // one workgroup per tile (see grid_tiles)
     const int q = (pm.ntiles + 7) >> 3;
     const int b = blockIdx.x;
     return (b & 7) * q + (b >> 3);
@@ -735,8 +736,10 @@ PairMap pair_map(const Geom& g, int T) {
     return pm;
 }
 
-// Workgroups per instance: a multiple of 8 (see xcd_tile).
-int grid_tiles(const PairMap& pm) { return (pm.ntiles + 7) / 8 * 8; }
+// Workgroups per instance: a multiple of 8 (see xcd_tile); instances of fewer
+// than 8 tiles (short SH grids) get one workgroup per tile, not 8 of which
+// most would exit at once.
+int grid_tiles(const PairMap& pm) { return pm.ntiles < 8 ? pm.ntiles : (pm.ntiles + 7) / 8 * 8; }
 
 template <class R>
 const typename Cplx<R>::type* tw_of(const FftPlan& f) {
