@@ -96,9 +96,11 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
     // Consecutive threads always read consecutive elements of one row.
     constexpr int A_NQ = (S::A_KIND == ROWS_OUT) ? (TM + 7) / 8 : (TK / 8) * ((TM + 31) / 32);
     constexpr int B_NQ = (S::B_KIND == ROWS_OUT) ? (TN / 8) : (TK / 4);
-    T ra[A_NQ], rb[B_NQ];
+    // two register buffers per operand: the chunk after next is in flight while
+    // the current one is computed (loads get two chunks of MFMA work to land)
+    T ra0[A_NQ], rb0[B_NQ], ra1[A_NQ], rb1[B_NQ];
 
-    auto load_a = [&](int k0) {
+    auto load_a = [&](int k0, T (&ra)[A_NQ]) {
         if constexpr (S::A_KIND == ROWS_OUT) {
             const int k = tid & (TK - 1);
 #pragma unroll
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
             }
         }
     };
-    auto store_a = [&]() {
+    auto store_a = [&](const T (&ra)[A_NQ]) {
         if constexpr (S::A_KIND == ROWS_OUT) {
             const int k = tid & (TK - 1);
 #pragma unroll
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
     // B: ROWS_K -> thread (k = tid / 64 + 4q, j = tid % 64); ROWS_OUT -> rows j
     // hold the contraction axis: thread (j = tid / 32 + 8q, k = tid % 32); with
     // B_PAIR the rows are indexed by j / 2 and hold (k, re/im) pairs.
-    auto load_b = [&](int k0) {
+    auto load_b = [&](int k0, T (&rb)[B_NQ]) {
         if constexpr (S::B_KIND == ROWS_K) {
             const int j = tid & 63;
 #pragma unroll
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
             }
         }
     };
-    auto store_b = [&]() {
+    auto store_b = [&](const T (&rb)[B_NQ]) {
         if constexpr (S::B_KIND == ROWS_K) {
             const int j = tid & 63;
 #pragma unroll
@@ -183,16 +185,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc[q] = typename M::acc_t{0, 0, 0, 0};
 
-    load_a(0);
-    load_b(0);
-    for (int k0 = 0; k0 < P.K; k0 += TK) {
-        store_a();
-        store_b();
-        __syncthreads();
-        if (k0 + TK < P.K) {  // prefetch the next chunk into registers
-            load_a(k0 + TK);
-            load_b(k0 + TK);
-        }
+    auto compute = [&](int k0) {
         const int kpad = (min(TK, P.K - k0) + 3) & ~3;
         for (int kk = 0; kk < kpad; kk += 4) {
             const int kr = kk + (lane >> 4);
@@ -205,7 +198,30 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
                 }
             }
         }
+    };
+    // one chunk: stage its registers, refill them with the chunk two ahead,
+    // compute from LDS
+    auto chunk = [&](int k0, T (&ra)[A_NQ], T (&rb)[B_NQ]) {
+        store_a(ra);
+        store_b(rb);
         __syncthreads();
+        if (k0 + 2 * TK < P.K) {
+            load_a(k0 + 2 * TK, ra);
+            load_b(k0 + 2 * TK, rb);
+        }
+        compute(k0);
+        __syncthreads();
+    };
+
+    load_a(0, ra0);
+    load_b(0, rb0);
+    if (TK < P.K) {
+        load_a(TK, ra1);
+        load_b(TK, rb1);
+    }
+    for (int k0 = 0; k0 < P.K; k0 += 2 * TK) {
+        chunk(k0, ra0, rb0);
+        if (k0 + TK < P.K) chunk(k0 + TK, ra1, rb1);
     }
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
