@@ -459,15 +459,19 @@ __device__ inline void tile_origin(const PairMap& pm, int tile, int& h0, int& c0
     }
 }
 
-// Tile widths up to kTwiddleGlobalMaxT keep the twiddle tables in global
+// Narrow tiles keep the twiddle tables in global
 // memory (read through L1/L2, where every workgroup finds the same few KB);
 // wider tiles stage them in LDS. Only long sequences take such narrow tiles,
 // and beside them the tables would take a third of the LDS or more and cut
 // the workgroups per CU (n = 1023, FP64, T = 4: 50 KB of tables beside 65 KB
-// of data). Fixed per width, so the compiler knows each table's address
-// space (LDS or global loads, never flat ones).
-constexpr int kTwiddleGlobalMaxT = 4;
-__host__ __device__ constexpr bool twiddles_in_lds(int T) { return T > kTwiddleGlobalMaxT; }
+// of data). FP64 keeps them global up to T = 8 (n = 511: 66 KB of data
+// alone, two workgroups per CU only without the tables); FP32 only up to
+// T = 4 (n = 1023, T = 8: the forward pass measured 15% slower with global
+// tables than with them in LDS). Fixed per width and element size, so the compiler
+// knows each table's address space (LDS or global loads, never flat ones).
+__host__ __device__ constexpr bool twiddles_in_lds(int T, size_t csize) {
+    return T > (csize == 16 ? 8 : 4);
+}
 
 template <int T, class C>
 __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsigned short*& perm,
@@ -476,7 +480,7 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = d.n;
     data = reinterpret_cast<C*>(smem_raw);
-    if constexpr (twiddles_in_lds(T)) {
+    if constexpr (twiddles_in_lds(T, sizeof(C))) {
         C* tl = data + n * T;
         C* tp = tl + n;
         perm = reinterpret_cast<unsigned short*>(tp + (d.npass > 0 ? d.npass - 1 : 0) * n);
@@ -720,7 +724,7 @@ int lds_limit() {
 }
 
 size_t lds_bytes(int n, int npass, int T, size_t csize) {
-    const size_t tables = twiddles_in_lds(T) ? (size_t)(n + (npass > 0 ? npass - 1 : 0) * n) * csize : 0;
+    const size_t tables = twiddles_in_lds(T, csize) ? (size_t)(n + (npass > 0 ? npass - 1 : 0) * n) * csize : 0;
     return (size_t)n * T * csize + tables + 2 * (size_t)n;
 }
 
