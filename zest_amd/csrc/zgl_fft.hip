@@ -430,15 +430,14 @@ __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, 
 
 struct PairMap {
     int ncol, nth, jh, lat_stride, lon_stride;
-    int tiles_per_h;  // Zernike: tiles per latitude pair (ncol > 1)
-    int ntiles;       // tiles per instance
+    int ntiles;  // tiles per instance
 };
 
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD
 // b % 8), each with its own L2. The grid has 8 * ceil(ntiles / 8) workgroups
 // per instance and workgroup b takes tile (b % 8) * q + b / 8, so every XCD
-// walks a contiguous range of tiles: the column tiles of one latitude pair,
-// which share cache lines at their edges, meet in the same L2.
+// walks a contiguous range of tiles: neighbouring column tiles, which share
+// cache lines at their edges, meet in the same L2.
 __device__ inline int xcd_tile(const PairMap& pm) {
     if (pm.ntiles < 8) return blockIdx.x;  // one workgroup per tile (see grid_tiles)
     const int q = (pm.ntiles + 7) >> 3;
@@ -446,17 +445,18 @@ __device__ inline int xcd_tile(const PairMap& pm) {
     return (b & 7) * q + (b >> 3);
 }
 
-// (h, first column) of a tile: Zernike tiles run over the radial nodes of one
-// latitude pair; SH tiles (ncol = 1) run over T consecutive latitude pairs.
+// A tile holds T consecutive columns f = h * ncol + c of the flattened
+// (latitude pair h, radial node c) index, so a tile may end one pair and
+// start the next and every tile but the last is full (SH grids: ncol = 1,
+// f = h). Every column a thread touches is threadIdx.x % T (kThreads is a
+// multiple of T), so each thread resolves its (h, c) once; h >= jh marks a
+// column past the end.
 template <int T>
-__device__ inline void tile_origin(const PairMap& pm, int tile, int& h0, int& c0) {
-    if (pm.ncol > 1) {
-        h0 = tile / pm.tiles_per_h;
-        c0 = (tile - h0 * pm.tiles_per_h) * T;
-    } else {
-        h0 = tile * T;
-        c0 = 0;
-    }
+__device__ inline void thread_column(const PairMap& pm, int tile, int& h, int& c) {
+    static_assert(kThreads % T == 0, "a thread keeps one column");
+    const int f = tile * T + (int)threadIdx.x % T;
+    h = f / pm.ncol;
+    c = f - h * pm.ncol;
 }
 
 // Narrow tiles keep the twiddle tables in global
@@ -518,8 +518,9 @@ void fft_forward_kernel(
     unsigned short* perm;
     stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
-    int h0, c0;
-    tile_origin<T>(pm, tile, h0, c0);
+    int th, tc;
+    thread_column<T>(pm, tile, th, tc);
+    const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
     const G* g = grid.at(blockIdx.y);
     C* sp = spec + (long)blockIdx.y * spec_stride;
 
@@ -531,12 +532,10 @@ void fft_forward_kernel(
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * kThreads;
-                const int i = e / T, t = e % T;
-                const int h = pm.ncol > 1 ? h0 : h0 + t;
-                const int c = pm.ncol > 1 ? c0 + t : 0;
+                const int i = e / T, h = th, c = tc;
                 av[u] = 0;
                 bv[u] = 0;
-                if (e < n * T && h < pm.jh && c < pm.ncol) {
+                if (e < n * T && h < pm.jh) {
                     const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
                     const G* row = g + (long)i * pm.lon_stride + c;
                     av[u] = (R)row[(long)jn * pm.lat_stride];
@@ -579,10 +578,8 @@ void fft_forward_kernel(
     run_passes<T, -1, MF>(data, twl, twp, d);
 
     for (int e = threadIdx.x; e < nm * T; e += kThreads) {
-        const int m = e / T, t = e % T;
-        const int h = pm.ncol > 1 ? h0 : h0 + t;
-        const int c = pm.ncol > 1 ? c0 + t : 0;
-        if (h >= pm.jh || c >= pm.ncol) continue;
+        const int m = e / T, t = e % T, h = th, c = tc;
+        if (h >= pm.jh) continue;
         const C zm = data[m * T + t];
         const C zc = data[(m ? n - m : 0) * T + t];
         // two-for-one split: A = (Z_m + conj Z_{n-m}) / 2, B = (Z_m - conj Z_{n-m}) / (2i)
@@ -622,8 +619,9 @@ void fft_backward_kernel(
     unsigned short* perm;
     stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
-    int h0, c0;
-    tile_origin<T>(pm, tile, h0, c0);
+    int th, tc;
+    thread_column<T>(pm, tile, th, tc);
+    const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
     G* g = grid.at(blockIdx.y);
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
@@ -632,12 +630,10 @@ void fft_backward_kernel(
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const int e = e0 + u * kThreads;
-            const int m = e / T, t = e % T;
-            const int h = pm.ncol > 1 ? h0 : h0 + t;
-            const int c = pm.ncol > 1 ? c0 + t : 0;
+            const int m = e / T, h = th, c = tc;
             ae[u] = mk<C>(0, 0);
             ao[u] = mk<C>(0, 0);
-            if (e < F * T && h < pm.jh && c < pm.ncol && m < nm) {
+            if (e < F * T && h < pm.jh && m < nm) {
                 ae[u] = sp[((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c];
                 ao[u] = sp[((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c];
             }
@@ -646,8 +642,7 @@ void fft_backward_kernel(
         for (int u = 0; u < kBatch; ++u) {
             const int e = e0 + u * kThreads;
             if (e >= F * T) continue;
-            const int m = e / T, t = e % T;
-            const int h = pm.ncol > 1 ? h0 : h0 + t;
+            const int m = e / T, t = e % T, h = th;
             const R mu = m > 0 ? R(0.5) : R(1);
             C Sa, Sb;
             if (pm.nth / 2 + h == pm.jh - 1 - h) {
@@ -674,10 +669,8 @@ void fft_backward_kernel(
 
     if (LK == GRID_COLS) {
         for (int e = threadIdx.x; e < n * T; e += kThreads) {
-            const int i = e / T, t = e % T;
-            const int h = pm.ncol > 1 ? h0 : h0 + t;
-            const int c = pm.ncol > 1 ? c0 + t : 0;
-            if (h >= pm.jh || c >= pm.ncol) continue;
+            const int i = e / T, t = e % T, h = th, c = tc;
+            if (h >= pm.jh) continue;
             C z = data[i * T + t];
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
             if (wlat) {  // output weights (the transpose of the forward transform)
@@ -735,8 +728,7 @@ PairMap pair_map(const Geom& g, int T) {
     pm.jh = g.jh;
     pm.lat_stride = g.lat_stride;
     pm.lon_stride = g.lon_stride;
-    pm.tiles_per_h = (g.ncol + T - 1) / T;
-    pm.ntiles = pm.ncol > 1 ? pm.jh * pm.tiles_per_h : (pm.jh + T - 1) / T;
+    pm.ntiles = (int)(((long)pm.jh * pm.ncol + T - 1) / T);
     return pm;
 }
 
