@@ -242,8 +242,7 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
     const int j = lane & 15, g = lane >> 4;
     const int K = P.K, steps = (K + 3) >> 2;
     const int N = P.N;
-    const int ntile = (N + TNW - 1) / TNW;
-    const int tiles = gn * ntile;
+    const int tiles = (gn * N + TNW - 1) / TNW;
     const int ntw = tiles > wave ? (tiles - wave + 3) >> 2 : 0;  // tiles of this wave
     const int nchunks = max(1, (steps + kKC - 1) / kKC);
     const int U = ntw * nchunks;
@@ -258,20 +257,24 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
     V cpart = {};
     const T* Ac = As + (long)st.center_lds_row() * AS;
 
-    // unit u -> (instance, first column of this lane, chunk)
+    // unit u -> (instance, first column of this lane, chunk). A tile's TNW
+    // columns run over the flattened (instance, column) index of the group,
+    // so a tile may end one instance and start the next and only the group's
+    // last tile is partial (N is a multiple of 16, so a lane's VW columns
+    // never straddle two instances); lanes past the group get inst >= g0 + gn
     auto unit = [&](int u, int& inst, int& col, int& ch) {
         const int tw = u / nchunks;
         ch = u - tw * nchunks;
-        const int t = wave + 4 * tw;
-        const int gi = t / ntile;
+        const int fc = (wave + 4 * tw) * TNW + VW * j;
+        const int gi = fc / N;
         inst = g0 + gi;
-        col = (t - gi * ntile) * TNW + VW * j;
+        col = fc - gi * N;
     };
 
     auto load = [&](V (&buf)[kKC], int u) {
         int inst, col, ch;
         unit(u, inst, col, ch);
-        const bool colok = col < N;  // N is a multiple of 16, so VW columns are all in or all out
+        const bool colok = inst < g0 + gn;
         const T* bb = st.b_base(inst);
 #pragma unroll
         for (int s = 0; s < kKC; ++s) {
@@ -304,7 +307,7 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
         }
         if (ch != nchunks - 1) return;
         // store the finished tile and clear the accumulators
-        const bool colok = col < N;
+        const bool colok = inst < g0 + gn;
         T* cb = st.c_base(inst);
 #pragma unroll
         for (int q = 0; q < NSA; ++q) {
