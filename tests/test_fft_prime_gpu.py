@@ -111,3 +111,36 @@ def test_long_lengths_global_twiddles(device, monkeypatch, order, layout):
         c_ref = oracle.st_forward(g_ref, order, layout=layout)
         print(f"sh {order} layout {layout}: backward {_rel(g[b], g_ref):.2e}, forward {_rel(co[b], c_ref):.2e}")
         assert _rel(g[b], g_ref) <= 1e-12 and _rel(co[b], c_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("kind,order,layout", [("zernike", 20, 0), ("zernike", 33, 0),
+                                               ("sh", 41, 0), ("sh", 41, 1)])
+def test_fp32_tile_widths(device, monkeypatch, kind, order, layout):
+    """FP32 plans without a matrix-core pass (2N-1 = 39 = 3 * 13, 65 = 5 * 13,
+    81 = 3^4): width 32 runs 512 threads per workgroup, the others 256. All
+    widths give bit-identical results, and width 32 matches the oracle at the
+    FP32 level."""
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    zt = kind == "zernike"
+    gen = synth.zernike_coeffs if zt else synth.sh_coeffs
+    c = np.stack([gen(order, 700 + b) for b in range(3)])
+    res = {}
+    for tile in ("32", "16", "8"):
+        monkeypatch.setenv("ZGL_FFT_TILE", tile)
+        res[tile] = _roundtrip(device, kind, order, c, layout=layout, precision=zest_amd.FP32)
+    for tile in ("32", "8"):
+        assert np.array_equal(res[tile][0], res["16"][0]), tile
+        assert np.array_equal(res[tile][1], res["16"][1]), tile
+    g, co = res["32"]
+    for b in range(3):
+        if zt:
+            g_ref = oracle.zt_backward(c[b], order)
+            c_ref = oracle.zt_forward(g_ref, order)
+        else:
+            g_ref = oracle.st_backward(c[b], order, layout=layout)
+            c_ref = oracle.st_forward(g_ref, order, layout=layout)
+        print(f"{kind} {order} layout {layout} fp32 T=32: backward {_rel(g[b], g_ref):.2e}, "
+              f"forward {_rel(co[b], c_ref):.2e}")
+        assert _rel(g[b], g_ref) < 5e-6 and _rel(co[b], c_ref) < 5e-6

@@ -54,6 +54,15 @@ constexpr int kMaxBigItems = 9;
 constexpr int kMaxRegRadix = 17;  // larger primes use the LDS-staged pass
 constexpr int kBatch = 8;         // global loads in flight per thread
 
+// Threads per workgroup of a kernel variant: 512 for FP32 tiles of 32 column
+// pairs without the matrix-core pass (their grid rows are read in 128-byte
+// segments, and two workgroups per CU at n = 255 still make 16 waves), 256
+// otherwise.
+template <int T, class R, int MF>
+__host__ __device__ constexpr int fft_threads() {
+    return (sizeof(R) == 4 && T == 32 && MF == 0) ? 2 * kThreads : kThreads;
+}
+
 template <class R> struct Cplx;
 template <> struct Cplx<double> { using type = double2; };
 template <> struct Cplx<float> { using type = float2; };
@@ -148,10 +157,10 @@ __device__ inline C twiddled(C v, const C* tw, int pos) {
     return cmul(v, w);
 }
 
-template <int P, int T, int DIR, class C>
+template <int P, int T, int DIR, int NT, class C>
 __device__ __forceinline__ void pass_reg(C* data, const C* twl, const C* tw, int n, int L) {
     const int nbf = n / P;
-    for (int it = threadIdx.x; it < nbf * T; it += kThreads) {
+    for (int it = threadIdx.x; it < nbf * T; it += NT) {
         const int t = it % T;
         const int bf = it / T;
         const int bq = bf / L;
@@ -172,7 +181,7 @@ __device__ __forceinline__ void pass_reg(C* data, const C* twl, const C* tw, int
 // Prime radix P > 17: work item = (butterfly, transform, s in [0, H]) producing
 // y_s and y_{P-s}; results are held in registers across a barrier because the
 // DFT reads every input slot of its butterfly.
-template <int T, int DIR, class C>
+template <int T, int DIR, int NT, class C>
 __device__ __forceinline__ void pass_big(C* data, const C* twl, const C* tw, int n, int P, int L) {
     using R = decltype(C{}.x);
     const int H = (P - 1) / 2;
@@ -182,7 +191,7 @@ __device__ __forceinline__ void pass_big(C* data, const C* twl, const C* tw, int
     C ya[kMaxBigItems], yb[kMaxBigItems];
 #pragma unroll
     for (int c = 0; c < kMaxBigItems; ++c) {
-        const int it = threadIdx.x + c * kThreads;
+        const int it = threadIdx.x + c * NT;
         if (it >= items) break;
         const int s = it % (H + 1);
         const int rest = it / (H + 1);
@@ -229,7 +238,7 @@ __device__ __forceinline__ void pass_big(C* data, const C* twl, const C* tw, int
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kMaxBigItems; ++c) {
-        const int it = threadIdx.x + c * kThreads;
+        const int it = threadIdx.x + c * NT;
         if (it >= items) break;
         const int s = it % (H + 1);
         const int rest = it / (H + 1);
@@ -387,8 +396,9 @@ __device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, in
 // register DFT compiled in, so fewer registers and more waves per SIMD).
 constexpr int kLeanRadix = 7;
 
-template <int T, int DIR, int MF, class C>
+template <int T, int DIR, int MF, int NT, class C>
 __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d) {
+    static_assert(MF == 0 || NT == kThreads, "the matrix-core pass deals its tiles to four waves");
     const int n = d.n;
     for (int ps = 0; ps < d.npass; ++ps) {
         __syncthreads();
@@ -396,24 +406,24 @@ __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, 
         const C* tw = ps ? twp + (ps - 1) * n : nullptr;
         if constexpr (MF == 2) {
             switch (P) {
-                case 2: pass_reg<2, T, DIR>(data, twl, tw, n, L); break;
-                case 3: pass_reg<3, T, DIR>(data, twl, tw, n, L); break;
-                case 4: pass_reg<4, T, DIR>(data, twl, tw, n, L); break;
-                case 5: pass_reg<5, T, DIR>(data, twl, tw, n, L); break;
-                case 7: pass_reg<7, T, DIR>(data, twl, tw, n, L); break;
+                case 2: pass_reg<2, T, DIR, NT>(data, twl, tw, n, L); break;
+                case 3: pass_reg<3, T, DIR, NT>(data, twl, tw, n, L); break;
+                case 4: pass_reg<4, T, DIR, NT>(data, twl, tw, n, L); break;
+                case 5: pass_reg<5, T, DIR, NT>(data, twl, tw, n, L); break;
+                case 7: pass_reg<7, T, DIR, NT>(data, twl, tw, n, L); break;
                 default: pass_mfma<T, DIR>(data, twl, tw, n, P, L); break;
             }
             continue;
         }
         switch (P) {
-            case 2: pass_reg<2, T, DIR>(data, twl, tw, n, L); break;
-            case 3: pass_reg<3, T, DIR>(data, twl, tw, n, L); break;
-            case 4: pass_reg<4, T, DIR>(data, twl, tw, n, L); break;
-            case 5: pass_reg<5, T, DIR>(data, twl, tw, n, L); break;
-            case 7: pass_reg<7, T, DIR>(data, twl, tw, n, L); break;
-            case 11: pass_reg<11, T, DIR>(data, twl, tw, n, L); break;
-            case 13: pass_reg<13, T, DIR>(data, twl, tw, n, L); break;
-            case 17: pass_reg<17, T, DIR>(data, twl, tw, n, L); break;
+            case 2: pass_reg<2, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 3: pass_reg<3, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 4: pass_reg<4, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 5: pass_reg<5, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 7: pass_reg<7, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 11: pass_reg<11, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 13: pass_reg<13, T, DIR, NT>(data, twl, tw, n, L); break;
+            case 17: pass_reg<17, T, DIR, NT>(data, twl, tw, n, L); break;
             default:
                 if constexpr (MF) {
                     if (mfma_radix(P)) {
@@ -421,7 +431,7 @@ __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, 
                         break;
                     }
                 }
-                pass_big<T, DIR>(data, twl, tw, n, P, L);
+                pass_big<T, DIR, NT>(data, twl, tw, n, P, L);
                 break;
         }
     }
@@ -448,12 +458,12 @@ __device__ inline int xcd_tile(const PairMap& pm) {
 // A tile holds T consecutive columns f = h * ncol + c of the flattened
 // (latitude pair h, radial node c) index, so a tile may end one pair and
 // start the next and every tile but the last is full (SH grids: ncol = 1,
-// f = h). Every column a thread touches is threadIdx.x % T (kThreads is a
-// multiple of T), so each thread resolves its (h, c) once; h >= jh marks a
-// column past the end.
-template <int T>
+// f = h). Every column a thread touches is threadIdx.x % T (the thread count
+// NT is a multiple of T), so each thread resolves its (h, c) once; h >= jh
+// marks a column past the end.
+template <int T, int NT>
 __device__ inline void thread_column(const PairMap& pm, int tile, int& h, int& c) {
-    static_assert(kThreads % T == 0, "a thread keeps one column");
+    static_assert(NT % T == 0, "a thread keeps one column");
     const int f = tile * T + (int)threadIdx.x % T;
     h = f / pm.ncol;
     c = f - h * pm.ncol;
@@ -473,7 +483,7 @@ __host__ __device__ constexpr bool twiddles_in_lds(int T, size_t csize) {
     return T > (csize == 16 ? 8 : 4);
 }
 
-template <int T, class C>
+template <int T, int NT, class C>
 __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsigned short*& perm,
                                     const FftDesc& d, const C* twg, const C* twpg,
                                     const unsigned short* permg) {
@@ -484,8 +494,8 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
         C* tl = data + n * T;
         C* tp = tl + n;
         perm = reinterpret_cast<unsigned short*>(tp + (d.npass > 0 ? d.npass - 1 : 0) * n);
-        for (int i = threadIdx.x; i < n; i += kThreads) tl[i] = twg[i];
-        for (int i = threadIdx.x; i < (d.npass - 1) * n; i += kThreads) tp[i] = twpg[i];
+        for (int i = threadIdx.x; i < n; i += NT) tl[i] = twg[i];
+        for (int i = threadIdx.x; i < (d.npass - 1) * n; i += NT) tp[i] = twpg[i];
         twl = tl;
         twp = tp;
     } else {
@@ -493,17 +503,19 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
         twl = twg;
         twp = twpg;
     }
-    for (int i = threadIdx.x; i < n; i += kThreads) perm[i] = permg[i];
+    for (int i = threadIdx.x; i < n; i += NT) perm[i] = permg[i];
     __syncthreads();
 }
 
-// Waves per SIMD asked of the lean variant (3 in FP64, 4 in FP32); none of
-// the others.
-template <class R, int MF>
-constexpr int lean_waves() { return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : 1; }
+// Waves per SIMD asked of the lean variant (3 in FP64, 4 in FP32) and of the
+// 512-thread one (4: two workgroups per CU); none of the others.
+template <int T, class R, int MF>
+constexpr int lean_waves() {
+    return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : (fft_threads<T, R, MF>() > kThreads ? 4 : 1);
+}
 
 template <int T, int LK, class R, class G, int MF>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(lean_waves<R, MF>())))
+__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
 void fft_forward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
@@ -511,15 +523,16 @@ void fft_forward_kernel(
     long spec_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
+    constexpr int NT = fft_threads<T, R, MF>();
     const int tile = xcd_tile(pm);
     if (tile >= pm.ntiles) return;
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
-    stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
+    stage_tables<T, NT>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
     int th, tc;
-    thread_column<T>(pm, tile, th, tc);
+    thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
     const G* g = grid.at(blockIdx.y);
     C* sp = spec + (long)blockIdx.y * spec_stride;
@@ -527,11 +540,11 @@ void fft_forward_kernel(
     // Loads are issued in batches of kBatch per thread before any LDS store,
     // so each thread keeps that many independent global reads in flight.
     if (LK == GRID_COLS) {
-        for (int e0 = threadIdx.x; e0 < n * T; e0 += kThreads * kBatch) {
+        for (int e0 = threadIdx.x; e0 < n * T; e0 += NT * kBatch) {
             R av[kBatch], bv[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * kThreads;
+                const int e = e0 + u * NT;
                 const int i = e / T, h = th, c = tc;
                 av[u] = 0;
                 bv[u] = 0;
@@ -544,16 +557,16 @@ void fft_forward_kernel(
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * kThreads;
+                const int e = e0 + u * NT;
                 if (e < n * T) data[perm[e / T] * T + e % T] = mk<C>(av[u], bv[u]);
             }
         }
     } else {
-        for (int e0 = threadIdx.x; e0 < n * T; e0 += kThreads * kBatch) {
+        for (int e0 = threadIdx.x; e0 < n * T; e0 += NT * kBatch) {
             R av[kBatch], bv[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * kThreads;
+                const int e = e0 + u * NT;
                 const int t = e / n, i = e - t * n;
                 const int h = h0 + t;
                 av[u] = 0;
@@ -566,7 +579,7 @@ void fft_forward_kernel(
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * kThreads;
+                const int e = e0 + u * NT;
                 if (e < n * T) {
                     const int t = e / n, i = e - t * n;
                     data[perm[i] * T + t] = mk<C>(av[u], bv[u]);
@@ -575,9 +588,9 @@ void fft_forward_kernel(
         }
     }
 
-    run_passes<T, -1, MF>(data, twl, twp, d);
+    run_passes<T, -1, MF, NT>(data, twl, twp, d);
 
-    for (int e = threadIdx.x; e < nm * T; e += kThreads) {
+    for (int e = threadIdx.x; e < nm * T; e += NT) {
         const int m = e / T, t = e % T, h = th, c = tc;
         if (h >= pm.jh) continue;
         const C zm = data[m * T + t];
@@ -605,31 +618,32 @@ void fft_forward_kernel(
 }
 
 template <int T, int LK, class R, class G, int MF>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(lean_waves<R, MF>())))
+__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
 void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
     const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
     const typename Cplx<R>::type* __restrict__ spec, long spec_stride, Inst<G> grid, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
     const double* __restrict__ wrad, double center_factor) {
     using C = typename Cplx<R>::type;
+    constexpr int NT = fft_threads<T, R, MF>();
     const int tile = xcd_tile(pm);
     if (tile >= pm.ntiles) return;
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
-    stage_tables<T>(data, twl, twp, perm, d, twg, twpg, permg);
+    stage_tables<T, NT>(data, twl, twp, perm, d, twg, twpg, permg);
     const int n = d.n;
     int th, tc;
-    thread_column<T>(pm, tile, th, tc);
+    thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
     G* g = grid.at(blockIdx.y);
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
-    for (int e0 = threadIdx.x; e0 < F * T; e0 += kThreads * kBatch) {
+    for (int e0 = threadIdx.x; e0 < F * T; e0 += NT * kBatch) {
         C ae[kBatch], ao[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const int e = e0 + u * kThreads;
+            const int e = e0 + u * NT;
             const int m = e / T, h = th, c = tc;
             ae[u] = mk<C>(0, 0);
             ao[u] = mk<C>(0, 0);
@@ -640,7 +654,7 @@ void fft_backward_kernel(
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const int e = e0 + u * kThreads;
+            const int e = e0 + u * NT;
             if (e >= F * T) continue;
             const int m = e / T, t = e % T, h = th;
             const R mu = m > 0 ? R(0.5) : R(1);
@@ -665,10 +679,10 @@ void fft_backward_kernel(
         }
     }
 
-    run_passes<T, +1, MF>(data, twl, twp, d);
+    run_passes<T, +1, MF, NT>(data, twl, twp, d);
 
     if (LK == GRID_COLS) {
-        for (int e = threadIdx.x; e < n * T; e += kThreads) {
+        for (int e = threadIdx.x; e < n * T; e += NT) {
             const int i = e / T, t = e % T, h = th, c = tc;
             if (h >= pm.jh) continue;
             C z = data[i * T + t];
@@ -684,7 +698,7 @@ void fft_backward_kernel(
             if (js != jn) row[(long)js * pm.lat_stride] = (G)z.y;
         }
     } else {
-        for (int e = threadIdx.x; e < n * T; e += kThreads) {
+        for (int e = threadIdx.x; e < n * T; e += NT) {
             const int t = e / n, i = e - t * n;
             const int h = h0 + t;
             if (h >= pm.jh) continue;
@@ -776,7 +790,8 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
-    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
+    const int nt = f.mfma_pass ? kThreads : fft_threads<T, R, 0>();
+    k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
                                  L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
     check_hip(hipGetLastError(), "fft_forward launch");
@@ -799,7 +814,8 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
-    k<<<grd, kThreads, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
+    const int nt = f.mfma_pass ? kThreads : fft_threads<T, R, 0>();
+    k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
                                  pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
     check_hip(hipGetLastError(), "fft_backward launch");
