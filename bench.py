@@ -45,10 +45,10 @@ HBM_PEAK_GBS = 8000.0
 # -> (fetch bytes, write bytes). Reported as roofline.traffic only for that workload;
 # the commands and the full table are in profiles/zernike_n128_b64_kernels.md.
 MEASURED_TRAFFIC = {
-    ("fft_forward", 128, 64, "mixed_grid32"): (1189e6, 1162e6),
-    ("fft_forward", 128, 64, "fp64"): (2330e6, 2317e6),
-    ("fft_backward", 128, 64, "mixed_grid32"): (1168e6, 1190e6),
-    ("fft_backward", 128, 64, "fp64"): (2319e6, 2307e6),
+    ("fft_forward", 128, 64, "mixed_grid32"): (1192e6, 1160e6),
+    ("fft_forward", 128, 64, "fp64"): (2332e6, 2317e6),
+    ("fft_backward", 128, 64, "mixed_grid32"): (1168e6, 1157e6),
+    ("fft_backward", 128, 64, "fp64"): (2319e6, 2308e6),
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
