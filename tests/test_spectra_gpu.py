@@ -57,3 +57,34 @@ def test_strides(device):
     assert (o[:, nent:] == -5.0).all()
     for i in range(batch):
         assert rel(o[i, :nent], oracle.zt_power_spectrum(a[i, :nc], order)) < 1e-14
+
+
+@pytest.mark.parametrize("shift,pad", [(1, 0), (0, 3), (1, 5)])
+def test_unaligned_instances(device, shift, pad):
+    """Instance bases off 16-byte alignment (a base shifted by one double, or an
+    odd stride in doubles) take the 8-byte-load form; aligned ones the 16-byte
+    one. Both match the oracle, power and cross spectra."""
+    import torch
+    import oracle
+    import zest_amd
+    from zest_amd import synth
+    order, batch = 33, 3
+    nc = 2 * zest_amd.zt_num_coeffs(order)
+    stride = nc + pad
+    nent = (order + 1) ** 2 // 4
+    bufs = []
+    for seed in (0, 50):
+        flat = np.zeros(shift + batch * stride)
+        for i in range(batch):
+            flat[shift + i * stride: shift + i * stride + nc] = synth.zernike_coeffs(order, seed + i).ravel()
+        bufs.append(flat)
+    plan = zest_amd.Plan(zest_amd.ZERNIKE, 4)
+    da = torch.from_numpy(bufs[0]).to(device)[shift:]
+    db = torch.from_numpy(bufs[1]).to(device)[shift:]
+    p = plan.power_spectrum(da, order, batch=batch, coeff_stride=stride).cpu().numpy()
+    x = plan.power_spectrum(da, order, batch=batch, coeff_stride=stride, other=db).cpu().numpy()
+    for i in range(batch):
+        a_i = bufs[0][shift + i * stride: shift + i * stride + nc]
+        b_i = bufs[1][shift + i * stride: shift + i * stride + nc]
+        assert rel(p[i, :nent], oracle.zt_power_spectrum(a_i, order)) < 1e-14
+        assert rel(x[i, :nent], oracle.zt_power_spectrum(a_i, order, 0, b_i)) < 1e-14
