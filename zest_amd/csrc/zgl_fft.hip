@@ -15,14 +15,16 @@
 // of its mirror nth-1-j at the same radial node. The two real columns are
 // packed as one complex sequence z = a + i b (the two-for-one real FFT), so
 // one complex FFT yields both spectra, and the pair is exactly what the parity
-// split needs. For Zernike grids a tile holds T consecutive radial nodes of one
-// latitude pair, so its loads are contiguous runs of T doubles per longitude
-// row. The T sequences live in LDS as [n][T] complex values (lanes of a wave
-// touch consecutive columns: conflict-free) and are transformed in place by a
-// mixed-radix decimation-in-time FFT: the input is stored in digit-reversed
-// order during the load, and each pass of radix p multiplies its inputs by a
-// precomputed per-slot twiddle table (kept in LDS, so no index arithmetic
-// beyond the butterfly base) and runs p-point DFTs in registers (the
+// split needs. For Zernike grids a tile holds T consecutive entries of the
+// flattened (latitude pair, radial node) index, so its loads are contiguous
+// runs of up to T values per longitude row. The T sequences live in LDS as
+// [n][T] complex values (lanes of a wave touch consecutive columns:
+// conflict-free) and are transformed in place by a mixed-radix
+// decimation-in-time FFT: the input is stored in digit-reversed order during
+// the load, and each pass of radix p multiplies its inputs by a precomputed
+// per-slot twiddle table (in LDS for wide tiles, read through the caches for
+// narrow ones; no index arithmetic beyond the butterfly base) and runs p-point
+// DFTs in registers (the
 // symmetric cos/sin form for odd p). Prime radices 19..255 run on the matrix
 // cores (pass_mfma, in kernel variants of their own); larger primes use an
 // LDS-staged variant that computes two outputs per work item.
