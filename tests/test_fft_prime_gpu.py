@@ -144,3 +144,27 @@ def test_fp32_tile_widths(device, monkeypatch, kind, order, layout):
         print(f"{kind} {order} layout {layout} fp32 T=32: backward {_rel(g[b], g_ref):.2e}, "
               f"forward {_rel(co[b], c_ref):.2e}")
         assert _rel(g[b], g_ref) < 5e-6 and _rel(co[b], c_ref) < 5e-6
+
+
+def test_fp32_width32_headline_shape(device, monkeypatch):
+    """The headline shape (order 128, 2N-1 = 255 = 3 * 5 * 17, 136 radial nodes,
+    so width-32 tiles span latitude pairs): FP32 width 32 (512 threads) is
+    bit-identical to width 16 with FP32-stored grids too."""
+    import torch
+    import zest_amd
+    from zest_amd import synth
+    order, batch = 128, 2
+    c = torch.from_numpy(np.stack([synth.zernike_coeffs(order, 800 + b) for b in range(batch)])).to(device)
+    plan_shape = (batch,) + zest_amd.zt_shape(order)
+    res = {}
+    for tile in ("32", "16"):
+        monkeypatch.setenv("ZGL_FFT_TILE", tile)
+        plan = zest_amd.Plan(zest_amd.ZERNIKE, order, precision=zest_amd.FP32)
+        g = torch.empty(plan_shape, dtype=torch.float32, device=device)
+        co = torch.zeros_like(c)
+        plan.backward(c, g, order, batch=batch)
+        plan.forward(g, co, order, batch=batch)
+        torch.cuda.synchronize()
+        res[tile] = (g.cpu().numpy(), co.cpu().numpy())
+    assert np.array_equal(res["32"][0], res["16"][0])
+    assert np.array_equal(res["32"][1], res["16"][1])
