@@ -57,12 +57,12 @@ constexpr int kMaxRegRadix = 17;  // larger primes use the LDS-staged pass
 constexpr int kBatch = 8;         // global loads in flight per thread
 
 // Threads per workgroup of a kernel variant: 512 for FP32 tiles of 32 column
-// pairs without the matrix-core pass (their grid rows are read in 128-byte
+// pairs in the plain and lean variants (their grid rows are read in 128-byte
 // segments, and two workgroups per CU at n = 255 still make 16 waves), 256
 // otherwise.
 template <int T, class R, int MF>
 __host__ __device__ constexpr int fft_threads() {
-    return (sizeof(R) == 4 && T == 32 && MF == 0) ? 2 * kThreads : kThreads;
+    return (sizeof(R) == 4 && T == 32 && MF != 1) ? 2 * kThreads : kThreads;
 }
 
 template <class R> struct Cplx;
@@ -262,8 +262,8 @@ __device__ __forceinline__ void pass_big(C* data, const C* twl, const C* tw, int
 // combination as pass_big). Rows s and the contraction index r go to 16x16x4
 // MFMA tiles (cos and sin from the LDS twiddle table, index r*s mod P kept
 // incrementally); the columns are the (butterfly, sequence) pairs of the
-// pass, NC = (n / P) * T. A round computes at most 8 tiles (two per wave) --
-// every M block of up to 8 / MB column blocks -- holds them in registers
+// pass, NC = (n / P) * T. A round computes two tiles per wave (8 or 16) --
+// every M block of up to 2 waves / MB column blocks -- holds them in registers
 // across a barrier (the pass is in place) and writes them back. Each column's
 // arithmetic is the same whatever T, so results do not depend on the tile
 // width. The pass runs in its own kernel variants (its accumulators would
@@ -291,15 +291,16 @@ template <> struct PassMfma<float> {
     __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-template <int T, int DIR, class C>
+template <int T, int DIR, int NT, class C>
 __device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, int n, int P, int L) {
+    constexpr int NW = NT / 64;  // waves
     using R = decltype(C{}.x);
     using M = PassMfma<R>;
     using A = typename M::acc;
     const int H = (P - 1) / 2;
     const int NC = (n / P) * T;
     const int MB = (H + 16) >> 4, NB = (NC + 15) >> 4;
-    const int NBR = max(1, 4 * kMfmaTasks / MB);  // column blocks per round
+    const int NBR = max(1, NW * kMfmaTasks / MB);  // column blocks per round
     const int KS = (H + 4) >> 2;       // k-steps over r = 0..H
     const int stride = n / P;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -313,7 +314,7 @@ __device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, in
 #pragma unroll
         for (int q = 0; q < kMfmaTasks; ++q) {
             ar[q] = ai[q] = br[q] = bi[q] = A{0, 0, 0, 0};
-            const int task = wave + 4 * q;
+            const int task = wave + NW * q;
             const int mb = task % MB, nb = nb0 + task / MB;
             // A operand of this lane: row s, contraction index r = 4 kk + g
             sA[q] = 16 * mb + i;
@@ -331,7 +332,7 @@ __device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, in
             const int r = 4 * kk + g;
 #pragma unroll
             for (int q = 0; q < kMfmaTasks; ++q) {
-                if (wave + 4 * q >= ntask) break;  // uniform in the wave
+                if (wave + NW * q >= ntask) break;  // uniform in the wave
                 R ca = 0, sa = 0;
                 if (sA[q] <= H && r <= H) {
                     const C w = twl[idx[q] * stride];
@@ -362,7 +363,7 @@ __device__ __forceinline__ void pass_mfma(C* data, const C* twl, const C* tw, in
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kMfmaTasks; ++q) {
-            const int task = wave + 4 * q;
+            const int task = wave + NW * q;
             if (task >= ntask) break;
             const int mb = task % MB, nb = nb0 + task / MB;
             const int c = 16 * nb + i;  // output column of this lane
@@ -400,7 +401,6 @@ constexpr int kLeanRadix = 7;
 
 template <int T, int DIR, int MF, int NT, class C>
 __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, const FftDesc& d) {
-    static_assert(MF == 0 || NT == kThreads, "the matrix-core pass deals its tiles to four waves");
     const int n = d.n;
     for (int ps = 0; ps < d.npass; ++ps) {
         __syncthreads();
@@ -413,7 +413,7 @@ __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, 
                 case 4: pass_reg<4, T, DIR, NT>(data, twl, tw, n, L); break;
                 case 5: pass_reg<5, T, DIR, NT>(data, twl, tw, n, L); break;
                 case 7: pass_reg<7, T, DIR, NT>(data, twl, tw, n, L); break;
-                default: pass_mfma<T, DIR>(data, twl, tw, n, P, L); break;
+                default: pass_mfma<T, DIR, NT>(data, twl, tw, n, P, L); break;
             }
             continue;
         }
@@ -429,7 +429,7 @@ __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, 
             default:
                 if constexpr (MF) {
                     if (mfma_radix(P)) {
-                        pass_mfma<T, DIR>(data, twl, tw, n, P, L);
+                        pass_mfma<T, DIR, NT>(data, twl, tw, n, P, L);
                         break;
                     }
                 }
@@ -767,7 +767,7 @@ const typename Cplx<R>::type* twp_of(const FftPlan& f) {
 // The lean variant is instantiated for the widths prime lengths use (others
 // fall back to the general one, which computes the same values).
 template <int T>
-constexpr int lean_mf() { return (T == 8 || T == 16) ? 2 : 1; }
+constexpr int lean_mf() { return (T == 8 || T == 16 || T == 32) ? 2 : 1; }
 
 template <class K>
 K pick_variant(const FftPlan& f, K none, K general, K lean) {
@@ -792,7 +792,8 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
-    const int nt = f.mfma_pass ? kThreads : fft_threads<T, R, 0>();
+    const int nt = !f.mfma_pass ? fft_threads<T, R, 0>()
+                   : f.lean ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
     k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
                                  L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
@@ -816,7 +817,8 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
-    const int nt = f.mfma_pass ? kThreads : fft_threads<T, R, 0>();
+    const int nt = !f.mfma_pass ? fft_threads<T, R, 0>()
+                   : f.lean ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
     k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
                                  pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
