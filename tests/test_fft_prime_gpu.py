@@ -35,9 +35,11 @@ def _roundtrip(device, kind, order, c, layout=0, precision=0):
 # 2N-1: 19, 31, 127, 3 * 59, 5 * 37, 191 and 11 * 19 (the general kernel
 # variant; the others run the lean one, whose other radices are at most 7);
 # SH 2L-1 = 31, 199 and 437 = 23 * 19 (two matrix-core passes, the second
-# with input twiddles)
+# with input twiddles); SH 263 and 383, primes above 255 (the 512-thread
+# variant)
 CASES = [("zernike", 10), ("zernike", 16), ("zernike", 64), ("zernike", 89), ("zernike", 93),
-         ("zernike", 96), ("zernike", 105), ("sh", 16), ("sh", 100), ("sh", 219)]
+         ("zernike", 96), ("zernike", 105), ("sh", 16), ("sh", 100), ("sh", 219),
+         ("sh", 132), ("sh", 192)]
 
 
 @pytest.mark.parametrize("kind,order", CASES)
@@ -168,3 +170,30 @@ def test_fp32_width32_headline_shape(device, monkeypatch):
         res[tile] = (g.cpu().numpy(), co.cpu().numpy())
     assert np.array_equal(res["32"][0], res["16"][0])
     assert np.array_equal(res["32"][1], res["16"][1])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_wide_prime_zernike(device, monkeypatch, precision):
+    """Order 192 (2N-1 = 383, a prime above 255): the matrix-core pass on eight
+    waves against the VALU pass (ZGL_FFT_MFMA=0) to rounding level, FP64 and
+    FP32; FP32 widths 16, 8 and 4 bit-identical."""
+    from zest_amd import synth
+    order = 192
+    c = np.stack([synth.zernike_coeffs(order, 900 + b) for b in range(2)])
+    g_m, c_m = _roundtrip(device, "zernike", order, c, precision=precision)
+    monkeypatch.setenv("ZGL_FFT_MFMA", "0")
+    g_v, c_v = _roundtrip(device, "zernike", order, c, precision=precision)
+    tol = 1e-13 if precision == 0 else 1e-5
+    for b in range(2):
+        print(f"order {order} precision {precision}: grid {_rel(g_m[b], g_v[b]):.2e}, "
+              f"coefficients {_rel(c_m[b], c_v[b]):.2e} vs the VALU pass")
+        assert _rel(g_m[b], g_v[b]) <= tol and _rel(c_m[b], c_v[b]) <= tol
+    if precision == 1:
+        monkeypatch.delenv("ZGL_FFT_MFMA")
+        res = {}
+        for tile in ("16", "8", "4"):
+            monkeypatch.setenv("ZGL_FFT_TILE", tile)
+            res[tile] = _roundtrip(device, "zernike", order, c[:1], precision=precision)
+        for tile in ("8", "4"):
+            assert np.array_equal(res[tile][0], res["16"][0]), tile
+            assert np.array_equal(res[tile][1], res["16"][1]), tile

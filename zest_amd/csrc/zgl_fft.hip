@@ -62,7 +62,7 @@ constexpr int kBatch = 8;         // global loads in flight per thread
 // otherwise.
 template <int T, class R, int MF>
 __host__ __device__ constexpr int fft_threads() {
-    return (sizeof(R) == 4 && T == 32 && MF != 1) ? 2 * kThreads : kThreads;
+    return (MF == 3 || (sizeof(R) == 4 && T == 32 && MF != 1)) ? 2 * kThreads : kThreads;
 }
 
 template <class R> struct Cplx;
@@ -272,6 +272,12 @@ constexpr int kMfmaTasks = 2;       // tiles per wave and round
 constexpr int kMaxMfmaRadix = 255;  // MB = ceil((H + 1) / 16) <= 4 * kMfmaTasks
 
 __host__ __device__ inline bool mfma_radix(int P) { return P > kMaxRegRadix && P <= kMaxMfmaRadix; }
+// Primes up to 511 (MB <= 16) fit the pass on eight waves: the 512-thread
+// lean variant (MF = 3) of the widths wide_mf gives it.
+constexpr int kMaxWideRadix = 511;
+__host__ __device__ inline bool wide_radix(int P) { return P > kMaxMfmaRadix && P <= kMaxWideRadix; }
+template <int T>
+constexpr int wide_mf() { return (T == 4 || T == 8 || T == 16) ? 3 : 1; }
 
 template <class R> struct PassMfma;
 template <> struct PassMfma<double> {
@@ -406,7 +412,7 @@ __device__ __forceinline__ void run_passes(C* data, const C* twl, const C* twp, 
         __syncthreads();
         const int P = d.radix[ps], L = d.L[ps];
         const C* tw = ps ? twp + (ps - 1) * n : nullptr;
-        if constexpr (MF == 2) {
+        if constexpr (MF >= 2) {
             switch (P) {
                 case 2: pass_reg<2, T, DIR, NT>(data, twl, tw, n, L); break;
                 case 3: pass_reg<3, T, DIR, NT>(data, twl, tw, n, L); break;
@@ -513,7 +519,7 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
 // 512-thread one (4: two workgroups per CU); none of the others.
 template <int T, class R, int MF>
 constexpr int lean_waves() {
-    return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : (fft_threads<T, R, MF>() > kThreads ? 4 : 1);
+    return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : (fft_threads<T, R, MF>() > kThreads ? (sizeof(R) == 8 ? 2 : 4) : 1);
 }
 
 template <int T, int LK, class R, class G, int MF>
@@ -770,8 +776,9 @@ template <int T>
 constexpr int lean_mf() { return (T == 8 || T == 16 || T == 32) ? 2 : 1; }
 
 template <class K>
-K pick_variant(const FftPlan& f, K none, K general, K lean) {
+K pick_variant(const FftPlan& f, K none, K general, K lean, K wide) {
     if (!f.mfma_pass) return none;
+    if (f.wide) return wide;
     return f.lean ? lean : general;
 }
 
@@ -785,15 +792,18 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     dim3 grd(grid_tiles(pm), nb);
     auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_forward_kernel<T, GRID_COLS, R, G, 0>,
                                                       fft_forward_kernel<T, GRID_COLS, R, G, 1>,
-                                                      fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>)
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>,
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, wide_mf<T>()>)
                                         : pick_variant(f, fft_forward_kernel<T, GRID_ROWS, R, G, 0>,
                                                       fft_forward_kernel<T, GRID_ROWS, R, G, 1>,
-                                                      fft_forward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>);
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>,
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, wide_mf<T>()>);
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
     const int nt = !f.mfma_pass ? fft_threads<T, R, 0>()
-                   : f.lean ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
+                   : f.wide   ? fft_threads<T, R, wide_mf<T>()>()
+                   : f.lean   ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
     k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
                                  L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
@@ -810,15 +820,18 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     dim3 grd(grid_tiles(pm), nb);
     auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_backward_kernel<T, GRID_COLS, R, G, 0>,
                                                       fft_backward_kernel<T, GRID_COLS, R, G, 1>,
-                                                      fft_backward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>)
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>,
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, wide_mf<T>()>)
                                         : pick_variant(f, fft_backward_kernel<T, GRID_ROWS, R, G, 0>,
                                                       fft_backward_kernel<T, GRID_ROWS, R, G, 1>,
-                                                      fft_backward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>);
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>,
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, wide_mf<T>()>);
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
     const int nt = !f.mfma_pass ? fft_threads<T, R, 0>()
-                   : f.lean ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
+                   : f.wide   ? fft_threads<T, R, wide_mf<T>()>()
+                   : f.lean   ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
     k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
                                  pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
                                  L.center_factor);
@@ -853,11 +866,12 @@ void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> gr
 
 // Whether T column pairs per workgroup fit the LDS and the per-thread item
 // budget of the LDS-staged large-prime pass.
-bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf) {
+bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf, bool wide = false) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     if (lds_bytes(n, npass, T, csize) > limit) return false;
+    const bool wide_t = wide && (T == 4 || T == 8 || T == 16);  // widths with wide_mf<T>() == 3
     for (int k = 0; k < npass; ++k)
-        if (rad[k] > kMaxRegRadix && !(mf && mfma_radix(rad[k]))) {
+        if (rad[k] > kMaxRegRadix && !(mf && mfma_radix(rad[k])) && !(wide_t && wide_radix(rad[k]))) {
             const long items = (long)(n / rad[k]) * T * ((rad[k] - 1) / 2 + 1);
             if ((items + kThreads - 1) / kThreads > kMaxBigItems) return false;
         }
@@ -866,15 +880,15 @@ bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf) {
 
 // The widest tile (at most 16) that leaves room for two workgroups per CU
 // (80 KB of LDS each), else the widest that fits at all.
-[[maybe_unused]] int choose_tile(int n, const std::vector<int>& rad, size_t csize, bool mf) {
+[[maybe_unused]] int choose_tile(int n, const std::vector<int>& rad, size_t csize, bool mf, bool wide) {
     const size_t limit = (size_t)std::min(lds_limit(), 160 * 1024);
     const int npass = (int)rad.size();
     for (int T = 16; T >= 1; T /= 2)
-        if (lds_bytes(n, npass, T, csize) <= 80 * 1024 && tile_fits(n, rad.data(), npass, T, csize, mf))
+        if (lds_bytes(n, npass, T, csize) <= 80 * 1024 && tile_fits(n, rad.data(), npass, T, csize, mf, wide))
             return T;
     int T = 16;
     for (; T > 1; T /= 2)
-        if (tile_fits(n, rad.data(), npass, T, csize, mf)) break;
+        if (tile_fits(n, rad.data(), npass, T, csize, mf, wide)) break;
     if (lds_bytes(n, npass, T, csize) > limit)
         throw Error{ZGL_ERROR_NOT_SUPPORTED, "longitude count too large for the LDS"};
     return T;
@@ -897,9 +911,10 @@ std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
     std::vector<int> out;
     const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
     const int T0 = p.tile[fp32 ? 1 : 0];
-    if (T0 < 32 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 2 * T0, cs, p.mfma_pass)) out.push_back(2 * T0);
+    if (T0 < 32 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 2 * T0, cs, p.mfma_pass, p.wide))
+        out.push_back(2 * T0);
     for (int T = T0; T >= 1 && out.size() < 3; T /= 2)
-        if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs, p.mfma_pass)) out.push_back(T);
+        if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs, p.mfma_pass, p.wide)) out.push_back(T);
     return out;
 }
 
@@ -926,22 +941,30 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
         p.desc.L[i] = L;
         L *= rad[i];
     }
-    // prime radices in (17, kMaxMfmaRadix] run on the matrix cores unless
-    // ZGL_FFT_MFMA=0 (comparison runs)
-    for (int r : rad) p.mfma_pass |= mfma_radix(r);
+    // prime radices in (17, kMaxMfmaRadix] run on the matrix cores, and those
+    // in (kMaxMfmaRadix, kMaxWideRadix] too when every other radix is lean
+    // (the 512-thread variant), unless ZGL_FFT_MFMA=0 (comparison runs)
+    bool has_wide = false, lean_all = true;
+    for (int r : rad) {
+        p.mfma_pass |= mfma_radix(r);
+        has_wide |= wide_radix(r);
+        lean_all &= r <= kLeanRadix || mfma_radix(r) || wide_radix(r);
+    }
+    p.wide = has_wide && lean_all;
+    p.mfma_pass |= p.wide;
     if (const char* e = std::getenv("ZGL_FFT_MFMA"))
-        if (e[0] == '0') p.mfma_pass = false;
+        if (e[0] == '0') p.mfma_pass = p.wide = false;
     p.lean = p.mfma_pass;
-    for (int r : rad) p.lean &= r <= kLeanRadix || mfma_radix(r);
+    for (int r : rad) p.lean &= r <= kLeanRadix || mfma_radix(r) || (p.wide && wide_radix(r));
     const bool mf = p.mfma_pass;
-    p.tile[0] = choose_tile(n, rad, sizeof(double2), mf);
-    p.tile[1] = choose_tile(n, rad, sizeof(float2), mf);
+    p.tile[0] = choose_tile(n, rad, sizeof(double2), mf, p.wide);
+    p.tile[1] = choose_tile(n, rad, sizeof(float2), mf, p.wide);
     // ZGL_FFT_TILE=T forces a width where it fits (comparison and test runs)
     if (const char* e = std::getenv("ZGL_FFT_TILE")) {
         const int T = std::atoi(e);
         if (T == 1 || T == 2 || T == 4 || T == 8 || T == 16 || T == 32) {
-            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(double2), mf)) p.tile[0] = T;
-            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(float2), mf)) p.tile[1] = T;
+            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(double2), mf, p.wide)) p.tile[0] = T;
+            if (tile_fits(n, rad.data(), (int)rad.size(), T, sizeof(float2), mf, p.wide)) p.tile[1] = T;
         }
     }
 

@@ -44,6 +44,7 @@ struct FftPlan {
     int tile[2] = {16, 16};          // column pairs per workgroup: FP64, FP32
     bool mfma_pass = false;          // a prime radix runs on the matrix cores
     bool lean = false;               // ... and every other radix is at most 7
+    bool wide = false;               // a lean plan with a prime radix in (255, 511]: 512 threads
 };
 
 // Geometry of one order for a plan, both transform kinds.
