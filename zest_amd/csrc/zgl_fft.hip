@@ -26,8 +26,9 @@
 // narrow ones; no index arithmetic beyond the butterfly base) and runs p-point
 // DFTs in registers (the
 // symmetric cos/sin form for odd p). Prime radices 19..255 run on the matrix
-// cores (pass_mfma, in kernel variants of their own); larger primes use an
-// LDS-staged variant that computes two outputs per work item.
+// cores (pass_mfma, in kernel variants of their own), and 257..511 too in
+// lean plans (a 512-thread variant); larger primes use an LDS-staged variant
+// that computes two outputs per work item.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
