@@ -103,6 +103,10 @@ struct zgl_plan {
             hipGraphExec_t exec = nullptr;
             long* offs = nullptr;  // device: grid offsets [njobs], coefficient offsets [njobs]
             size_t offs_cap = 0;   // longs
+            // recorded on the caller's stream after each launch that reads
+            // `offs` or runs `exec`: both are rewritten or destroyed only once
+            // that work has completed
+            hipEvent_t done = nullptr;
         } cache[2];
         std::vector<double> lane_cost;  // modelled work per lane of the last build
     } rg;
@@ -165,9 +169,11 @@ void ensure(T*& ptr, size_t& cap, size_t need) {
 
 void ragged_release(zgl_plan::Ragged& r) {
     for (auto& c : r.cache) {
+        if (c.done) (void)hipEventSynchronize(c.done);
         if (c.exec) (void)hipGraphExecDestroy(c.exec);
         if (c.graph) (void)hipGraphDestroy(c.graph);
         if (c.offs) (void)hipFree(c.offs);
+        if (c.done) (void)hipEventDestroy(c.done);
     }
     for (auto l : r.lanes)
         if (l) (void)hipStreamDestroy(l);
@@ -223,12 +229,13 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
 
 enum Stage { ST_FFT_FWD, ST_LEG_FWD, ST_RAD_FWD, ST_RAD_BWD, ST_LEG_BWD, ST_FFT_BWD,
              ST_EVAL_TABLES, ST_EVAL_RAD, ST_EVAL_LEG, ST_EVAL_LON, ST_ROTATE, ST_ROTATE_WIDE, ST_POLAR,
-             ST_SPECTRUM, ST_COUNT };
+             ST_SPECTRUM, ST_FUSED_FWD, ST_FUSED_BWD, ST_COUNT };
 const char* const kStageNames[ST_COUNT] = {"fft_forward", "legendre_forward", "radial_forward",
                                            "radial_backward", "legendre_backward", "fft_backward",
                                            "eval_tables", "eval_radial", "eval_legendre",
                                            "eval_longitude", "rotate", "rotate_wide", "polar_rotate",
-                                           "power_spectrum"};
+                                           "power_spectrum", "legendre_radial_forward",
+                                           "radial_legendre_backward"};
 
 hipEvent_t pooled_event(zgl_plan* p) {
     if (!p->pool.empty()) {
@@ -331,6 +338,14 @@ int legendre_math(const zgl_plan* p) {
     return sizeof(T) == 4 && p->precision == ZGL_BF16X3 ? kGroupX3 : 0;
 }
 
+// The fused FP32 colatitude + radial kernels replace the two stage kernels
+// for FP32 arithmetic at orders up to 128 (not for the BF16x3 mode, whose
+// colatitude products are formed differently).
+template <class T>
+bool use_fused(const zgl_plan* p, const OrderTables& t) {
+    return sizeof(T) == 4 && p->precision != ZGL_BF16X3 && t.rfuse_f && zt_fused_fits(t);
+}
+
 // T: arithmetic (double or float); G: grid element type in memory.
 // adjoint: the transpose of the backward transform (no quadrature weights,
 // no prefactor, no kappa_n), otherwise the forward transform.
@@ -361,7 +376,11 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
                                   fft_forward<T, G>(L2, grid.from(b0), sc.spec, ch.spec_stride, nb, s);
                               });
         timed(p, ST_FFT_FWD, nb, s, [&] { fft_forward<T, G>(L, grid.from(b0), sc.spec, ch.spec_stride, nb, s); });
-        if (p->kind == ZGL_ZERNIKE) {
+        if (p->kind == ZGL_ZERNIKE && use_fused<T>(p, t)) {
+            ZtGemmArgs a{&t, (int)nt, kappa, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride,
+                         coeffs.from(b0), nb};
+            timed(p, ST_FUSED_FWD, nb, s, [&] { zt_fused_forward(a, s); });
+        } else if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, kappa, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride,
                          coeffs.from(b0), nb};
             a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_FWD, s, legendre_candidates<T>(p), [&](int c) {
@@ -417,7 +436,10 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
             coeff_kappa_scale(cin, sc.rcoef, ncw, (int)nt, 1, nb, s);
             cin = Inst<double>{sc.rcoef, ncw, nullptr};
         }
-        if (p->kind == ZGL_ZERNIKE) {
+        if (p->kind == ZGL_ZERNIKE && use_fused<T>(p, t)) {
+            ZtGemmArgs a{&t, (int)nt, 0, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride, cin, nb};
+            timed(p, ST_FUSED_BWD, nb, s, [&] { zt_fused_backward(a, s); });
+        } else if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, 0, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride, cin, nb};
             a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_BWD, s, kGroupCandidates, [&](int c) {
                 ZtGemmArgs b = a;
@@ -749,7 +771,19 @@ double job_cost(size_t N, size_t count) {
     return (double)count * n * n * n;
 }
 
+// Waits for the last launch that used the cache (its offsets and graph).
+void wait_cache(zgl_plan::Ragged::Cached& c) {
+    if (c.done) check_hip(hipEventSynchronize(c.done), "ragged: wait for the previous call");
+}
+
+// Marks the end of a launch that uses the cache on stream s.
+void mark_cache(zgl_plan::Ragged::Cached& c, hipStream_t s) {
+    if (!c.done) check_hip(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), "ragged event");
+    check_hip(hipEventRecord(c.done, s), "ragged event");
+}
+
 void drop_graph(zgl_plan::Ragged::Cached& c) {
+    wait_cache(c);
     if (c.exec) (void)hipGraphExecDestroy(c.exec);
     if (c.graph) (void)hipGraphDestroy(c.graph);
     c.exec = nullptr;
@@ -801,8 +835,11 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         const bool use_graph = s != nullptr && !plan->profiling && ragged_graphs();
         if (use_graph && C.valid && same_jobs(C.jobs, jobs, njobs)) {
             check_hip(hipGraphLaunch(C.exec, s), "ragged graph launch");
+            mark_cache(C, s);
             return ZGL_SUCCESS;
         }
+        // the previous call of this direction may still be reading C.offs or
+        // running C.exec: drop_graph waits for it before either changes
         drop_graph(C);
 
         // order-sorted queue of non-empty jobs, largest order first
@@ -881,6 +918,7 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         if (!use_graph) {
             enqueue();
             check_hip(hipGetLastError(), "ragged");
+            mark_cache(C, s);
             return ZGL_SUCCESS;
         }
         check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
@@ -897,6 +935,7 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         C.jobs.assign(jobs, jobs + njobs);
         C.valid = true;
         check_hip(hipGraphLaunch(C.exec, s), "ragged graph launch");
+        mark_cache(C, s);
         return ZGL_SUCCESS;
     });
 }
@@ -1049,6 +1088,12 @@ extern "C" zgl_status zgl_evaluate(zgl_plan* plan, const double* coeffs, size_t 
     const size_t kMaxPoints = (size_t)1 << 30;
     if (nlon > kMaxPoints || nlat > kMaxPoints || nrad > kMaxPoints || batch > kMaxPoints)
         return fail(ZGL_ERROR_INVALID_VALUE, "point or instance count too large");
+    // the point count must fit the kernels' signed 64-bit indexing with room
+    // for the instance offsets: at most 2^40 points per instance (each factor
+    // is below 2^30, so the checks themselves cannot overflow)
+    const size_t kMaxGrid = (size_t)1 << 40;
+    if (nlon && nlat && (nlat > kMaxGrid / nlon || (nrad && nrad > kMaxGrid / (nlon * nlat))))
+        return fail(ZGL_ERROR_INVALID_VALUE, "evaluation grid has too many points");
     const long npts = (long)(nlon * nlat * nrad);
     const long csz = 2 * (long)coeff_size(plan, exp_order);
     const long cs = coeff_stride ? (long)coeff_stride : csz;

@@ -79,6 +79,14 @@ struct OrderTables {
     int4* work_rf = nullptr; int n_rf = 0;  // forward radial
     int4* work_rb = nullptr; int n_rb = 0;  // backward radial
     int4* work_sb = nullptr; int n_sb = 0;  // backward Legendre, streaming kernel (Zernike)
+    // fused FP32 colatitude + radial kernels (Zernike, orders <= 128): R_nl(r_k)
+    // by 16-node chunk, [l][kc][d][16] (forward) and [l][kc][16][d] (backward),
+    // n = l + 2d, d < fuse_dp; (m, par) problems, costliest first
+    float* rfuse_f = nullptr;
+    float* rfuse_b = nullptr;
+    int fuse_nkc = 0, fuse_dp = 0;
+    int2* work_ff = nullptr; int n_ff = 0;
+    int2* work_fb = nullptr; int n_fb = 0;
 };
 
 // Rows per GEMM tile are 16 * NS; the work lists step by that many rows.
@@ -187,6 +195,13 @@ template <class T> bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStrea
 template <class T> bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s);
 template <class T> bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s);
 template <class T> bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s);
+// Fused FP32 colatitude + radial stages (zgl_fused.hip), orders <= 128:
+// false when the order does not fit (nothing launched); on only with
+// ZGL_FUSED=1 at table build and call time.
+bool zt_fused_fits(const OrderTables& t);
+void zt_fused_tables(OrderTables& t, hipStream_t s);
+bool zt_fused_forward(const ZtGemmArgs& a, hipStream_t s);
+bool zt_fused_backward(const ZtGemmArgs& a, hipStream_t s);
 // ZGL_LEGENDRE_ENGINE=tiled selects the tiled LDS engine for every order and
 // stage.
 bool legendre_stream_enabled();

@@ -317,6 +317,7 @@ void ensure_fp32_tables(OrderTables& t, hipStream_t s) {
     }
     check_hip(hipGetLastError(), "fp32 tables");
     check_hip(hipStreamSynchronize(s), "fp32 tables");
+    if (t.rtab) zt_fused_tables(t, s);
 }
 
 void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, int sh_phase,
@@ -442,7 +443,8 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
 void free_tables(OrderTables& t) {
     for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.zh, (void*)t.rtab, (void*)t.ptab32,
                     (void*)t.rtab32, (void*)t.work_lf, (void*)t.work_lb, (void*)t.work_rf,
-                    (void*)t.work_rb, (void*)t.work_sb})
+                    (void*)t.work_rb, (void*)t.work_sb, (void*)t.rfuse_f, (void*)t.rfuse_b,
+                    (void*)t.work_ff, (void*)t.work_fb})
         if (p) (void)hipFree(p);
     fft_plan_destroy(t.fft);
     t = OrderTables{};
