@@ -32,7 +32,9 @@
  * With ZGL_MEM_DEVICE the work is enqueued on `stream` and the call returns
  * without waiting; with ZGL_MEM_HOST the call is synchronous, as zest is.
  * A plan must not be used from two threads at once (zest's instances are not
- * thread-safe either); distinct plans are independent.
+ * thread-safe either); distinct plans are independent. A plan's device calls
+ * share its scratch buffers, so they must be ordered: issue them on one stream,
+ * or synchronize between calls on different streams.
  */
 #ifndef ZGL_H
 #define ZGL_H

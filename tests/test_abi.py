@@ -107,3 +107,38 @@ def test_euler_angles_from_rotation_matrix_matches_zest():
         assert np.abs(zest_amd.euler_angles_from_rotation_matrix(q) - oracle.ref_euler_angles(q)).max() < 1e-15
     a = zest_amd.euler_angles_from_rotation_matrix(np.eye(3))
     assert a[1] == 0.0
+
+
+def test_binding_rejects_undersized_buffers():
+    """The binding checks every buffer against the batch, strides and orders
+    it passes to the C ABI and raises before any call (no device needed: an
+    order-0 plan builds no tables)."""
+    import zest_amd
+    zt = zest_amd.Plan(zest_amd.ZERNIKE, 0)
+    g, c = np.zeros(zest_amd.zt_shape(8)), np.zeros((zest_amd.zt_num_coeffs(8), 2))
+    small_g, small_c = np.zeros(g.size - 1), np.zeros(c.size - 2)
+    for call in (lambda: zt.forward(small_g, c, 8), lambda: zt.forward(g, small_c, 8),
+                 lambda: zt.backward(small_c, g, 8), lambda: zt.backward(c, small_g, 8),
+                 lambda: zt.forward(g, c, 8, batch=2),
+                 lambda: zt.backward(c, g, 8, batch=2),
+                 lambda: zt.backward_adjoint(g, small_c, 8),
+                 lambda: zt.forward_adjoint(small_c, g, 8),
+                 lambda: zt.forward(np.zeros(2 * g.size), np.zeros(2 * c.size), 8, batch=2,
+                                    grid_stride=g.size + 1),
+                 lambda: zt.power_spectrum(small_c, 8),
+                 lambda: zt.power_spectrum(c, 8, out=np.zeros(3)),
+                 lambda: zt.rotate(small_c, 8, np.zeros(3)),
+                 lambda: zt.rotate(c, 8, np.zeros(2)),
+                 lambda: zt.polar_rotate(c, 8, np.zeros(1), batch=2),
+                 lambda: zt.evaluate(small_c, 8, np.zeros(3), np.zeros(2), np.zeros(2)),
+                 lambda: zt.evaluate(c, 8, np.zeros(3), np.zeros(2), np.zeros(2), values=np.zeros(11))):
+        with pytest.raises(ValueError):
+            call()
+    # truncated forward: only the min(exp_order, grid_order) prefix is written
+    zt._check_sizes(g, np.zeros((zest_amd.zt_num_coeffs(4), 2)), 8, 4, 1, 0, 0)
+    sh = zest_amd.Plan(zest_amd.SH, 0, layout=1)
+    gs = np.zeros(zest_amd.st_shape(16, 1))
+    with pytest.raises(ValueError):
+        sh.forward(gs[:-1], np.zeros((zest_amd.st_num_coeffs(16), 2)), 16)
+    with pytest.raises(ValueError):
+        sh._jobs([(16, np.zeros(3), np.zeros(3))])

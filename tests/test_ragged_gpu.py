@@ -136,8 +136,39 @@ def test_ragged_rejects_bad_jobs(device):
     import zest_amd
     plan = zest_amd.Plan(zest_amd.ZERNIKE, 4)
     c = torch.zeros((zest_amd.zt_num_coeffs(4), 2), dtype=torch.float64, device=device)
-    with pytest.raises(zest_amd.ZglError):
+    # the binding checks the tensors against each job's order
+    with pytest.raises(ValueError):
         plan.forward_ragged([(4, None, c)])
-    with pytest.raises(zest_amd.ZglError):
+    with pytest.raises(ValueError):
         plan.forward_ragged([(1 << 20, c, c)])
+    # the C ABI's own checks, reached through a raw job array
+    with pytest.raises(zest_amd.ZglError):
+        plan.forward_ragged(zest_amd.make_jobs([(4, None, c)]))
+    with pytest.raises(zest_amd.ZglError):
+        plan.forward_ragged(zest_amd.make_jobs([(1 << 20, c, c)]))
     plan.forward_ragged([])  # nothing to do
+
+
+def test_ragged_job_lists_back_to_back_without_sync(device):
+    """Two different job lists on one (non-blocking) stream with no host
+    synchronisation between them: the second call changes the device offset
+    table and drops the first call's graph only after the first call's work
+    has completed, so both results are bit-identical to per-order calls."""
+    import torch
+    import zest_amd
+    plan = zest_amd.Plan(zest_amd.ZERNIKE, 8)
+    stream = torch.cuda.Stream()
+    lists = ([5, 9, 17, 9, 3, 17, 12], [20, 4, 4, 11, 20, 7])
+    made = [_make_jobs(device, orders, 500 + 50 * k) for k, orders in enumerate(lists)]
+    outs = [[(N, g, torch.zeros_like(c)) for N, g, c in jobs] for jobs, _ in made]
+    with torch.cuda.stream(stream):
+        for rep in range(2):  # the second pass replays both graphs in turn
+            for (jobs, _), out in zip(made, outs):
+                plan.backward_ragged(jobs, stream=stream)
+                plan.forward_ragged(out, stream=stream)
+    torch.cuda.synchronize()
+    for (jobs, coeffs), out, orders in zip(made, outs, lists):
+        ref_grids, ref_out = _plain(device, plan, orders, coeffs)
+        for i, (N, g, c) in enumerate(out):
+            assert np.array_equal(g.cpu().numpy(), ref_grids[i]), (i, N)
+            assert np.array_equal(c.cpu().numpy(), ref_out[i]), (i, N)

@@ -2,8 +2,9 @@
 harmonic transforms.
 
 The product is ``libzgl.so`` -- hand-written HIP kernels for gfx950 behind the
-C ABI declared in ``include/zgl.h`` -- together with the C++20 drop-in header
-``include/zest_amd/glq_transformer.hpp``. This module is a thin ctypes binding
+C ABI declared in ``include/zgl.h`` -- together with the C++20 drop-in headers
+``include/zest_amd/zernike_glq_transformer.hpp`` and
+``include/zest_amd/sh_glq_transformer.hpp``. This module is a thin ctypes binding
 of that C ABI used by the tests and the benchmark; it moves no data itself and
 has no CPU fallback: importing it without the built library raises.
 
@@ -159,6 +160,23 @@ def st_num_coeffs(order: int) -> int:
     return order * (order + 1) // 2
 
 
+def _numel(x) -> int:
+    return int(x.size) if isinstance(x, np.ndarray) else int(x.numel())
+
+
+def _need(x, name: str, batch: int, stride: int, per: int):
+    """Raise ValueError unless `x` holds `batch` instances of `per` elements
+    `stride` elements apart (the last one needs only `per`)."""
+    if batch <= 0 or per <= 0:
+        return
+    if stride < 0:
+        raise ValueError(f"{name}: negative instance stride")
+    need = (batch - 1) * stride + per
+    if _numel(x) < need:
+        raise ValueError(f"{name} holds {_numel(x)} elements, {need} needed "
+                         f"({batch} instances of {per}, {stride} apart)")
+
+
 def _ptr_and_mem(x, allow_f32: bool = False):
     """(pointer, memory kind, is_float32) of a torch tensor or numpy array."""
     if isinstance(x, np.ndarray):
@@ -243,6 +261,14 @@ class Plan:
         return {arr[i].name.decode(): (arr[i].launches, arr[i].instances, arr[i].total_ms)
                 for i in range(min(n.value, 16))}
 
+    def _check_sizes(self, grid, coeffs, grid_order, exp_order, batch, grid_stride, coeff_stride):
+        """The buffers hold `batch` instances at the strides the C ABI will use."""
+        nt = min(exp_order, grid_order)
+        gsz = self.grid_size(grid_order)
+        _need(grid, "grid", batch, grid_stride or gsz, gsz)
+        _need(coeffs, "coefficients", batch, coeff_stride or 2 * self.coeff_size(exp_order),
+              2 * self.coeff_size(nt))
+
     def grid_shape(self, order: int):
         return zt_shape(order) if self.kind == ZERNIKE else st_shape(order, self.layout)
 
@@ -255,6 +281,7 @@ class Plan:
         cp, cm, _ = _ptr_and_mem(coeffs)
         if gm != cm:
             raise ValueError("grid and coefficients must live in the same memory")
+        self._check_sizes(grid, coeffs, grid_order, exp_order, batch, grid_stride, coeff_stride)
         fn = LIB.zgl_forward_grid32 if g32 else LIB.zgl_forward
         _check(fn(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
                   coeff_stride, gm, _stream_ptr(stream, gm)))
@@ -268,6 +295,7 @@ class Plan:
         gp, gm, g32 = _ptr_and_mem(grid, allow_f32=True)
         if gm != cm:
             raise ValueError("grid and coefficients must live in the same memory")
+        self._check_sizes(grid, coeffs, grid_order, exp_order, batch, grid_stride, coeff_stride)
         fn = LIB.zgl_backward_grid32 if g32 else LIB.zgl_backward
         _check(fn(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                   grid_stride, gm, _stream_ptr(stream, gm)))
@@ -303,6 +331,10 @@ class Plan:
         mems.add(vm)
         if len(mems) != 1:
             raise ValueError("coefficients, points and values must live in the same memory")
+        csz = 2 * self.coeff_size(order)
+        _need(coeffs, "coefficients", batch, coeff_stride or csz, csz)
+        npts = nlon * nlat * nrad
+        _need(values, "values", batch, value_stride or npts, npts)
         _check(LIB.zgl_evaluate(self._h, cp, order, lp, nlon, tp, nlat, rp or None, nrad, vp, batch,
                                 coeff_stride, value_stride, vm, _stream_ptr(stream, vm)))
         return values
@@ -321,6 +353,11 @@ class Plan:
                 out = torch.empty((batch, nent), dtype=torch.float64, device=coeffs.device)
         ap, am, _ = _ptr_and_mem(coeffs)
         op, om, _ = _ptr_and_mem(out)
+        csz = 2 * self.coeff_size(order)
+        _need(coeffs, "coefficients", batch, coeff_stride or csz, csz)
+        if other is not None:
+            _need(other, "coefficients", batch, coeff_stride or csz, csz)
+        _need(out, "output", batch, out_stride or nent, nent)
         if other is None:
             if am != om:
                 raise ValueError("coefficients and output must live in the same memory")
@@ -342,6 +379,9 @@ class Plan:
         ap, am, _ = _ptr_and_mem(angles)
         if am != cm:
             raise ValueError("angles and coefficients must live in the same memory")
+        csz = 2 * self.coeff_size(order)
+        _need(coeffs, "coefficients", batch, coeff_stride or csz, csz)
+        _need(angles, "angles", 1, 0, 3 * batch)
         _check(LIB.zgl_rotate(self._h, cp, order, ap, rotation_type, batch, coeff_stride, cm,
                               _stream_ptr(stream, cm)))
         return coeffs
@@ -353,6 +393,9 @@ class Plan:
         ap, am, _ = _ptr_and_mem(angles)
         if am != cm:
             raise ValueError("angles and coefficients must live in the same memory")
+        csz = 2 * self.coeff_size(order)
+        _need(coeffs, "coefficients", batch, coeff_stride or csz, csz)
+        _need(angles, "angles", 1, 0, batch)
         _check(LIB.zgl_polar_rotate(self._h, cp, order, ap, rotation_type, batch, coeff_stride, cm,
                                     _stream_ptr(stream, cm)))
         return coeffs
@@ -363,10 +406,15 @@ class Plan:
         _check(LIB.zgl_wignerd_pi2(self._h, order, out.ctypes.data, MEM_HOST, None))
         return out
 
-    @staticmethod
-    def _jobs(jobs):
+    def _jobs(self, jobs):
         if isinstance(jobs, ctypes.Array):
             return jobs
+        for order, grid, coeffs in jobs:
+            if order:
+                if grid is None or coeffs is None:
+                    raise ValueError("a job of order > 0 needs a grid and coefficients")
+                _need(grid, "job grid", 1, 0, self.grid_size(order))
+                _need(coeffs, "job coefficients", 1, 0, 2 * self.coeff_size(order))
         return make_jobs(jobs)
 
     def forward_ragged(self, jobs, stream=None):
@@ -397,6 +445,7 @@ class Plan:
         cp, cm, _ = _ptr_and_mem(coeffs)
         if gm != cm:
             raise ValueError("grid and coefficients must live in the same memory")
+        self._check_sizes(grid, coeffs, grid_order, exp_order, batch, grid_stride, coeff_stride)
         _check(LIB.zgl_backward_adjoint(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
                                         coeff_stride, gm, _stream_ptr(stream, gm)))
 
@@ -409,6 +458,7 @@ class Plan:
         gp, gm, _ = _ptr_and_mem(grid)
         if gm != cm:
             raise ValueError("grid and coefficients must live in the same memory")
+        self._check_sizes(grid, coeffs, grid_order, exp_order, batch, grid_stride, coeff_stride)
         _check(LIB.zgl_forward_adjoint(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                                        grid_stride, gm, _stream_ptr(stream, gm)))
 
@@ -418,8 +468,10 @@ def make_jobs(jobs):
     device tensors (None for order 0)."""
     arr = (Job * max(1, len(jobs)))()
     for i, (order, grid, coeffs) in enumerate(jobs):
+        import torch
         for t in (grid, coeffs):
-            if t is not None and (not t.is_cuda or not t.is_contiguous() or t.dtype.itemsize != 8):
+            if t is not None and (not isinstance(t, torch.Tensor) or not t.is_cuda
+                                  or not t.is_contiguous() or t.dtype != torch.float64):
                 raise ValueError("ragged jobs take contiguous float64 device tensors")
         arr[i].order = order
         arr[i].grid = grid.data_ptr() if grid is not None else None
