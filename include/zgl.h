@@ -134,6 +134,26 @@ zgl_status zgl_backward(zgl_plan *plan, const double *coeffs, size_t exp_order,
                         ptrdiff_t coeff_stride, ptrdiff_t grid_stride,
                         zgl_mem mem, void *stream);
 
+/* zgl_forward without any allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. Arguments as
+ * zgl_forward, device memory only (FP64 grids), any precision of the plan.
+ * The call allocates nothing (no hipMalloc, hipFree, event or host-heap
+ * allocation): its scratch is a static device array of 128 MiB that every GPU
+ * the library runs on holds until the process ends, and the instances are
+ * processed in chunks that fit it. Requirements, each reported as
+ * ZGL_ERROR_INVALID_VALUE: the plan already holds the tables of grid_order
+ * (created at that order or resized to it), and one instance of that order
+ * fits the array (Zernike: orders up to 175 in FP64, 145 in FP32 with
+ * refinement, 222 in FP32; SH in FP64: up to 2362). The
+ * plan's tuned tile shapes are used; no tuning or stage
+ * timing happens during the call. All calls on one GPU share the array:
+ * calls that overlap on a GPU (other streams, other threads, other plans)
+ * race, so order them. */
+zgl_status zgl_forward_noalloc(zgl_plan *plan, const double *grid, size_t grid_order,
+                               double *coeffs, size_t exp_order, size_t batch,
+                               ptrdiff_t grid_stride, ptrdiff_t coeff_stride, void *stream);
+
 /* Ragged batches: many instances of uneven orders in one call.
  * Replaces a loop of forward_transform / backward_transform calls over
  * transformers of different orders (include/zernike_glq_transformer.hpp:640-679,

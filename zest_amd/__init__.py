@@ -66,6 +66,8 @@ def _load():
     lib.zgl_forward.restype = i32
     lib.zgl_backward.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, i32, vp]
     lib.zgl_backward.restype = i32
+    lib.zgl_forward_noalloc.argtypes = [vp, vp, sz, vp, sz, sz, pd, pd, vp]
+    lib.zgl_forward_noalloc.restype = i32
     lib.zgl_forward_ragged.argtypes = [vp, ctypes.POINTER(Job), sz, vp]
     lib.zgl_forward_ragged.restype = i32
     lib.zgl_backward_ragged.argtypes = [vp, ctypes.POINTER(Job), sz, vp]
@@ -125,6 +127,7 @@ EXPORTS = (
     "zgl_plan_create", "zgl_plan_destroy", "zgl_plan_resize", "zgl_plan_order",
     "zgl_grid_size", "zgl_coeff_size", "zgl_forward", "zgl_backward",
     "zgl_forward_grid32", "zgl_backward_grid32", "zgl_backward_adjoint", "zgl_forward_adjoint",
+    "zgl_forward_noalloc",
     "zgl_forward_ragged", "zgl_backward_ragged", "zgl_ragged_lane_costs",
     "zgl_plan_autotune", "zgl_plan_tuning", "zgl_evaluate",
     "zgl_power_spectrum", "zgl_cross_power_spectrum", "zgl_rotate", "zgl_polar_rotate",
@@ -300,6 +303,20 @@ class Plan:
         _check(fn(self._h, cp, exp_order, gp, grid_order, batch, coeff_stride,
                   grid_stride, gm, _stream_ptr(stream, gm)))
 
+
+    def forward_noalloc(self, grid, coeffs, grid_order: int, exp_order: int | None = None,
+                        batch: int = 1, grid_stride: int = 0, coeff_stride: int = 0, stream=None):
+        """zgl_forward_noalloc: forward() with no allocation, scratch from the
+        library's static device array (device tensors, FP64 grids; the plan
+        must hold the tables of grid_order)."""
+        exp_order = grid_order if exp_order is None else exp_order
+        gp, gm, _ = _ptr_and_mem(grid)
+        cp, cm, _ = _ptr_and_mem(coeffs)
+        if gm != MEM_DEVICE or cm != MEM_DEVICE:
+            raise ValueError("the no-allocation forward takes device tensors")
+        self._check_sizes(grid, coeffs, grid_order, exp_order, batch, grid_stride, coeff_stride)
+        _check(LIB.zgl_forward_noalloc(self._h, gp, grid_order, cp, exp_order, batch, grid_stride,
+                                       coeff_stride, _stream_ptr(stream, MEM_DEVICE)))
 
     def evaluate(self, coeffs, order: int, longitudes, colatitudes, radii=None, values=None,
                  batch: int = 1, coeff_stride: int = 0, value_stride: int = 0, stream=None):

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <new>
 #include <string>
@@ -84,6 +85,9 @@ struct zgl_plan {
     // instances per workgroup
     bool autotune = false;
     std::map<std::tuple<size_t, int, int, int>, int> tuned;
+    // instances per chunk at most (0: no limit beyond the scratch budget);
+    // set by zgl_forward_noalloc for the duration of its call
+    size_t chunk_cap = 0;
     // stage timing (zgl_profile_enable)
     bool profiling = false;
     struct Rec { int stage; int nb; hipEvent_t start, stop; };
@@ -128,6 +132,11 @@ struct zgl_plan {
 namespace zgl {
 namespace {
 
+// Set while zgl_forward_noalloc runs on this thread: a buffer that would
+// have to grow, or a table not yet built, is an error there, never an
+// allocation.
+thread_local bool t_no_alloc = false;
+
 size_t grid_size(const zgl_plan* p, size_t order) {
     if (order == 0) return 0;
     const size_t lon_min = 2 * order - 1;
@@ -144,6 +153,10 @@ size_t coeff_size(const zgl_plan* p, size_t order) {
 const OrderTables& tables_for(zgl_plan* p, size_t order) {
     auto it = p->tables.find(order);
     if (it != p->tables.end()) return it->second;
+    if (t_no_alloc)
+        throw Error{ZGL_ERROR_INVALID_VALUE,
+                    "no tables for order " + std::to_string(order) +
+                        ": create or resize the plan to that order before the no-allocation call"};
     OrderTables t;
     try {
         build_tables(t, p->kind, p->znorm, p->shnorm, p->phase, p->layout, (int)order,
@@ -155,10 +168,16 @@ const OrderTables& tables_for(zgl_plan* p, size_t order) {
     return p->tables.emplace(order, t).first->second;
 }
 
+void no_alloc_check(size_t need, size_t cap) {
+    if (t_no_alloc && need > cap)
+        throw Error{ZGL_ERROR_INVALID_VALUE, "scratch of the no-allocation path too small (internal)"};
+}
+
 // Growing a buffer frees the old one: wait for queued work that may use it.
 template <class T>
 void ensure(T*& ptr, size_t& cap, size_t need) {
     if (need <= cap) return;
+    no_alloc_check(need, cap);
     if (ptr) check_hip(hipDeviceSynchronize(), "scratch growth");
     if (ptr) check_hip(hipFree(ptr), "free scratch");
     ptr = nullptr;
@@ -185,6 +204,7 @@ void ragged_release(zgl_plan::Ragged& r) {
 
 void ensure_bytes(void*& ptr, size_t& cap, size_t need) {
     if (need <= cap) return;
+    no_alloc_check(need, cap);
     if (ptr) check_hip(hipDeviceSynchronize(), "scratch growth");
     if (ptr) check_hip(hipFree(ptr), "free scratch");
     ptr = nullptr;
@@ -223,6 +243,7 @@ Chunking chunking(const zgl_plan* p, const Geom& g, size_t batch, size_t csize) 
     }
     size_t nb = per ? budget / per : cap;
     nb = std::max<size_t>(1, std::min({nb, cap, batch}));
+    if (p->chunk_cap) nb = std::min(nb, p->chunk_cap);
     c.nb = (int)nb;
     return c;
 }
@@ -306,15 +327,18 @@ int tune(hipStream_t s, const std::vector<int>& cands, F&& launch) {
     return pick;
 }
 
-template <class F>
+// `cands` yields the candidates (built only when a stage is tuned)
+template <class CF, class F>
 int tuned_choice(zgl_plan* p, size_t order, int rbytes, int gbytes, int stage, hipStream_t s,
-                 const std::vector<int>& cands, F&& launch) {
+                 CF&& cands_of, F&& launch) {
     const auto key = std::make_tuple(order, rbytes, stage, gbytes);
     auto it = p->tuned.find(key);
     if (it != p->tuned.end()) return it->second;
-    if (!p->autotune || cands.size() < 2) return 0;
+    if (!p->autotune) return 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+    const std::vector<int> cands = cands_of();
+    if (cands.size() < 2) return 0;
     const int c = tune(s, cands, launch);
     p->tuned[key] = c;
     return c;
@@ -369,7 +393,7 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
     const int kappa = !adjoint && p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0;
     for (size_t b0 = 0; b0 < batch; b0 += ch.nb) {
         const int nb = (int)std::min<size_t>(ch.nb, batch - b0);
-        L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_FWD, s, fft_tiles(t.fft, sizeof(T) == 4),
+        L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_FWD, s, [&] { return fft_tiles(t.fft, sizeof(T) == 4); },
                               [&](int c) {
                                   FftLaunch L2 = L;
                                   L2.tile = c;
@@ -383,14 +407,14 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
         } else if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, kappa, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride,
                          coeffs.from(b0), nb};
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_FWD, s, legendre_candidates<T>(p), [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_FWD, s, [&] { return legendre_candidates<T>(p); }, [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_legendre_forward<T>(b, s);
             });
             a.group |= legendre_math<T>(p);
             timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_FWD, s, kGroupCandidates, [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_FWD, s, [&] { return kGroupCandidates; }, [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_radial_forward<T>(b, s);
@@ -441,13 +465,13 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
             timed(p, ST_FUSED_BWD, nb, s, [&] { zt_fused_backward(a, s); });
         } else if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, 0, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride, cin, nb};
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_BWD, s, kGroupCandidates, [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_BWD, s, [&] { return kGroupCandidates; }, [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_radial_backward<T>(b, s);
             });
             timed(p, ST_RAD_BWD, nb, s, [&] { zt_radial_backward<T>(a, s); });
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_BWD, s, legendre_candidates<T>(p), [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_LEG_BWD, s, [&] { return legendre_candidates<T>(p); }, [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_legendre_backward<T>(b, s);
@@ -458,7 +482,7 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
             StGemmArgs a{&t, (int)nt, sc.spec, ch.spec_stride, cin, nb, sc.flm};
             timed(p, ST_LEG_BWD, nb, s, [&] { st_legendre_backward<T>(a, s); });
         }
-        L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_BWD, s, fft_tiles(t.fft, sizeof(T) == 4),
+        L.tile = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_FFT_BWD, s, [&] { return fft_tiles(t.fft, sizeof(T) == 4); },
                               [&](int c) {
                                   FftLaunch L2 = L;
                                   L2.tile = c;
@@ -538,6 +562,10 @@ zgl_status guarded(F&& f) {
     }
 }
 
+// Resolves the device address of the no-allocation path's static scratch on
+// the current device, when there is one (defined with the array).
+void resolve_static_scratch();
+
 zgl_status check_conventions(zgl_kind kind, int zn, int sn, int sp, int layout) {
     if (kind != ZGL_ZERNIKE && kind != ZGL_SH) return fail(ZGL_ERROR_INVALID_VALUE, "unknown transform kind");
     if (zn != ZGL_ZNORM_NORMED && zn != ZGL_ZNORM_UNNORMED)
@@ -587,6 +615,7 @@ zgl_status zgl_plan_create(zgl_plan** plan, zgl_kind kind, int zernike_norm, int
             zgl_plan_destroy(p);
             throw;
         }
+        resolve_static_scratch();
         *plan = p;
         return ZGL_SUCCESS;
     });
@@ -958,6 +987,140 @@ zgl_status zgl_backward(zgl_plan* plan, const double* coeffs, size_t exp_order, 
     return backward_entry<double>(plan, coeffs, exp_order, grid, grid_order, batch, coeff_stride,
                                   grid_stride, mem, stream);
 }
+
+}  // extern "C"
+
+// ------------------------------------------------- forward without allocation
+// Device scratch of zgl_forward_noalloc: the spectrum, flm / staging and
+// refinement buffers of one chunk of instances, carved out of this array.
+// Every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` is only a stand-in for the device address: the
+// address on the current device comes from hipGetSymbolAddress, resolved once
+// per device (at plan creation, or on the entry point's first call there).
+static constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace zgl {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+void* static_scratch() {
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "current device");
+    if (dev < 0 || dev >= kMaxDevices)
+        throw Error{ZGL_ERROR_NOT_SUPPORTED, "device index beyond the static scratch table"};
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "static scratch address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+void resolve_static_scratch() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return;  // no device: resolved on the entry point's first call instead
+    }
+    try {
+        (void)static_scratch();
+    } catch (const Error&) {
+        (void)hipGetLastError();  // reported by the entry point, if it is used
+    }
+}
+
+// For the duration of a no-allocation call: no autotuning or stage timing
+// (both create events), chunks capped to what the static scratch holds, and
+// any buffer growth or table build turned into an error.
+struct NoAllocScope {
+    zgl_plan* p;
+    bool autotune, profiling;
+    NoAllocScope(zgl_plan* plan, size_t cap) : p(plan), autotune(plan->autotune), profiling(plan->profiling) {
+        p->autotune = false;
+        p->profiling = false;
+        p->chunk_cap = cap;
+        t_no_alloc = true;
+    }
+    ~NoAllocScope() {
+        t_no_alloc = false;
+        p->chunk_cap = 0;
+        p->autotune = autotune;
+        p->profiling = profiling;
+    }
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace zgl
+
+extern "C" zgl_status zgl_forward_noalloc(zgl_plan* plan, const double* grid, size_t grid_order,
+                                          double* coeffs, size_t exp_order, size_t batch,
+                                          ptrdiff_t grid_stride, ptrdiff_t coeff_stride, void* stream) {
+    using namespace zgl;
+    if (!plan) return fail(ZGL_ERROR_INVALID_VALUE, "plan is NULL");
+    if (grid_order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "grid order exceeds the supported maximum");
+    const size_t nt = std::min(exp_order, grid_order);
+    const long gsz = (long)grid_size(plan, grid_order);
+    const long csz = 2 * (long)coeff_size(plan, exp_order);
+    const long gs = grid_stride ? (long)grid_stride : gsz;
+    const long cs = coeff_stride ? (long)coeff_stride : csz;
+    if (batch > 1 && (gs < gsz || cs < 2 * (long)coeff_size(plan, nt)))
+        return fail(ZGL_ERROR_INVALID_VALUE, "instance strides overlap");
+    if (grid_order && plan->tables.find(grid_order) == plan->tables.end())
+        return fail(ZGL_ERROR_INVALID_VALUE,
+                    "no tables for this grid order: create or resize the plan to it before the "
+                    "no-allocation call");
+    return guarded([&]() -> zgl_status {
+        plan->order = grid_order;
+        if (batch == 0 || nt == 0) return ZGL_SUCCESS;
+        if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
+        const Geom& g = plan->tables.find(grid_order)->second.g;
+        // bytes of scratch per instance: spectrum and flm (SH: staging) in the
+        // widest arithmetic the call runs, and the refinement buffers
+        const bool wide = plan->precision == ZGL_FP64 || plan->precision == ZGL_FP32_REFINE;
+        const size_t csize = wide ? sizeof(double2) : sizeof(float2);
+        const Chunking one = chunking(plan, g, 1, csize);
+        const size_t spec_b = (size_t)one.spec_stride * csize, flm_b = (size_t)one.flm_stride * csize;
+        const size_t ncw = 2 * coeff_size(plan, nt);
+        const bool refine = plan->precision == ZGL_FP32_REFINE;
+        const size_t per = spec_b + flm_b + (refine ? sizeof(double) * ((size_t)gsz + ncw) : 0);
+        const size_t nb_max = std::min<size_t>(batch, (kStaticScratchBytes - 4 * 256) / per);
+        if (nb_max == 0)
+            return fail(ZGL_ERROR_INVALID_VALUE,
+                        "one instance of this order needs " + std::to_string(per) +
+                            " bytes of scratch; the no-allocation path holds " +
+                            std::to_string(kStaticScratchBytes) + " (zgl_forward has no such limit)");
+        unsigned char* base = static_cast<unsigned char*>(static_scratch());
+        Scratch sc;
+        size_t off = 0;
+        sc.spec = base + off; sc.spec_cap = nb_max * spec_b; off = align256(off + sc.spec_cap);
+        sc.flm = base + off; sc.flm_cap = nb_max * flm_b; off = align256(off + sc.flm_cap);
+        if (refine) {
+            sc.rgrid = reinterpret_cast<double*>(base + off); sc.rgrid_cap = nb_max * (size_t)gsz;
+            off = align256(off + sc.rgrid_cap * sizeof(double));
+            sc.rcoef = reinterpret_cast<double*>(base + off); sc.rcoef_cap = nb_max * ncw;
+            off = align256(off + sc.rcoef_cap * sizeof(double));
+        }
+        if (off > kStaticScratchBytes) throw Error{ZGL_ERROR_HIP, "static scratch layout (internal)"};
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        {
+            NoAllocScope scope(plan, nb_max);
+            forward_any<double>(plan, sc, Inst<const double>{grid, gs, nullptr}, grid_order,
+                                Inst<double>{coeffs, cs, nullptr}, nt, batch, s);
+        }
+        check_hip(hipGetLastError(), "forward (no allocation)");
+        return ZGL_SUCCESS;
+    });
+}
+
+extern "C" {
 
 zgl_status zgl_forward_ragged(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void* stream) {
     return ragged_entry(plan, jobs, njobs, stream, true);
