@@ -27,6 +27,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "zgl_internal.hpp"
 
@@ -184,6 +185,23 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
     typename M::acc_t acc[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc[q] = typename M::acc_t{0, 0, 0, 0};
+    // FP32 products: each chunk's FP32 partial sums are folded into FP64
+    // accumulators (the contraction runs over hundreds of latitudes or degrees
+    // at high order; within a 32-deep chunk the FP32 sum is short)
+    constexpr bool kFold = std::is_same_v<T, float>;
+    dbl4 acc64[kFold ? NS : 1];
+#pragma unroll
+    for (int q = 0; q < (kFold ? NS : 1); ++q) acc64[q] = dbl4{0, 0, 0, 0};
+    auto fold = [&] {
+        if constexpr (kFold) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc64[q][r] += (double)acc[q][r];
+                acc[q] = typename M::acc_t{0, 0, 0, 0};
+            }
+        }
+    };
 
     auto compute = [&](int k0) {
         const int kpad = (min(TK, P.K - k0) + 3) & ~3;
@@ -210,6 +228,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
             load_b(k0 + 2 * TK, rb);
         }
         compute(k0);
+        fold();
         __syncthreads();
     };
 
@@ -233,7 +252,11 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __rest
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = s * 16 + M::row(lane, r);
-            if (i < rows) st.c(P, m0 + i, n0 + j, acc[q][r]);
+            if constexpr (kFold) {
+                if (i < rows) st.c(P, m0 + i, n0 + j, (T)acc64[q][r]);
+            } else {
+                if (i < rows) st.c(P, m0 + i, n0 + j, acc[q][r]);
+            }
         }
     }
 }
