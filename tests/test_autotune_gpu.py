@@ -38,8 +38,9 @@ def test_tuned_results_identical(device, precision):
     x3 = any(st.startswith("legendre") and choice >= 16 for _, _, _, st, choice in tuning)
     for (g0, c0), (g1, c1) in zip(out[False][0], out[True][0]):
         if x3:
-            assert np.abs(g0 - g1).max() <= 5e-6 * np.abs(g0).max()
-            assert np.abs(c0 - c1).max() <= 5e-6 * np.abs(c0).max()
+            # two FP32-level results against each other, each within the 1e-6 gate
+            assert np.abs(g0 - g1).max() <= 2e-6 * np.abs(g0).max()
+            assert np.abs(c0 - c1).max() <= 2e-6 * np.abs(c0).max()
         else:
             assert np.array_equal(g0, g1) and np.array_equal(c0, c1)
     stages = {(o, st) for o, _, _, st, _ in tuning}

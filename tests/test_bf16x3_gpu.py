@@ -54,7 +54,7 @@ def test_bf16x3_zernike(device, monkeypatch, order, plm):
         er = _rel(x3r[0][b], c_ref)
         print(f"order {order} {plm}: bf16x3 fwd {ef:.2e} bwd {eb:.2e}; fp32 fwd {ef32:.2e} "
               f"bwd {eb32:.2e}; refined {er:.2e}")
-        assert ef < 5e-6 and eb < 5e-6
+        assert ef < 1e-6 and eb < 1e-6
         assert er < 1e-10
 
 

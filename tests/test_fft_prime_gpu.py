@@ -75,7 +75,7 @@ def test_prime_radix_fp32(device, kind, order):
         g_ref = oracle.zt_backward(c[b], order) if zt else oracle.st_backward(c[b], order)
         c_ref = oracle.zt_forward(g_ref, order) if zt else oracle.st_forward(g_ref, order)
         print(f"{kind} {order} fp32: backward {_rel(g[b], g_ref):.2e}, forward {_rel(co[b], c_ref):.2e}")
-        assert _rel(g[b], g_ref) < 5e-6 and _rel(co[b], c_ref) < 5e-6
+        assert _rel(g[b], g_ref) < 1e-6 and _rel(co[b], c_ref) < 1e-6
 
 
 @pytest.mark.parametrize("kind,order,layout", [("zernike", 96, 0), ("sh", 100, 1)])
@@ -145,7 +145,7 @@ def test_fp32_tile_widths(device, monkeypatch, kind, order, layout):
             c_ref = oracle.st_forward(g_ref, order, layout=layout)
         print(f"{kind} {order} layout {layout} fp32 T=32: backward {_rel(g[b], g_ref):.2e}, "
               f"forward {_rel(co[b], c_ref):.2e}")
-        assert _rel(g[b], g_ref) < 5e-6 and _rel(co[b], c_ref) < 5e-6
+        assert _rel(g[b], g_ref) < 1e-6 and _rel(co[b], c_ref) < 1e-6
 
 
 def test_fp32_width32_headline_shape(device, monkeypatch):
