@@ -12,16 +12,21 @@ the mixed form; the FP64 form is timed the same way beside it.
 Two mixed forms are timed: with the grids stored in FP64 (zgl_forward /
 zgl_backward of an FP32 plan) and with the grids stored in FP32
 (zgl_forward_grid32 / zgl_backward_grid32); tables and coefficients are FP64
-in both. Accuracy gate (err <= 1e-6): max over instances of
-max|c - c64| / max|c64|, where c64 is the FP64 forward of the FP64 backward
-grids and c the mixed forward of the same grids as that form stores them (so
-the FP32-grid form's error includes rounding the grids to FP32). The plain
-FP32 forward is used when it meets the gate, else the forward with one
-FP64-residual refinement step. The headline is the faster form that meets
-the gate; the other is reported beside it.
+in both. Accuracy gate (err <= 1e-6), normwise: max over instances of
+max|x - x64| / max|x64|. A form qualifies when (1) its forward of the FP64
+backward grids (as the form stores them) and (2) its backward are within the
+gate against FP64, and (3) the output of its timed step itself -- the mixed
+backward into the form's grid, then the mixed forward of that grid -- is
+within the gate against the FP64 step's output. The plain FP32 forward is
+used when it passes, else the forward with one FP64-residual refinement step.
+The headline is the faster qualifying form; the other is reported beside it.
+Elementwise errors with a floor, max |x - x64| / max(|x64|, 1e-3 max|x64|),
+are reported beside the normwise ones.
 
 The dominant kernel's roofline comes from HIP events recorded around each
-kernel launch on its own stream during the timed region. The CPU baseline
+kernel launch on its own stream during the timed region; every stage also
+reports its binding roof, max(bytes / HBM peak, flops / MFMA peak), and the
+step the sum of those roofs against its measured time. The CPU baseline
 times the upstream zest library (compiled from its sources into oracle/_ref)
 on the host cores over a bounded sample, or the C oracle when that library is
 absent.
@@ -54,20 +59,25 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 
 def stage_work(order: int, real_bytes: int, grid_bytes: int = 8):
-    """Algorithmic bytes / flops of one instance per stage (one direction).
-    The grid holds `grid_bytes` reals per point; the spectrum holds complex
-    values of the compute type."""
+    """Algorithmic HBM bytes and flops of one instance per stage (one
+    direction). The grid holds `grid_bytes` reals per point; the spectrum and
+    flm hold complex values of the compute type; coefficients are FP64 pairs.
+    Flops count the parity-split contractions (a real factor times a complex
+    value: 4 flops per term) and nothing for the FFT (its arithmetic is far
+    below the machine balance)."""
     nphi, nth, nr = 2 * order - 1, order + 1, (order + 1 + 7) // 8 * 8
     F = nphi // 2 + 1
     ncoef = (order + 1) * (order + 3) * (2 * order + 1) // 24
-    fft_bytes = nphi * nth * nr * grid_bytes + F * nth * nr * 2 * real_bytes
-    legendre_flops = 2 * (order * (order + 1) // 2) * nth * nr
-    radial_flops = 4 * ncoef * nr
-    return {
-        "fft_forward": ("hbm", fft_bytes), "fft_backward": ("hbm", fft_bytes),
-        "legendre_forward": ("mfma", legendre_flops), "legendre_backward": ("mfma", legendre_flops),
-        "radial_forward": ("mfma", radial_flops), "radial_backward": ("mfma", radial_flops),
-    }
+    ntri = order * (order + 1) // 2
+    spec = F * nth * nr * 2 * real_bytes
+    flm = ntri * nr * 2 * real_bytes
+    fft = (nphi * nth * nr * grid_bytes + spec, 0)
+    leg = (spec + flm, 2 * ntri * nth * nr)
+    rad = (flm + 16 * ncoef, 4 * ncoef * nr)
+    fused = (spec + 16 * ncoef, leg[1] + rad[1])
+    return {"fft_forward": fft, "fft_backward": fft, "legendre_forward": leg, "legendre_backward": leg,
+            "radial_forward": rad, "radial_backward": rad,
+            "legendre_radial_forward": fused, "radial_legendre_backward": fused}
 
 
 def _traffic(stage, order, instances, form):
@@ -77,70 +87,124 @@ def _traffic(stage, order, instances, form):
 
 
 def stage_table(stats, order, dtype, grid_bytes=8):
+    """Per stage: the achieved HBM bandwidth and MFMA rate against their peaks,
+    the binding roof max(bytes / HBM peak, flops / MFMA peak) and the fraction
+    of it reached (roof time / measured time)."""
     work = stage_work(order, 8 if dtype == "f64" else 4, grid_bytes)
     stages = {}
     for name, (launches, inst, ms) in stats.items():
         if name not in work or launches == 0:
             continue
-        bound, per_inst = work[name]
-        per_launch = per_inst * inst / launches
+        nbytes, flops = work[name]
         avg_s = ms / launches / 1e3
+        b_launch, f_launch = nbytes * inst / launches, flops * inst / launches
+        t_hbm = b_launch / (HBM_PEAK_GBS * 1e9)
+        t_mfma = f_launch / (PEAK_TFLOPS[dtype] * 1e12)
+        bound = "hbm" if t_hbm >= t_mfma else "mfma"
         if bound == "hbm":
-            achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+            achieved, peak, unit = b_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
         else:
-            achieved, peak, unit = per_launch / avg_s / 1e12, PEAK_TFLOPS[dtype], "TFLOP/s"
+            achieved, peak, unit = f_launch / avg_s / 1e12, PEAK_TFLOPS[dtype], "TFLOP/s"
         stages[name] = {"ms_per_launch": round(ms / launches, 5), "launches": launches,
                         "instances_per_launch": round(inst / launches, 3), "bound": bound,
                         "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                        "frac": round(achieved / peak, 4), "total_ms": ms}
+                        "frac": round(achieved / peak, 4),
+                        "hbm_gbs": round(b_launch / avg_s / 1e9, 1),
+                        "mfma_tflops": round(f_launch / avg_s / 1e12, 2),
+                        "roof_ms": round(max(t_hbm, t_mfma) * 1e3, 5), "total_ms": ms}
     total = sum(v["total_ms"] for v in stages.values()) or 1.0
     for v in stages.values():
         v["share"] = round(v.pop("total_ms") / total, 4)
     return stages
 
 
+def pipeline_roofline(stages, steps, ms_per_step):
+    """The summed roof time of every stage launch in the timed region (the
+    sum over stages of max(bytes / HBM peak, flops / MFMA peak)) per step,
+    against the measured step time."""
+    if not stages or not ms_per_step:
+        return None
+    roof = sum(v["roof_ms"] * v["launches"] for v in stages.values()) / steps
+    return {"roof_ms_per_step": round(roof, 4), "ms_per_step": ms_per_step,
+            "frac": round(roof / ms_per_step, 4)}
+
+
+def cpu_info():
+    """The host CPU as this process sees it: model name, logical CPUs, the
+    CPUs this process may run on (the box's share), and the ISA of the zest
+    build that the baseline loads."""
+    import oracle
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "zest_build": oracle.REF_ISA}
+
+
 def cpu_baseline(order: int, seconds: float):
-    """zest (or the oracle) fwd+bwd pairs/s on the host, one transformer per thread."""
+    """zest (or the oracle) fwd+bwd pairs/s on the host, one transformer per
+    thread, on every CPU the process may use and on one: after a warm-up
+    second, the pairs completed in each one-second epoch (nanobench's
+    minEpochTime(1s), benchmarks/benchmark_zernike_glq_forward_transform.cpp:52-54),
+    the median epoch reported."""
     import numpy as np
     import oracle
     from zest_amd import synth
-    threads = max(1, min(16, os.cpu_count() or 1))
+    info = cpu_info()
     kind = "reference" if oracle.REF is not None else "port"
     c = synth.zernike_coeffs(order, 0)
-    counts = [0] * threads
-    deadline = [0.0]
 
-    def work(t):
-        if kind == "reference":
-            z = oracle.ZestZernike(order, 1, 0, -1)
-            g = np.zeros(oracle.zt_shape(order))
-            co = np.zeros_like(c)
-            while time.perf_counter() < deadline[0]:
-                z.backward(c, out=g)
-                z.forward(g, out=co)
-                counts[t] += 1
-        else:
-            while time.perf_counter() < deadline[0]:
-                g = oracle.zt_backward(c, order)
-                oracle.zt_forward(g, order)
-                counts[t] += 1
+    def run(threads, epochs):
+        stamps = [[] for _ in range(threads)]
+        t0 = time.perf_counter()
+        end = t0 + 1.0 + epochs
 
-    # ctypes releases the GIL during each call, so the threads run in parallel.
-    t0 = time.perf_counter()
-    deadline[0] = t0 + seconds
-    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
-    for th in ths:
-        th.start()
-    for th in ths:
-        th.join()
-    elapsed = time.perf_counter() - t0
-    total = sum(counts)
+        def work(t):
+            if kind == "reference":
+                z = oracle.ZestZernike(order, 1, 0, -1)
+                g = np.zeros(oracle.zt_shape(order))
+                co = np.zeros_like(c)
+                while time.perf_counter() < end:
+                    z.backward(c, out=g)
+                    z.forward(g, out=co)
+                    stamps[t].append(time.perf_counter())
+            else:
+                while time.perf_counter() < end:
+                    g = oracle.zt_backward(c, order)
+                    oracle.zt_forward(g, order)
+                    stamps[t].append(time.perf_counter())
+
+        # ctypes releases the GIL during each call, so the threads run in parallel
+        ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        allst = np.sort(np.concatenate([np.asarray(x) for x in stamps if x] or [np.zeros(0)]))
+        rates = [int(((allst >= t0 + 1.0 + k) & (allst < t0 + 2.0 + k)).sum()) for k in range(epochs)]
+        return float(np.median(rates)), len(allst), rates
+
+    threads = max(1, info["affinity"])
+    epochs_all = max(3, int(seconds * 0.6))
+    epochs_one = max(3, int(seconds * 0.3))
+    v_all, n_all, r_all = run(threads, epochs_all)
+    v_one, n_one, r_one = run(1, epochs_one)
+    build = (f"zest compiled from its sources (-O3 -march={oracle.REF_ISA})" if kind == "reference"
+             else "C restatement of zest")
     return {
-        "value": round(total / elapsed, 3), "unit": "transforms/s", "cores": threads, "kind": kind,
-        "sample": (f"{total} Zernike backward+forward pairs at order {order} in {elapsed:.1f} s, "
-                   f"{threads} threads, one transformer each, FP64; "
-                   + ("zest compiled from its sources (-O3 -march=x86-64-v3)" if kind == "reference"
-                      else "C restatement of zest")),
+        "value": round(v_all, 3), "unit": "transforms/s", "cores": threads, "kind": kind,
+        "single_core_value": round(v_one, 3),
+        "cpu": info,
+        "sample": (f"Zernike backward+forward pairs at order {order}, FP64, one transformer per "
+                   f"thread: median of {epochs_all} one-second epochs on {threads} threads "
+                   f"(pairs per epoch {r_all}; {n_all} in all, warm-up second included) and of "
+                   f"{epochs_one} on 1 thread ({r_one}); {build}"),
     }
 
 
@@ -203,8 +267,15 @@ def run_headline(args):
             p.autotune(True)
 
     def relerr(a, ref):
+        """max over instances of max|a - ref| / max|ref| (normwise)"""
         dims = tuple(range(1, ref.dim()))
         return float(((a.double() - ref).abs().amax(dim=dims) / ref.abs().amax(dim=dims)).max())
+
+    def elemerr(a, ref, floor=1e-3):
+        """max over entries of |a - ref| / max(|ref|, floor * max|ref| of the instance)"""
+        dims = tuple(range(1, ref.dim()))
+        scale = torch.maximum(ref.abs(), floor * ref.abs().amax(dim=dims, keepdim=True))
+        return float(((a.double() - ref).abs() / scale).max())
 
     # accuracy of the mixed forms against FP64 on the FP64 backward grids
     if not args.no_accuracy:
@@ -228,9 +299,12 @@ def run_headline(args):
         torch.cuda.synchronize()
         e32, e32r, ebwd = relerr(c32, c64), relerr(c32r, c64), relerr(gout, dg)
         acc[form] = {"fwd_max_rel_err_vs_fp64": e32, "refined_fwd_max_rel_err_vs_fp64": e32r,
-                     "bwd_max_rel_err_vs_fp64": ebwd}
+                     "bwd_max_rel_err_vs_fp64": ebwd,
+                     "fwd_elementwise_err_vs_fp64": elemerr(c32, c64),
+                     "bwd_elementwise_err_vs_fp64": elemerr(gout, dg)}
         fwd_plan, fwd_name = (p32, "fp32") if e32 <= 1e-6 else (p32r, "fp32+refine")
-        gate = min(e32, e32r) <= 1e-6
+        # the timed step's own output is gated after the timed loop as well
+        gate = min(e32, e32r) <= 1e-6 and ebwd <= 1e-6
         forms[form] = (gdtype, fwd_plan, fwd_name, gate)
         del gin, c32, c32r, gout
     if not args.no_accuracy:
@@ -261,18 +335,30 @@ def run_headline(args):
                 a = stats.get(k, (0, 0, 0.0))
                 stats[k] = (a[0] + v[0], a[1] + v[1], a[2] + v[2])
             p.profile(False)
-        return elapsed, stats
+        # the error of what the timed step produced (backward into this form's
+        # grid, forward of that grid) against the FP64 step's output
+        step_err = None if args.no_accuracy else (relerr(dco, c64), elemerr(dco, c64))
+        return elapsed, stats, step_err
 
     results = {}
     if args.only in (None, "fp64"):
         results["fp64"] = timed(p64, p64, dg)
-    for form, (gdtype, fwd_plan, _, gate) in forms.items():
+    for form, (gdtype, fwd_plan, fwd_name, gate) in forms.items():
         if gate and args.only in (None, form):
-            results[form] = timed(p32, fwd_plan, dg if gdtype == torch.float64 else dg32)
+            r = timed(p32, fwd_plan, dg if gdtype == torch.float64 else dg32)
+            if r[2] is not None and r[2][0] > 1e-6 and fwd_plan is p32:
+                # the plain FP32 step misses the gate: time the refined forward instead
+                forms[form] = (gdtype, p32r, "fp32+refine", gate)
+                r = timed(p32, p32r, dg if gdtype == torch.float64 else dg32)
+            if r[2] is not None:
+                acc[form]["timed_step_max_rel_err_vs_fp64"] = r[2][0]
+                acc[form]["timed_step_elementwise_err_vs_fp64"] = r[2][1]
+            if r[2] is None or r[2][0] <= 1e-6:
+                results[form] = r
 
     mixed = [f for f in ("mixed", "mixed_grid32") if f in results]
     head = max(mixed, key=lambda f: -results[f][0]) if mixed else "fp64"
-    elapsed, stats = results[head]
+    elapsed, stats, _ = results[head]
     dtype = "f32" if head != "fp64" else "f64"
     gbytes = 4 if head == "mixed_grid32" else 8
     value = B * args.steps / elapsed
@@ -302,21 +388,25 @@ def run_headline(args):
                      "traffic": _traffic(dom_name, N, B, head),
                      "kernel": dom_name},
         "accuracy": acc,
+        "pipeline_roofline": pipeline_roofline(stages, args.steps, round(elapsed / args.steps * 1e3, 4)),
         "stages": stages,
         "tuning": {"fp64": [list(t) for t in p64.tuning()], "fp32": [list(t) for t in p32.tuning()]},
     }
     for form in mixed:
         if form == head:
             continue
-        e, st = results[form]
-        out[form] = {"value": round(B * args.steps / e, 2),
-                     "ms_per_step": round(e / args.steps * 1e3, 4), "forward": forms[form][2],
-                     "stages": stage_table(st, N, "f32", 4 if form == "mixed_grid32" else 8)}
+        e, st, _ = results[form]
+        ms = round(e / args.steps * 1e3, 4)
+        stt = stage_table(st, N, "f32", 4 if form == "mixed_grid32" else 8)
+        out[form] = {"value": round(B * args.steps / e, 2), "ms_per_step": ms, "forward": forms[form][2],
+                     "pipeline_roofline": pipeline_roofline(stt, args.steps, ms), "stages": stt}
     if "fp64" in results:
-        e64, s64 = results["fp64"]
+        e64, s64, _ = results["fp64"]
         out["fp64_value"] = round(B * args.steps / e64, 2)
         out["fp64_ms_per_step"] = round(e64 / args.steps * 1e3, 4)
         out["fp64_stages"] = stage_table(s64, N, "f64")
+        out["fp64_pipeline_roofline"] = pipeline_roofline(out["fp64_stages"], args.steps,
+                                                          out["fp64_ms_per_step"])
         if head != "fp64":
             out["mixed_vs_fp64"] = round(value / out["fp64_value"], 3)
     if not args.no_cpu:
@@ -488,14 +578,14 @@ def run_grad(args):
     with the closed form 2 diag(1/nu) F(B c - g)."""
     import numpy as np
     import torch
-    import oracle
     import zest_amd
     from zest_amd import synth
     dev = torch.device("cuda:0")
     N, B = 64, 256
     nphi, nth, nr = zest_amd.zt_shape(N)
-    z, wz = oracle.gl_nodes_weights(nth)
-    x, wx = oracle.gl_nodes_weights(nr)
+    # the quadrature nodes of the ball grid (ascending, as zest's)
+    z, wz = np.polynomial.legendre.leggauss(nth)
+    x, wx = np.polynomial.legendre.leggauss(nr)
     r = 0.5 * (1.0 + x)
     phi = 2.0 * np.pi * np.arange(nphi) / nphi
     st = np.sqrt(np.maximum(0.0, 1.0 - z * z))

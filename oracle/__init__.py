@@ -59,9 +59,33 @@ def _load_oracle():
     return lib
 
 
+def _host_has_avx512() -> bool:
+    """x86-64-v4 (AVX-512 F, BW, CD, DQ, VL) on this host, from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        pass
+    return False
+
+
+# Which build of compiled zest REF is: "x86-64-v4" (AVX-512, when the host has
+# it and that build exists) or "x86-64-v3" (AVX2 + FMA); None without zest.
+REF_ISA = None
+
+
 def _load_ref():
-    path = os.path.join(_HERE, "_ref", "libzest_ref.so")
-    if not os.path.exists(path):
+    global REF_ISA
+    v3 = os.path.join(_HERE, "_ref", "libzest_ref.so")
+    v4 = os.path.join(_HERE, "_ref", "libzest_ref_v4.so")
+    if os.path.exists(v4) and _host_has_avx512():
+        path, REF_ISA = v4, "x86-64-v4"
+    elif os.path.exists(v3):
+        path, REF_ISA = v3, "x86-64-v3"
+    else:
         return None
     lib = ctypes.CDLL(path)
     lib.zref_zt_create.restype = _ptr
