@@ -270,8 +270,12 @@ public:
     }
 
     // Backward transform of an expansion holding only even or only odd l
-    // (zest: 680-700): the missing rows are zero, so it equals the backward
-    // transform of the full expansion with those rows zeroed.
+    // (zest: 680-700, 1053-1133): the missing rows are zero, so it equals the
+    // backward transform of the full expansion with those rows zeroed. As in
+    // zest, the rows read are those of a view of the same data truncated to
+    // min(expansion order, grid order): that view's order fixes the parity,
+    // and row l sits at (l*l)/4 in the row-skipping layout whichever parity
+    // it has.
     template <row_skipping_real_sh_expansion Expansion>
         requires(std::remove_cvref_t<Expansion>::norm == sh_norm_param) &&
                 (std::remove_cvref_t<Expansion>::phase == sh_phase_param) &&
@@ -279,10 +283,10 @@ public:
                 has_parity<Expansion>
     void backward_transform(Expansion&& expansion, SphereGLQGridSpan<double, GridLayout> values) {
         const std::size_t order = std::min<std::size_t>(expansion.order(), values.order());
+        typename std::remove_cvref_t<Expansion>::ConstView truncated(expansion.data(), order);
         RealSHExpansion<sh_norm_param, sh_phase_param> full(order);
-        for (auto l : expansion.indices()) {
-            if (std::size_t(l) >= order) break;
-            auto row = expansion[l];
+        for (auto l : truncated.indices()) {
+            auto row = truncated[l];
             for (std::size_t m = 0; m <= std::size_t(l); ++m) full(l, m) = row[m];
         }
         backward_transform(full, values);

@@ -98,6 +98,9 @@ def _load_ref():
     lib.zref_st_destroy.argtypes = [_ptr]
     lib.zref_st_forward.argtypes = [_ptr, _ptr, _c_size, _ptr, _c_size]
     lib.zref_st_backward.argtypes = [_ptr, _ptr, _c_size, _ptr, _c_size]
+    lib.zref_st_backward_zernike_row.argtypes = [_ptr, _ptr, _c_size, _c_size, _ptr, _c_size]
+    lib.zref_rotate_l.argtypes = [_c_size, _ptr, _ptr, _ptr, _ptr]
+    lib.zref_polar_rotate_l.argtypes = [_c_size, _ptr, _ptr]
     lib.zref_gl_nodes_weights.argtypes = [_c_size, _c_int, _ptr, _ptr]
     lib.zref_gl_angles.argtypes = [_c_size, _ptr]
     lib.zref_plm_table.argtypes = [_c_int, _c_int, _c_size, _ptr, _c_size, _ptr]
@@ -229,6 +232,30 @@ def st_backward(coeffs: np.ndarray, order: int, exp_order: int | None = None,
     grid = np.zeros(st_shape(order, layout))
     ORACLE.zo_st_backward(sh_norm, sh_phase, layout, exp_order, _p(coeffs), order, _p(grid))
     return grid
+
+
+def st_row_skipping_expansion(zcoeffs: np.ndarray, zorder: int, n: int, order: int) -> np.ndarray:
+    """The full SH expansion that zest's row-skipping backward overload
+    transforms when given row n of a Zernike expansion of order zorder and a
+    grid of the given order (include/sh_glq_transformer.hpp:690-699).
+
+    zest views row n's data (n + 1 rows of one parity, row l at (l*l)//4 in
+    RowSkippingTriangleLayout, include/layout.hpp:334-370) truncated to
+    o = min(n + 1, order); the truncated view's rows are l = (o+1) % 2, ...,
+    o - 1 in steps of 2 (ParityIndexRange, include/indexing.hpp:226-227), so a
+    truncation that changes the parity of the order reads the same data as rows
+    of the other parity. Returns the (st_num_coeffs(o), 2) array with those rows
+    filled and the others zero."""
+    if not 0 <= n < zorder:
+        raise ValueError("row n out of range")
+    zcoeffs = np.asarray(zcoeffs, dtype=np.float64)
+    o = min(n + 1, order)
+    row = zcoeffs[zt_num_coeffs(n):zt_num_coeffs(n + 1)]
+    out = np.zeros((st_num_coeffs(o), 2))
+    for l in range((o + 1) % 2, o, 2):
+        start = l * l // 4
+        out[l * (l + 1) // 2:l * (l + 1) // 2 + l + 1] = row[start:start + l + 1]
+    return out
 
 
 # ---- evaluation on arbitrary grids (zt/st::GridEvaluator,
@@ -381,6 +408,20 @@ class ZestSH:
         REF.zref_st_backward(self._h, _p(coeffs), exp_order, _p(out), order)
         return out
 
+    def backward_zernike_row(self, zcoeffs: np.ndarray, zorder: int, n: int,
+                             order: int | None = None) -> np.ndarray:
+        """zest's row-skipping backward overload on row n of a Zernike
+        expansion of order zorder (only l of n's parity present)."""
+        order = self.order if order is None else order
+        if not 0 <= n < zorder:
+            raise ValueError("row n out of range")
+        zcoeffs = np.ascontiguousarray(zcoeffs, dtype=np.float64)
+        if zcoeffs.shape != (zt_num_coeffs(zorder), 2):
+            raise ValueError("zcoeffs must have shape (zt_num_coeffs(zorder), 2)")
+        out = np.zeros(st_shape(order, self.layout))
+        REF.zref_st_backward_zernike_row(self._h, _p(zcoeffs), zorder, n, _p(out), order)
+        return out
+
 
 def ref_gl_nodes_weights(n: int, packed: bool = False):
     size = (n + 1) // 2 if packed else n
@@ -456,6 +497,31 @@ def ref_polar_rotate(coeffs, order, kind, angle, rtype=0, zernike_norm=1, sh_nor
     c = np.array(coeffs, dtype=np.float64, order="C", copy=True)
     REF.zref_polar_rotate(kind, zernike_norm, sh_norm, sh_phase, order, _p(c), float(angle), rtype)
     return c
+
+
+def _cvec(a, n: int) -> np.ndarray:
+    a = np.array(a, dtype=np.complex128, order="C", copy=True)
+    if a.shape != (n,):
+        raise ValueError(f"expected {n} complex values")
+    return a
+
+
+def ref_rotate_l(vector, exp_gamma, exp_beta, exp_alpha) -> np.ndarray:
+    """zest::rotate_l on one degree-l vector of l + 1 complex values."""
+    n = len(vector)
+    v = _cvec(vector, n)
+    g, b, a = _cvec(exp_gamma, n), _cvec(exp_beta, n), _cvec(exp_alpha, n)
+    REF.zref_rotate_l(n - 1, _p(v), _p(g), _p(b), _p(a))
+    return v
+
+
+def ref_polar_rotate_l(vector, exp_alpha) -> np.ndarray:
+    """zest::polar_rotate_l on one degree-l vector of l + 1 complex values."""
+    n = len(vector)
+    v = _cvec(vector, n)
+    a = _cvec(exp_alpha, n)
+    REF.zref_polar_rotate_l(n - 1, _p(v), _p(a))
+    return v
 
 
 def ref_euler_angles(matrix) -> np.ndarray:

@@ -11,16 +11,18 @@
 //   zt::GLQTransformer::forward_transform / backward_transform
 //       (include/zernike_glq_transformer.hpp:640-679)
 //   st::GLQTransformer::forward_transform / backward_transform
-//       (include/sh_glq_transformer.hpp:632-668)
+//       (include/sh_glq_transformer.hpp:632-668, row-skipping 680-700)
 //   gl::gl_nodes_and_weights (include/gauss_legendre.hpp:1549-1562)
 //   st::PlmRecursion::plm_real (include/plm_recursion.hpp:108-113)
 //   zt::RadialZernikeRecursion::zernike (include/radial_zernike_recursion.hpp:151-154)
 //   zt::GridEvaluator::evaluate (include/grid_evaluator.hpp:227-265)
 //   st::GridEvaluator::evaluate (include/grid_evaluator.hpp:104-133)
-//   Rotor::rotate, Rotor::polar_rotate (include/rotor.hpp:124-380) and
-//   WignerdPiHalfCollection (src/wignerd_collection.cpp:29-104)
+//   Rotor::rotate, Rotor::polar_rotate (include/rotor.hpp:124-380),
+//   WignerdPiHalfCollection (src/wignerd_collection.cpp:29-104) and
+//   rotate_l / polar_rotate_l (include/wignerd_collection.hpp:122, 304)
 //   zt::power_spectrum (include/power_spectra.hpp:118-141; zest's st power
 //   spectra name a member the SH expansion types lack and do not compile)
+#include <complex>
 #include <cstddef>
 #include <cstring>
 #include <new>
@@ -78,6 +80,8 @@ struct StBase {
                          double* coeffs, std::size_t exp_order) = 0;
     virtual void backward(const double* coeffs, std::size_t exp_order,
                           double* grid, std::size_t grid_order) = 0;
+    virtual void backward_zernike_row(const double* zcoeffs, std::size_t zorder, std::size_t n,
+                                      double* grid, std::size_t grid_order) = 0;
 };
 
 template <SHNorm sn, SHPhase sp, typename Layout>
@@ -98,6 +102,15 @@ struct St final : StBase {
         t.backward_transform(
             Exp(reinterpret_cast<const std::array<double, 2>*>(coeffs), exp_order),
             Grid(grid, grid_order));
+    }
+    // The row-skipping overload, fed row n of a Zernike expansion (only l of
+    // n's parity present).
+    void backward_zernike_row(const double* zcoeffs, std::size_t zorder, std::size_t n,
+                              double* grid, std::size_t grid_order) override {
+        using Grid = zest::st::SphereGLQGridSpan<double, Layout>;
+        using Z = zest::zt::RealZernikeSpan<const std::array<double, 2>, ZernikeNorm::normed, sn, sp>;
+        Z z(reinterpret_cast<const std::array<double, 2>*>(zcoeffs), zorder);
+        t.backward_transform(z[n], Grid(grid, grid_order));
     }
 };
 
@@ -236,6 +249,14 @@ void zref_st_forward(void* h, const double* grid, std::size_t grid_order,
 void zref_st_backward(void* h, const double* coeffs, std::size_t exp_order,
                       double* grid, std::size_t grid_order) {
     static_cast<StBase*>(h)->backward(coeffs, exp_order, grid, grid_order);
+}
+
+// st::GLQTransformer::backward_transform of a row-skipping expansion
+// (include/sh_glq_transformer.hpp:680-700): row n of the Zernike expansion
+// zcoeffs of order zorder, RealZernikeSpan layout.
+void zref_st_backward_zernike_row(void* h, const double* zcoeffs, std::size_t zorder,
+                                  std::size_t n, double* grid, std::size_t grid_order) {
+    static_cast<StBase*>(h)->backward_zernike_row(zcoeffs, zorder, n, grid, grid_order);
 }
 
 // Gauss-Legendre nodes (cos style) and weights; packed != 0 selects the
@@ -400,6 +421,26 @@ void zref_polar_rotate(int kind, int zernike_norm, int sh_norm, int sh_phase, st
                      [&](auto e) { rotor.polar_rotate(e, angle, t); });
     else
         with_st_span(sh_norm, sh_phase, coeffs, order, [&](auto e) { rotor.polar_rotate(e, angle, t); });
+}
+
+// zest::rotate_l / zest::polar_rotate_l (include/wignerd_collection.hpp:122,
+// 304) on one degree-l vector of l + 1 complex values (interleaved re, im), in
+// place; the exp_* arrays hold l + 1 complex phase factors each.
+void zref_rotate_l(std::size_t l, double* vec, const double* exp_gamma, const double* exp_beta,
+                   const double* exp_alpha) {
+    using C = std::complex<double>;
+    zest::WignerdPiHalfCollection w(l + 1);
+    std::vector<C> temp(l + 1);
+    zest::rotate_l(std::span<C>(reinterpret_cast<C*>(vec), l + 1), w[l],
+                   std::span<const C>(reinterpret_cast<const C*>(exp_gamma), l + 1),
+                   std::span<const C>(reinterpret_cast<const C*>(exp_beta), l + 1),
+                   std::span<const C>(reinterpret_cast<const C*>(exp_alpha), l + 1), temp);
+}
+
+void zref_polar_rotate_l(std::size_t l, double* vec, const double* exp_alpha) {
+    using C = std::complex<double>;
+    zest::polar_rotate_l(std::span<C>(reinterpret_cast<C*>(vec), l + 1),
+                         std::span<const C>(reinterpret_cast<const C*>(exp_alpha), l + 1));
 }
 
 }  // extern "C"

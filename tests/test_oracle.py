@@ -252,6 +252,36 @@ def test_sh_matches_compiled_zest(conv, layout, order):
     assert rel(oracle.st_backward(c, order, sh_norm=sn, sh_phase=sp, layout=layout), s.backward(c)) < 1e-13
 
 
+@needs_ref
+@pytest.mark.parametrize("conv", CONV_ST)
+@pytest.mark.parametrize("layout", [0, 1])
+def test_sh_row_skipping_backward_matches_compiled_zest(conv, layout):
+    """zest's row-skipping backward overload (include/sh_glq_transformer.hpp:
+    680-700) on rows of a Zernike expansion equals the backward transform of
+    the full expansion that oracle.st_row_skipping_expansion builds, at grid
+    orders equal to, above and below the row's order, including truncations
+    that change the parity of the order (zest then reads the row's data as rows
+    of the other parity)."""
+    sn, sp = conv
+    zorder = 14
+    c = synth.zernike_coeffs(zorder, 11)
+    t = oracle.ZestSH(20, sn, sp, layout)
+    for n, order in [(0, 1), (1, 2), (1, 20), (8, 9), (8, 16), (8, 7), (13, 14), (13, 20), (13, 12), (13, 11)]:
+        e = oracle.st_row_skipping_expansion(c, zorder, n, order)
+        g = oracle.st_backward(e, order, min(n + 1, order), sh_norm=sn, sh_phase=sp, layout=layout)
+        assert rel(g, t.backward_zernike_row(c, zorder, n, order)) < 1e-13, (n, order)
+    # the parity-changing truncation is not "drop the rows at or above the
+    # grid order": that reading gives a different grid
+    e = oracle.st_row_skipping_expansion(c, zorder, 13, 11)
+    assert np.all(e[[l * (l + 1) // 2 for l in range(1, 11, 2)]] == 0.0)
+    naive = np.zeros_like(e)
+    row = c[oracle.zt_num_coeffs(13):oracle.zt_num_coeffs(14)]
+    for l in range(1, 11, 2):
+        naive[l * (l + 1) // 2:l * (l + 1) // 2 + l + 1] = row[l * l // 4:l * l // 4 + l + 1]
+    g_naive = oracle.st_backward(naive, 11, sh_norm=sn, sh_phase=sp, layout=layout)
+    assert rel(g_naive, t.backward_zernike_row(c, zorder, 13, 11)) > 1e-2
+
+
 # ---------------------------------------------------------------- known answers
 def ball_grid_coords(order):
     nphi, nth, nr = oracle.zt_shape(order)
