@@ -323,6 +323,32 @@ zgl_status zgl_profile_stats(zgl_plan *plan, zgl_stage_stats *out, int max_stage
  * exceptions; the C ABI returns a status and keeps the text here). */
 const char *zgl_last_error_string(void);
 
+/* Device allocator. zest allocates its buffers through its AlignedAllocator
+ * (include/alignment.hpp:157-187), whose failure is std::bad_alloc or
+ * std::bad_array_new_length (include/alignment.hpp:169-176); this is the
+ * counterpart for the device
+ * memory of the library, with no counterpart as a hook in zest: it is for
+ * callers that keep device memory in their own pool or caching allocator
+ * (a framework's, an arena sized up front) and for tests that make
+ * allocations fail.
+ *
+ * Every device allocation the library makes after the call -- tables, scratch,
+ * work lists, staging buffers -- is requested from alloc(bytes, user), which
+ * returns device memory of the current device, aligned to 256 bytes, or NULL
+ * when it cannot. A NULL makes the call that needed the memory return
+ * ZGL_ERROR_OUT_OF_MEMORY (the message names the request); the plan stays
+ * valid, holds nothing of the failed request and can be used again. Memory is
+ * handed back with free(ptr, bytes, user) by the allocator that made it, after
+ * the library has waited for all device work (hipDeviceSynchronize), so the
+ * allocator may be changed at any time, also while plans hold memory. alloc ==
+ * NULL (free ignored) restores hipMalloc / hipFree. Process-wide; may be
+ * called from any thread; alloc and free may be called from any thread that
+ * calls into the library. The static scratch of zgl_forward_noalloc is not
+ * allocated and does not pass through here. */
+typedef void *(*zgl_device_alloc_fn)(size_t bytes, void *user);
+typedef void (*zgl_device_free_fn)(void *ptr, size_t bytes, void *user);
+zgl_status zgl_set_device_allocator(zgl_device_alloc_fn alloc, zgl_device_free_fn free, void *user);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,3 +142,84 @@ def test_binding_rejects_undersized_buffers():
         sh.forward(gs[:-1], np.zeros((zest_amd.st_num_coeffs(16), 2)), 16)
     with pytest.raises(ValueError):
         sh._jobs([(16, np.zeros(3), np.zeros(3))])
+
+
+def _no_gpu_visible() -> bool:
+    import torch
+    return torch.cuda.device_count() == 0
+
+
+def test_allocator_hook_out_of_memory_on_first_request():
+    """An installed device allocator that returns NULL turns the first device
+    allocation into ZGL_ERROR_OUT_OF_MEMORY (zest's std::bad_alloc,
+    include/alignment.hpp:169-176): no plan is returned, the message names the
+    request, nothing is left allocated, and restoring hipMalloc leaves the
+    library as it was. No device work is reached: the first allocation of a
+    plan comes before its first kernel."""
+    import zest_amd
+    lib = zest_amd.LIB
+    calls = []
+    hook = zest_amd.DeviceAllocator(lambda n: calls.append(n) or 0, lambda p, n: None)
+    with hook:
+        for kind, layout in ((zest_amd.ZERNIKE, 0), (zest_amd.SH, 1)):
+            h = ctypes.c_void_p()
+            st = lib.zgl_plan_create(ctypes.byref(h), kind, 1, 0, -1, layout, 8, 0)
+            assert st == 2, (st, lib.zgl_last_error_string())
+            assert not h.value
+            msg = lib.zgl_last_error_string().decode()
+            assert "installed device allocator returned NULL" in msg and "table allocation" in msg, msg
+        # order 0 allocates nothing and succeeds under the failing allocator
+        h = ctypes.c_void_p()
+        assert lib.zgl_plan_create(ctypes.byref(h), 0, 1, 0, -1, 0, 0, 0) == 0
+        assert lib.zgl_plan_destroy(h) == 0
+    assert len(calls) == 2 and all(n > 0 for n in calls)
+    assert hook.live == 0
+    # an allocator without its free function is refused
+    assert lib.zgl_set_device_allocator(hook._c_alloc, zest_amd.DEVICE_FREE_FN(), None) == 1
+    assert b"free function" in lib.zgl_last_error_string()
+    zest_amd.restore_default_allocator()
+
+
+@pytest.mark.skipif(not _no_gpu_visible(),
+                    reason="hands host memory to the library as device memory: only without a GPU")
+def test_allocator_hook_releases_everything_when_a_later_request_fails():
+    """OOM injected at every point of plan creation in turn: the first k
+    requests succeed (host memory, 256-byte aligned; without a GPU no kernel
+    runs on it), request k + 1 fails. Whatever the failure point -- and the
+    HIP error that ends the sequence once the first table kernel is checked --
+    every block that was handed out is returned through the allocator's free,
+    with the size it was requested at, and no plan is returned."""
+    import zest_amd
+    lib = zest_amd.LIB
+    seen_statuses = set()
+    for kind, order, precision in ((zest_amd.ZERNIKE, 12, 0), (zest_amd.ZERNIKE, 12, 1), (zest_amd.SH, 16, 0)):
+        k = 0
+        while True:
+            held = {}
+
+            def alloc(n, k=k, held=held):
+                if len(held) >= k:
+                    return 0
+                buf = ctypes.create_string_buffer(n + 256)
+                addr = (ctypes.addressof(buf) + 255) // 256 * 256
+                held[addr] = (buf, n)
+                return addr
+
+            def free(p, n, held=held):
+                assert p in held and held[p][1] == n
+                del held[p]
+
+            hook = zest_amd.DeviceAllocator(alloc, free)
+            with hook:
+                h = ctypes.c_void_p()
+                st = lib.zgl_plan_create(ctypes.byref(h), kind, 1, 0, -1, 0, order, precision)
+            assert st in (2, 3), (kind, order, k, st)
+            assert not h.value
+            assert hook.live == 0 and not held, (kind, order, k, hook.live)
+            seen_statuses.add(st)
+            if st == 3:  # past the allocations before the first checked kernel
+                break
+            assert hook.calls == k + 1, (k, hook.calls)
+            k += 1
+        assert k >= 3, (kind, k)
+    assert seen_statuses == {2, 3}

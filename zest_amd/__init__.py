@@ -106,7 +106,14 @@ def _load():
     lib.zgl_profile_enable.restype = i32
     lib.zgl_profile_stats.argtypes = [vp, ctypes.POINTER(StageStats), i32, ctypes.POINTER(i32)]
     lib.zgl_profile_stats.restype = i32
+    lib.zgl_set_device_allocator.argtypes = [DEVICE_ALLOC_FN, DEVICE_FREE_FN, vp]
+    lib.zgl_set_device_allocator.restype = i32
     return lib
+
+
+# zgl_device_alloc_fn / zgl_device_free_fn of include/zgl.h
+DEVICE_ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+DEVICE_FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
 
 
 class Job(ctypes.Structure):
@@ -133,6 +140,7 @@ EXPORTS = (
     "zgl_power_spectrum", "zgl_cross_power_spectrum", "zgl_rotate", "zgl_polar_rotate",
     "zgl_wignerd_pi2", "zgl_euler_angles_from_rotation_matrix",
     "zgl_profile_enable", "zgl_profile_stats", "zgl_last_error_string",
+    "zgl_set_device_allocator",
 )
 
 
@@ -502,6 +510,76 @@ def euler_angles_from_rotation_matrix(matrix) -> np.ndarray:
     out = np.zeros(3)
     LIB.zgl_euler_angles_from_rotation_matrix(m.ctypes.data, out.ctypes.data)
     return out
+
+
+class DeviceAllocator:
+    """Routes the library's device allocations through two Python callables
+    (zgl_set_device_allocator): ``alloc(nbytes) -> int address or 0`` and
+    ``free(address, nbytes)``. A context manager: installed on entry, hipMalloc
+    restored on exit. Memory still held by plans then goes back through
+    ``free`` when the plans release it, so the callables must outlive it (the
+    object keeps the C callbacks alive until it is garbage-collected).
+    ``live`` counts the allocations made through it and not yet freed."""
+
+    def __init__(self, alloc, free):
+        self._alloc, self._free = alloc, free
+        self.live = 0
+        self.calls = 0
+
+        def c_alloc(nbytes, _user):
+            self.calls += 1
+            try:
+                p = int(self._alloc(int(nbytes)) or 0)
+            except Exception:  # e.g. torch's OutOfMemoryError: report NULL
+                p = 0
+            if p:
+                self.live += 1
+            return p or None
+
+        def c_free(ptr, nbytes, _user):
+            self.live -= 1
+            self._free(int(ptr), int(nbytes))
+
+        self._c_alloc = DEVICE_ALLOC_FN(c_alloc)
+        self._c_free = DEVICE_FREE_FN(c_free)
+
+    def install(self):
+        _check(LIB.zgl_set_device_allocator(self._c_alloc, self._c_free, None))
+
+    def __enter__(self):
+        self.install()
+        return self
+
+    def __exit__(self, *exc):
+        restore_default_allocator()
+        return False
+
+
+def restore_default_allocator():
+    """hipMalloc / hipFree for the library's device memory again."""
+    _check(LIB.zgl_set_device_allocator(DEVICE_ALLOC_FN(), DEVICE_FREE_FN(), None))
+
+
+def torch_device_allocator(fail_after: int | None = None) -> DeviceAllocator:
+    """A DeviceAllocator over torch's caching allocator on the current device
+    (torch.cuda.caching_allocator_alloc / _delete), so the library's tables
+    and scratch count in torch's memory statistics and reuse its cache.
+    ``fail_after=k`` makes every request after the first k return NULL (out of
+    memory), for testing."""
+    import torch
+
+    state = {"n": 0}
+
+    def alloc(nbytes):
+        state["n"] += 1
+        if fail_after is not None and state["n"] > fail_after:
+            return 0
+        return torch.cuda.caching_allocator_alloc(max(int(nbytes), 1))
+
+    def free(ptr, _nbytes):
+        torch.cuda.caching_allocator_delete(ptr)
+
+    return DeviceAllocator(alloc, free)
 
 
 def _stream_ptr(stream, mem):

@@ -13,6 +13,7 @@
 #include <new>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "zgl_internal.hpp"
@@ -25,6 +26,78 @@ void check_hip(hipError_t e, const char* what) {
                         ? ZGL_ERROR_OUT_OF_MEMORY
                         : ZGL_ERROR_HIP;
     throw Error{st, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+namespace {
+
+// The allocator installed by zgl_set_device_allocator, and the live
+// allocations made through one (with the allocator that made each, so that
+// it gets its memory back whatever is installed by then).
+struct DeviceAllocator {
+    zgl_device_alloc_fn alloc = nullptr;
+    zgl_device_free_fn free = nullptr;
+    void* user = nullptr;
+};
+struct HookedBlock {
+    DeviceAllocator by;
+    size_t bytes;
+};
+std::mutex g_alloc_mutex;
+DeviceAllocator g_allocator;
+std::unordered_map<void*, HookedBlock> g_hooked;
+
+}  // namespace
+
+void* dmalloc(size_t bytes, const char* what) {
+    DeviceAllocator a;
+    {
+        std::lock_guard<std::mutex> lock(g_alloc_mutex);
+        a = g_allocator;
+    }
+    if (!a.alloc) {
+        void* p = nullptr;
+        check_hip(hipMalloc(&p, bytes), what);
+        return p;
+    }
+    void* p = a.alloc(bytes, a.user);
+    if (!p)
+        throw Error{ZGL_ERROR_OUT_OF_MEMORY,
+                    std::string(what) + ": the installed device allocator returned NULL for " +
+                        std::to_string(bytes) + " bytes"};
+    if (reinterpret_cast<uintptr_t>(p) % 256 != 0) {
+        a.free(p, bytes, a.user);
+        throw Error{ZGL_ERROR_INVALID_VALUE,
+                    std::string(what) + ": the installed device allocator returned memory not aligned to 256 bytes"};
+    }
+    try {
+        std::lock_guard<std::mutex> lock(g_alloc_mutex);
+        g_hooked.emplace(p, HookedBlock{a, bytes});
+    } catch (...) {
+        a.free(p, bytes, a.user);
+        throw;
+    }
+    return p;
+}
+
+void dfree(void* p) noexcept {
+    if (!p) return;
+    HookedBlock b{};
+    bool hooked = false;
+    {
+        std::lock_guard<std::mutex> lock(g_alloc_mutex);
+        auto it = g_hooked.find(p);
+        if (it != g_hooked.end()) {
+            b = it->second;
+            g_hooked.erase(it);
+            hooked = true;
+        }
+    }
+    if (!hooked) {
+        (void)hipFree(p);  // synchronizes with the device itself
+        return;
+    }
+    (void)hipDeviceSynchronize();
+    b.by.free(p, b.bytes, b.by.user);
 }
 
 namespace {
@@ -62,8 +135,7 @@ struct Scratch {
     double* rcoef = nullptr;  // refinement corrections / scaled coefficients (chunk)
     size_t rcoef_cap = 0;     // doubles
     void release() {
-        for (void* q : {spec, flm, (void*)rgrid, (void*)rcoef})
-            if (q) (void)hipFree(q);
+        for (void* q : {spec, flm, (void*)rgrid, (void*)rcoef}) dfree(q);
         *this = Scratch{};
     }
 };
@@ -179,10 +251,10 @@ void ensure(T*& ptr, size_t& cap, size_t need) {
     if (need <= cap) return;
     no_alloc_check(need, cap);
     if (ptr) check_hip(hipDeviceSynchronize(), "scratch growth");
-    if (ptr) check_hip(hipFree(ptr), "free scratch");
+    dfree(ptr);
     ptr = nullptr;
     cap = 0;
-    check_hip(hipMalloc((void**)&ptr, need * sizeof(T)), "scratch allocation");
+    ptr = dmalloc_n<T>(need, "scratch allocation");
     cap = need;
 }
 
@@ -191,7 +263,7 @@ void ragged_release(zgl_plan::Ragged& r) {
         if (c.done) (void)hipEventSynchronize(c.done);
         if (c.exec) (void)hipGraphExecDestroy(c.exec);
         if (c.graph) (void)hipGraphDestroy(c.graph);
-        if (c.offs) (void)hipFree(c.offs);
+        dfree(c.offs);
         if (c.done) (void)hipEventDestroy(c.done);
     }
     for (auto l : r.lanes)
@@ -206,10 +278,10 @@ void ensure_bytes(void*& ptr, size_t& cap, size_t need) {
     if (need <= cap) return;
     no_alloc_check(need, cap);
     if (ptr) check_hip(hipDeviceSynchronize(), "scratch growth");
-    if (ptr) check_hip(hipFree(ptr), "free scratch");
+    dfree(ptr);
     ptr = nullptr;
     cap = 0;
-    check_hip(hipMalloc(&ptr, need), "scratch allocation");
+    ptr = dmalloc(need, "scratch allocation");
     cap = need;
 }
 
@@ -633,10 +705,10 @@ zgl_status zgl_plan_destroy(zgl_plan* plan) {
     plan->main.release();
     ragged_release(plan->rg);
     plan->eval.release();
-    for (auto& kv : plan->rot_work) (void)hipFree(kv.second.first);
+    for (auto& kv : plan->rot_work) dfree(kv.second.first);
     for (void* q : {(void*)plan->dgrid, (void*)plan->dcoef, (void*)plan->dpts, (void*)plan->wtab,
                     (void*)plan->rtemp, (void*)plan->dang})
-        if (q) (void)hipFree(q);
+        dfree(q);
     delete plan;
     return ZGL_SUCCESS;
 }
@@ -1233,6 +1305,13 @@ zgl_status zgl_profile_stats(zgl_plan* plan, zgl_stage_stats* out, int max_stage
 
 const char* zgl_last_error_string(void) { return g_last_error.c_str(); }
 
+zgl_status zgl_set_device_allocator(zgl_device_alloc_fn alloc, zgl_device_free_fn free, void* user) {
+    if (alloc && !free) return fail(ZGL_ERROR_INVALID_VALUE, "an allocator needs its free function");
+    std::lock_guard<std::mutex> lock(g_alloc_mutex);
+    g_allocator = alloc ? DeviceAllocator{alloc, free, user} : DeviceAllocator{};
+    return ZGL_SUCCESS;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------- evaluation
@@ -1383,12 +1462,11 @@ const double* wigner_for(zgl_plan* p, int order, hipStream_t s) {
     if (p->wtab_order < order) {
         if (p->wtab) {
             check_hip(hipDeviceSynchronize(), "d table growth");
-            check_hip(hipFree(p->wtab), "free d table");
+            dfree(p->wtab);
             p->wtab = nullptr;
             p->wtab_order = 0;
         }
-        check_hip(hipMalloc((void**)&p->wtab, std::max<long>(wignerd_size(order), 1) * sizeof(double)),
-                  "d table allocation");
+        p->wtab = dmalloc_n<double>((size_t)std::max<long>(wignerd_size(order), 1), "d table allocation");
         wignerd_table(order, p->wtab, s);
         p->wtab_order = order;
     }
@@ -1402,16 +1480,21 @@ std::pair<int4*, int> rot_work_for(zgl_plan* p, bool fused, int order, int nb, h
     if (p->rot_work.size() >= 64) {
         // many distinct batch sizes: start the cache over
         check_hip(hipDeviceSynchronize(), "work list cache");
-        for (auto& kv : p->rot_work) (void)hipFree(kv.second.first);
+        for (auto& kv : p->rot_work) dfree(kv.second.first);
         p->rot_work.clear();
     }
     const std::vector<int4> w = fused ? rotation_fused_work(p->kind == ZGL_ZERNIKE, order, nb)
                                       : rotation_work(p->kind == ZGL_ZERNIKE, order, nb);
-    int4* d = nullptr;
-    check_hip(hipMalloc((void**)&d, std::max<size_t>(w.size(), 1) * sizeof(int4)), "work list");
-    if (!w.empty()) {
-        check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s), "work list");
-        check_hip(hipStreamSynchronize(s), "work list");
+    int4* d = dmalloc_n<int4>(w.size(), "work list");
+    try {
+        if (!w.empty()) {
+            check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s),
+                      "work list");
+            check_hip(hipStreamSynchronize(s), "work list");
+        }
+    } catch (...) {
+        dfree(d);
+        throw;
     }
     return p->rot_work.emplace(key, std::make_pair(d, (int)w.size())).first->second;
 }

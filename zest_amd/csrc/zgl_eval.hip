@@ -231,20 +231,24 @@ template <class T>
 void grow(T*& p, size_t& cap, size_t need) {
     if (need <= cap) return;
     if (p) check_hip(hipDeviceSynchronize(), "scratch growth");
-    if (p) check_hip(hipFree(p), "free scratch");
+    dfree(p);
     p = nullptr;
     cap = 0;
-    check_hip(hipMalloc((void**)&p, std::max<size_t>(need, 1) * sizeof(T)), "scratch allocation");
+    p = dmalloc_n<T>(need, "scratch allocation");
     cap = need;
 }
 
 int4* upload(const std::vector<int4>& w, hipStream_t s) {
-    int4* d = nullptr;
-    check_hip(hipMalloc((void**)&d, std::max<size_t>(w.size(), 1) * sizeof(int4)), "work list");
-    if (!w.empty()) {
-        check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s),
-                  "work list");
-        check_hip(hipStreamSynchronize(s), "work list");
+    int4* d = dmalloc_n<int4>(w.size(), "work list");
+    try {
+        if (!w.empty()) {
+            check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s),
+                      "work list");
+            check_hip(hipStreamSynchronize(s), "work list");
+        }
+    } catch (...) {
+        dfree(d);
+        throw;
     }
     return d;
 }
@@ -265,10 +269,9 @@ constexpr int kShColumns = 64;                          // SH instances per chun
 }  // namespace
 
 void EvalCache::release() {
-    for (double* p : {pts, ptab, rtab, trig, flm, spec})
-        if (p) (void)hipFree(p);
-    for (auto& kv : leg_work) (void)hipFree(kv.second.first);
-    for (auto& kv : rad_work) (void)hipFree(kv.second.first);
+    for (double* p : {pts, ptab, rtab, trig, flm, spec}) dfree(p);
+    for (auto& kv : leg_work) dfree(kv.second.first);
+    for (auto& kv : rad_work) dfree(kv.second.first);
     *this = EvalCache{};
 }
 
@@ -347,7 +350,7 @@ void evaluate(EvalCache& ec, const EvalCall& e, hipStream_t s, const EvalTimer& 
     if (lw == ec.leg_work.end() && ec.leg_work.size() >= 64) {
         // many distinct point counts: start the cache over
         check_hip(hipDeviceSynchronize(), "work list cache");
-        for (auto& kv : ec.leg_work) (void)hipFree(kv.second.first);
+        for (auto& kv : ec.leg_work) dfree(kv.second.first);
         ec.leg_work.clear();
         lw = ec.leg_work.end();
     }

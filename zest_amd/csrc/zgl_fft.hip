@@ -897,9 +897,14 @@ bool tile_fits(int n, const int* rad, int npass, int T, size_t csize, bool mf, b
 
 template <class C>
 C* upload(const std::vector<C>& v, hipStream_t s, const char* what) {
-    C* d = nullptr;
-    check_hip(hipMalloc(&d, std::max<size_t>(v.size(), 1) * sizeof(C)), what);
-    if (!v.empty()) check_hip(hipMemcpyAsync(d, v.data(), v.size() * sizeof(C), hipMemcpyHostToDevice, s), what);
+    C* d = dmalloc_n<C>(v.size(), what);
+    try {
+        if (!v.empty())
+            check_hip(hipMemcpyAsync(d, v.data(), v.size() * sizeof(C), hipMemcpyHostToDevice, s), what);
+    } catch (...) {
+        dfree(d);
+        throw;
+    }
     return d;
 }
 
@@ -1016,8 +1021,7 @@ void fft_plan_create(FftPlan& p, int n, hipStream_t s) {
 }
 
 void fft_plan_destroy(FftPlan& p) {
-    for (void* q : {(void*)p.tw, (void*)p.tw32, (void*)p.twp, (void*)p.twp32, (void*)p.perm})
-        if (q) (void)hipFree(q);
+    for (void* q : {(void*)p.tw, (void*)p.tw32, (void*)p.twp, (void*)p.twp32, (void*)p.perm}) dfree(q);
     p = FftPlan{};
 }
 

@@ -539,9 +539,13 @@ bool fused_env() {
 
 int2* upload_pairs(const std::vector<int2>& v, hipStream_t s) {
     if (v.empty()) return nullptr;
-    int2* d = nullptr;
-    check_hip(hipMalloc((void**)&d, v.size() * sizeof(int2)), "work list");
-    check_hip(hipMemcpyAsync(d, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, s), "work list");
+    int2* d = dmalloc_n<int2>(v.size(), "work list");
+    try {
+        check_hip(hipMemcpyAsync(d, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, s), "work list");
+    } catch (...) {
+        dfree(d);
+        throw;
+    }
     return d;
 }
 
@@ -562,7 +566,7 @@ void zt_fused_tables(OrderTables& t, hipStream_t s) {
     const long total = (long)order * t.fuse_nkc * t.fuse_dp * kFK;
     for (int tr = 0; tr < 2; ++tr) {
         float*& out = tr ? t.rfuse_b : t.rfuse_f;
-        check_hip(hipMalloc((void**)&out, total * sizeof(float)), "fused radial table");
+        out = dmalloc_n<float>((size_t)total, "fused radial table");
         fused_rtab_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(t.rtab, out, order, g.ncol, t.fuse_nkc,
                                                                          t.fuse_dp, tr, total);
         check_hip(hipGetLastError(), "fused radial table");

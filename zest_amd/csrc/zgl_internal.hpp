@@ -26,6 +26,36 @@ struct Error {
 
 void check_hip(hipError_t e, const char* what);  // throws Error
 
+// Every device allocation of the library: through the allocator installed
+// with zgl_set_device_allocator, else hipMalloc. dmalloc throws Error
+// (ZGL_ERROR_OUT_OF_MEMORY when the memory is not there); dfree returns the
+// memory to the allocator that made it, after the device has finished the
+// work queued on it, and ignores NULL.
+void* dmalloc(size_t bytes, const char* what);
+void dfree(void* p) noexcept;
+
+template <class T>
+T* dmalloc_n(size_t count, const char* what) {
+    return static_cast<T*>(dmalloc((count ? count : 1) * sizeof(T), what));
+}
+
+// A device buffer that is freed when it goes out of scope (temporaries of
+// table generation: an exception on the way leaks nothing).
+struct DeviceTemp {
+    void* p = nullptr;
+    DeviceTemp() = default;
+    DeviceTemp(const DeviceTemp&) = delete;
+    DeviceTemp& operator=(const DeviceTemp&) = delete;
+    ~DeviceTemp() { dfree(p); }
+    template <class T>
+    T* alloc(size_t count, const char* what) {
+        dfree(p);
+        p = nullptr;
+        p = dmalloc_n<T>(count, what);
+        return static_cast<T*>(p);
+    }
+};
+
 // Mixed-radix DIT plan of one real-DFT length (device tables + pass list).
 struct FftDesc {
     int n;

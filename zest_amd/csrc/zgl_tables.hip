@@ -218,17 +218,19 @@ unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 409
 
 template <class T>
 T* dev_alloc(size_t count) {
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    check_hip(hipMalloc(&p, count * sizeof(T)), "table allocation");
-    return static_cast<T*>(p);
+    return dmalloc_n<T>(count, "table allocation");
 }
 
 int4* upload_work(const std::vector<int4>& w, hipStream_t s) {
     int4* d = dev_alloc<int4>(w.size());
-    if (!w.empty())
-        check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s),
-                  "upload work list");
+    try {
+        if (!w.empty())
+            check_hip(hipMemcpyAsync(d, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice, s),
+                      "upload work list");
+    } catch (...) {
+        dfree(d);
+        throw;
+    }
     return d;
 }
 
@@ -348,18 +350,21 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
     g.nm = order;
     if (order == 0) return;
 
-    // Latitude nodes and weights (all of them), then the north half.
-    double* z = dev_alloc<double>(g.nth);
-    double* w = dev_alloc<double>(g.nth);
+    // Latitude nodes and weights (all of them), then the north half. Each
+    // allocation lands in t (freed by free_tables) or in a DeviceTemp at once,
+    // so an allocation that fails part way leaks nothing.
+    DeviceTemp zt, wt;
+    double* z = zt.alloc<double>(g.nth, "table allocation");
+    double* w = wt.alloc<double>(g.nth, "table allocation");
     gl_kernel<<<(g.nth + 1) / 2 / 64 + 1, 64, 0, s>>>(g.nth, z, w);
-    double* zh = dev_alloc<double>(g.jh);
+    t.zh = dev_alloc<double>(g.jh);
+    double* zh = t.zh;
     t.wlat = dev_alloc<double>(g.jh);
     half_nodes_kernel<<<g.jh / 64 + 1, 64, 0, s>>>(g.nth, z, w, zh, t.wlat);
 
     // P table, unless the Legendre kernels recompute their P blocks: at high
     // order (the table is O(N^3): 67.6 MB at N = 256 in FP64) when the
     // streaming kernels run; ZGL_PLM=kernel / table forces either.
-    t.zh = zh;
     t.plm_qm = sh_norm == ZGL_SHNORM_QM ? 1 : 0;
     t.plm_phase = (double)sh_phase;
     {
@@ -380,8 +385,9 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
 
     if (kind == ZGL_ZERNIKE) {
         const int nr = g.ncol;
-        double* x = dev_alloc<double>(nr);
-        double* v = dev_alloc<double>(nr);
+        DeviceTemp xt, vt;
+        double* x = xt.alloc<double>(nr, "table allocation");
+        double* v = vt.alloc<double>(nr, "table allocation");
         gl_kernel<<<(nr + 1) / 2 / 64 + 1, 64, 0, s>>>(nr, x, v);
         t.wrad = dev_alloc<double>(nr);
         radial_nodes_kernel<<<nr / 64 + 1, 64, 0, s>>>(nr, x, v, t.wrad);
@@ -391,13 +397,9 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
             order, nr, zernike_norm == ZGL_ZNORM_NORMED ? 1 : 0, x, t.rtab);
         check_hip(hipGetLastError(), "table kernels");
         check_hip(hipStreamSynchronize(s), "table generation");
-        check_hip(hipFree(x), "free");
-        check_hip(hipFree(v), "free");
     }
     check_hip(hipGetLastError(), "table kernels");
     check_hip(hipStreamSynchronize(s), "table generation");
-    check_hip(hipFree(z), "free");
-    check_hip(hipFree(w), "free");
 
     fft_plan_create(t.fft, g.nphi, s);
 
@@ -445,7 +447,7 @@ void free_tables(OrderTables& t) {
                     (void*)t.rtab32, (void*)t.work_lf, (void*)t.work_lb, (void*)t.work_rf,
                     (void*)t.work_rb, (void*)t.work_sb, (void*)t.rfuse_f, (void*)t.rfuse_b,
                     (void*)t.work_ff, (void*)t.work_fb})
-        if (p) (void)hipFree(p);
+        dfree(p);
     fft_plan_destroy(t.fft);
     t = OrderTables{};
 }
