@@ -45,17 +45,23 @@
 extern "C" {
 #endif
 
+/* Each enum type has a last enumerator 0x7fffffff, as C APIs commonly do: the
+ * type is int-sized and any int the caller passes is a value of it (C++ code
+ * reading an out-of-range value of a narrower enum would be undefined), so an
+ * invalid value reaches the library's argument checks. */
 typedef enum zgl_status {
     ZGL_SUCCESS = 0,
     ZGL_ERROR_INVALID_VALUE = 1,
     ZGL_ERROR_OUT_OF_MEMORY = 2,
     ZGL_ERROR_HIP = 3,
-    ZGL_ERROR_NOT_SUPPORTED = 4
+    ZGL_ERROR_NOT_SUPPORTED = 4,
+    ZGL_STATUS_MAX_ENUM = 0x7fffffff /* every int is a value of the type */
 } zgl_status;
 
 typedef enum zgl_kind {
     ZGL_ZERNIKE = 0, /* zt::GLQTransformer: BallGLQGrid <-> RealZernikeExpansion */
-    ZGL_SH = 1       /* st::GLQTransformer: SphereGLQGrid <-> RealSHExpansion */
+    ZGL_SH = 1,      /* st::GLQTransformer: SphereGLQGrid <-> RealSHExpansion */
+    ZGL_KIND_MAX_ENUM = 0x7fffffff
 } zgl_kind;
 
 /* zest::zt::ZernikeNorm (include/zernike_conventions.hpp:35) */
@@ -77,14 +83,16 @@ typedef enum zgl_precision {
     ZGL_FP32_REFINE = 2, /* ZGL_FP32, plus one step of iterative refinement on the
                             forward transform: c += F32(g - B64(c)), the residual
                             formed by an FP64 backward transform */
-    ZGL_BF16X3 = 3       /* ZGL_FP32 with the Zernike Legendre products formed on the
+    ZGL_BF16X3 = 3,      /* ZGL_FP32 with the Zernike Legendre products formed on the
                             BF16 MFMA from operands split into three BF16 values
                             (six partial products, FP32 accumulation) */
+    ZGL_PRECISION_MAX_ENUM = 0x7fffffff
 } zgl_precision;
 
 typedef enum zgl_mem {
     ZGL_MEM_DEVICE = 0, /* grid and coefficient pointers are device memory */
-    ZGL_MEM_HOST = 1    /* host memory; staged through device buffers, synchronous */
+    ZGL_MEM_HOST = 1,   /* host memory; staged through device buffers, synchronous */
+    ZGL_MEM_MAX_ENUM = 0x7fffffff
 } zgl_mem;
 
 typedef struct zgl_plan zgl_plan;

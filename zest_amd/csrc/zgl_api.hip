@@ -109,6 +109,14 @@ zgl_status fail(zgl_status s, const std::string& msg) {
     return s;
 }
 
+// A pointer to T that is not aligned to sizeof(T) is no valid T* (and the
+// kernels' loads assume element alignment).
+template <class... P>
+bool misaligned(const P*... ptrs) {
+    return ((reinterpret_cast<uintptr_t>(ptrs) % sizeof(P) != 0) || ...);
+}
+constexpr const char* kMisaligned = "pointer not aligned to its element size";
+
 constexpr size_t kMaxOrder = 4096;
 
 size_t env_mb(const char* name, size_t dflt) {
@@ -757,6 +765,7 @@ zgl_status forward_entry(zgl_plan* plan, const G* grid, size_t grid_order, doubl
         plan->order = grid_order;
         if (batch == 0 || nt == 0) return ZGL_SUCCESS;
         if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
+        if (misaligned(grid, coeffs)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
             const Inst<const G> gi{grid, gs, nullptr};
@@ -815,6 +824,7 @@ zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order
         plan->order = grid_order;
         if (batch == 0 || grid_order == 0) return ZGL_SUCCESS;
         if (!grid || (!coeffs && nt > 0)) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
+        if (misaligned(grid, coeffs)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
             const Inst<const double> ci{coeffs, cs, nullptr};
@@ -928,6 +938,9 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         if (jobs[i].order > kMaxOrder) return fail(ZGL_ERROR_INVALID_VALUE, "job order exceeds the supported maximum");
         if (jobs[i].order && (!jobs[i].grid || !jobs[i].coeffs))
             return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer in a job");
+        if (jobs[i].order && misaligned(static_cast<const double*>(jobs[i].grid),
+                                        static_cast<const double*>(jobs[i].coeffs)))
+            return fail(ZGL_ERROR_INVALID_VALUE, std::string(kMisaligned) + " in a job");
     }
     return guarded([&]() -> zgl_status {
         hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1153,6 +1166,7 @@ extern "C" zgl_status zgl_forward_noalloc(zgl_plan* plan, const double* grid, si
         plan->order = grid_order;
         if (batch == 0 || nt == 0) return ZGL_SUCCESS;
         if (!grid || !coeffs) return fail(ZGL_ERROR_INVALID_VALUE, "NULL grid or coefficient pointer");
+        if (misaligned(grid, coeffs)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
         const Geom& g = plan->tables.find(grid_order)->second.g;
         // bytes of scratch per instance: spectrum and flm (SH: staging) in the
         // widest arithmetic the call runs, and the refinement buffers
@@ -1344,6 +1358,7 @@ extern "C" zgl_status zgl_evaluate(zgl_plan* plan, const double* coeffs, size_t 
     if (npts == 0 || batch == 0) return ZGL_SUCCESS;
     if (!values || !longitudes || !colatitudes || (zt && !radii) || (!coeffs && exp_order > 0))
         return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    if (misaligned(values, longitudes, colatitudes, radii, coeffs)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
     return guarded([&]() -> zgl_status {
         hipStream_t s = static_cast<hipStream_t>(stream);
         EvalCall e{plan->kind, plan->znorm, plan->shnorm == ZGL_SHNORM_QM ? 1 : 0, (double)plan->phase,
@@ -1407,6 +1422,7 @@ zgl_status spectrum_entry(zgl_plan* plan, const double* a, const double* b, size
     if (batch > 65535) return fail(ZGL_ERROR_INVALID_VALUE, "batch exceeds 65535 instances per call");
     if (order == 0 || batch == 0) return ZGL_SUCCESS;
     if (!a || !b || !out) return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    if (misaligned(a, b, out)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
     return guarded([&]() -> zgl_status {
         hipStream_t s = static_cast<hipStream_t>(stream);
         if (mem == ZGL_MEM_DEVICE) {
@@ -1511,6 +1527,7 @@ zgl_status rotate_entry(zgl_plan* plan, double* coeffs, size_t order, const doub
     if (batch > 65535) return fail(ZGL_ERROR_INVALID_VALUE, "batch exceeds 65535 instances per call");
     if (order < 2 || batch == 0) return ZGL_SUCCESS;
     if (!coeffs || !angles) return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    if (misaligned(coeffs, angles)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
     const bool zt = plan->kind == ZGL_ZERNIKE;
     const size_t nang = (polar ? 1 : 3) * batch;
     return guarded([&]() -> zgl_status {
@@ -1583,6 +1600,7 @@ extern "C" zgl_status zgl_wignerd_pi2(zgl_plan* plan, size_t order, double* out,
     if (mem != ZGL_MEM_DEVICE && mem != ZGL_MEM_HOST) return fail(ZGL_ERROR_INVALID_VALUE, "unknown memory kind");
     if (order == 0) return ZGL_SUCCESS;
     if (!out) return fail(ZGL_ERROR_INVALID_VALUE, "NULL pointer");
+    if (misaligned(out)) return fail(ZGL_ERROR_INVALID_VALUE, kMisaligned);
     return guarded([&]() -> zgl_status {
         hipStream_t s = static_cast<hipStream_t>(stream);
         const double* w = wigner_for(plan, (int)order, s);
