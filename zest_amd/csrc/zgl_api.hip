@@ -193,6 +193,7 @@ struct zgl_plan {
             hipEvent_t done = nullptr;
         } cache[2];
         std::vector<double> lane_cost;  // modelled work per lane of the last build
+        bool grid16 = false;  // every grid of the job list being built is 16-byte aligned
     } rg;
     // zgl_evaluate: tables and staging at the last evaluation grid
     zgl::EvalCache eval;
@@ -453,6 +454,14 @@ bool use_fused(const zgl_plan* p, const OrderTables& t) {
 // T: arithmetic (double or float); G: grid element type in memory.
 // adjoint: the transpose of the backward transform (no quadrature weights,
 // no prefactor, no kappa_n), otherwise the forward transform.
+// Whether every instance base of x is 16-byte aligned (for a ragged batch:
+// the grids of the job list being run).
+template <class P>
+bool aligned16(const zgl_plan* p, const Inst<P>& x) {
+    if (x.offs) return p->rg.grid16;
+    return reinterpret_cast<uintptr_t>(x.base) % 16 == 0 && (x.stride * (long)sizeof(P)) % 16 == 0;
+}
+
 template <class T, class G>
 void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<double> coeffs,
                  size_t nt, size_t batch, hipStream_t s, bool adjoint = false) {
@@ -466,6 +475,7 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
     L.g = &g;
     L.t = &t;
     L.layout_kind = (p->kind == ZGL_SH && p->layout == ZGL_LAYOUT_LATLON) ? GRID_ROWS : GRID_COLS;
+    L.grid_aligned16 = aligned16(p, grid);
     L.nm_used = (int)nt;
     L.weighted = !adjoint;
     L.scale = adjoint ? 1.0 : forward_prefactor(p, g);
@@ -524,6 +534,7 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
     L.g = &g;
     L.t = &t;
     L.layout_kind = (p->kind == ZGL_SH && p->layout == ZGL_LAYOUT_LATLON) ? GRID_ROWS : GRID_COLS;
+    L.grid_aligned16 = aligned16(p, grid);
     L.nm_used = (int)nt;
     L.weighted = adjoint;
     L.scale = adjoint ? forward_prefactor(p, g) : 1.0;
@@ -973,6 +984,8 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
 
         // per-instance element offsets from the lowest grid / coefficient address
         const size_t n = idx.size();
+        R.grid16 = true;
+        for (size_t k : idx) R.grid16 = R.grid16 && reinterpret_cast<uintptr_t>(jobs[k].grid) % 16 == 0;
         double* gbase = jobs[idx[0]].grid;
         double* cbase = jobs[idx[0]].coeffs;
         for (size_t k : idx) {

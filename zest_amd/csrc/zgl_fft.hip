@@ -56,6 +56,7 @@ constexpr int kThreads = 256;
 constexpr int kMaxBigItems = 9;
 constexpr int kMaxRegRadix = 17;  // larger primes use the LDS-staged pass
 constexpr int kBatch = 8;         // global loads in flight per thread
+constexpr bool kBwdSpecPairs = false;  // backward: FP32 spectrum read two values per lane
 
 // Threads per workgroup of a kernel variant: 512 for FP32 tiles of 32 column
 // pairs in the plain and lean variants (their grid rows are read in 128-byte
@@ -69,6 +70,30 @@ __host__ __device__ constexpr int fft_threads() {
 template <class R> struct Cplx;
 template <> struct Cplx<double> { using type = double2; };
 template <> struct Cplx<float> { using type = float2; };
+
+// Vectorised memory phases (VEC kernels, Zernike grids): each lane moves 16
+// bytes per global access -- VG = 16 / sizeof(G) consecutive grid columns of a
+// latitude row, or VS = 2 consecutive FP32 spectrum values -- so a wave keeps
+// 4x (FP32 grids) or 2x (FP64 grids, FP32 spectra) the bytes in flight per
+// load instruction of the one-value-per-lane form. The host takes these
+// variants when the columns of a row are a multiple of VG and every instance
+// base is 16-byte aligned.
+template <class G> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+template <class G, bool VEC> constexpr int grid_vec() { return VEC ? 16 / (int)sizeof(G) : 1; }
+template <class R, bool VEC> constexpr int spec_vec() { return (VEC && sizeof(R) == 4) ? 2 : 1; }
+template <class V> __device__ inline auto vget(const V& v, int k) {
+    if constexpr (sizeof(v.x) == 4) return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+    else return k == 0 ? v.x : v.y;
+}
+template <class V, class S> __device__ inline void vset(V& v, int k, S x) {
+    if constexpr (sizeof(v.x) == 4) {
+        if (k == 0) v.x = x; else if (k == 1) v.y = x; else if (k == 2) v.z = x; else v.w = x;
+    } else {
+        if (k == 0) v.x = x; else v.y = x;
+    }
+}
 
 template <class C> __device__ inline C mk(decltype(C{}.x) x, decltype(C{}.x) y) { C c; c.x = x; c.y = y; return c; }
 template <class C> __device__ inline C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
@@ -523,7 +548,7 @@ constexpr int lean_waves() {
     return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : (fft_threads<T, R, MF>() > kThreads ? (sizeof(R) == 8 ? 2 : 4) : 1);
 }
 
-template <int T, int LK, class R, class G, int MF>
+template <int T, int LK, class R, class G, int MF, bool VEC>
 __global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
 void fft_forward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
@@ -548,7 +573,40 @@ void fft_forward_kernel(
 
     // Loads are issued in batches of kBatch per thread before any LDS store,
     // so each thread keeps that many independent global reads in flight.
-    if (LK == GRID_COLS) {
+    if constexpr (VEC) {
+        // VG columns per lane; the lane's column group is threadIdx.x % (T / VG)
+        constexpr int VG = grid_vec<G, VEC>(), TQ = T / VG, KB = kBatch / 2;
+        using V = typename Vec16<G>::type;
+        static_assert(LK == GRID_COLS && T % VG == 0 && NT % TQ == 0, "vector tile");
+        const int f = tile * T + ((int)threadIdx.x % TQ) * VG;
+        const int h = f / pm.ncol, c = f - h * pm.ncol;
+        const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+        const G* col = g + c;
+        for (int e0 = threadIdx.x; e0 < n * TQ; e0 += NT * KB) {
+            V av[KB], bv[KB];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int e = e0 + u * NT;
+                const int i = e / TQ;
+                av[u] = V{};
+                bv[u] = V{};
+                if (e < n * TQ && h < pm.jh) {
+                    const G* row = col + (long)i * pm.lon_stride;
+                    av[u] = *reinterpret_cast<const V*>(row + (long)jn * pm.lat_stride);
+                    if (js != jn) bv[u] = *reinterpret_cast<const V*>(row + (long)js * pm.lat_stride);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int e = e0 + u * NT;
+                if (e < n * TQ) {
+                    C* dst = data + perm[e / TQ] * T + (e % TQ) * VG;
+#pragma unroll
+                    for (int k = 0; k < VG; ++k) dst[k] = mk<C>((R)vget(av[u], k), (R)vget(bv[u], k));
+                }
+            }
+        }
+    } else if (LK == GRID_COLS) {
         for (int e0 = threadIdx.x; e0 < n * T; e0 += NT * kBatch) {
             R av[kBatch], bv[kBatch];
 #pragma unroll
@@ -599,6 +657,47 @@ void fft_forward_kernel(
 
     run_passes<T, -1, MF, NT>(data, twl, twp, d);
 
+    if constexpr (spec_vec<R, VEC>() == 2) {
+        // two consecutive columns per lane: 16-byte spectrum stores
+        constexpr int TP = T / 2;
+        static_assert(T % 2 == 0 && NT % TP == 0, "column pairs");
+        const int f = tile * T + ((int)threadIdx.x % TP) * 2;
+        const int h = f / pm.ncol, c = f - h * pm.ncol;
+        if (h < pm.jh) {
+            const bool mid = pm.nth / 2 + h == pm.jh - 1 - h;
+            // the same expression as the one-column form, for identical roundings
+            const R W0 = (R)((wlat ? wlat[h] * (wrad ? wrad[c] : 1.0) : 1.0) * scale);
+            const R W1 = (R)((wlat ? wlat[h] * (wrad ? wrad[c + 1] : 1.0) : 1.0) * scale);
+            const R cf = (R)center_factor;
+            float4* sp4 = reinterpret_cast<float4*>(sp);
+            for (int e = threadIdx.x; e < nm * TP; e += NT) {
+                const int m = e / TP, t = (e % TP) * 2;
+                C E[2], O[2];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const C zm = data[m * T + t + k];
+                    const C zc = data[(m ? n - m : 0) * T + t + k];
+                    const C A = mk<C>(R(0.5) * (zm.x + zc.x), R(0.5) * (zm.y - zc.y));
+                    const C B = mk<C>(R(0.5) * (zm.y + zc.y), R(-0.5) * (zm.x - zc.x));
+                    const R W = k ? W1 : W0;
+                    const C Ha = mk<C>(W * A.x, -W * A.y);
+                    const C Hb = mk<C>(W * B.x, -W * B.y);
+                    if (mid) {
+                        E[k] = mk<C>(cf * Ha.x, cf * Ha.y);
+                        O[k] = mk<C>(0, 0);
+                    } else {
+                        E[k] = cadd(Ha, Hb);
+                        O[k] = csub(Ha, Hb);
+                    }
+                }
+                const long fe = ((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c;
+                const long fo = ((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c;
+                sp4[fe / 2] = make_float4(E[0].x, E[0].y, E[1].x, E[1].y);
+                sp4[fo / 2] = make_float4(O[0].x, O[0].y, O[1].x, O[1].y);
+            }
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < nm * T; e += NT) {
         const int m = e / T, t = e % T, h = th, c = tc;
         if (h >= pm.jh) continue;
@@ -626,7 +725,7 @@ void fft_forward_kernel(
     }
 }
 
-template <int T, int LK, class R, class G, int MF>
+template <int T, int LK, class R, class G, int MF, bool VEC>
 __global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
 void fft_backward_kernel(
     FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
@@ -648,6 +747,59 @@ void fft_backward_kernel(
     G* g = grid.at(blockIdx.y);
     const C* sp = spec + (long)blockIdx.y * spec_stride;
     const int F = n / 2 + 1;
+    if constexpr (spec_vec<R, VEC>() == 2 && kBwdSpecPairs) {
+        // two consecutive columns per lane: 16-byte spectrum loads
+        constexpr int TP = T / 2;
+        static_assert(T % 2 == 0 && NT % TP == 0, "column pairs");
+        const int f = tile * T + ((int)threadIdx.x % TP) * 2;
+        const int h = f / pm.ncol, c = f - h * pm.ncol;
+        const bool mid = pm.nth / 2 + h == pm.jh - 1 - h;
+        const float4* sp4 = reinterpret_cast<const float4*>(sp);
+        for (int e0 = threadIdx.x; e0 < F * TP; e0 += NT * kBatch) {
+            float4 ae[kBatch], ao[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * NT;
+                const int m = e / TP;
+                ae[u] = make_float4(0, 0, 0, 0);
+                ao[u] = make_float4(0, 0, 0, 0);
+                if (e < F * TP && h < pm.jh && m < nm) {
+                    ae[u] = sp4[(((long)(m * 2 + 0) * pm.jh + h) * pm.ncol + c) / 2];
+                    ao[u] = sp4[(((long)(m * 2 + 1) * pm.jh + h) * pm.ncol + c) / 2];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * NT;
+                if (e >= F * TP) continue;
+                const int m = e / TP, t0 = (e % TP) * 2;
+                const R mu = m > 0 ? R(0.5) : R(1);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const C a = k ? mk<C>(ae[u].z, ae[u].w) : mk<C>(ae[u].x, ae[u].y);
+                    const C o = k ? mk<C>(ao[u].z, ao[u].w) : mk<C>(ao[u].x, ao[u].y);
+                    C Sa, Sb;
+                    if (mid) {
+                        Sa = mk<C>(mu * a.x, -mu * a.y);
+                        Sb = mk<C>(0, 0);
+                    } else {
+                        const C Ev = cadd(a, o), Dv = csub(a, o);
+                        Sa = mk<C>(mu * Ev.x, -mu * Ev.y);
+                        Sb = mk<C>(mu * Dv.x, -mu * Dv.y);
+                    }
+                    C zm, zp = mk<C>(0, 0);
+                    if (m == 0 || 2 * m == n) {
+                        zm = mk<C>(Sa.x, Sb.x);
+                    } else {
+                        zm = mk<C>(Sa.x - Sb.y, Sa.y + Sb.x);
+                        zp = mk<C>(Sa.x + Sb.y, -Sa.y + Sb.x);
+                    }
+                    data[perm[m] * T + t0 + k] = zm;
+                    if (m > 0 && 2 * m != n) data[perm[n - m] * T + t0 + k] = zp;
+                }
+            }
+        }
+    } else {
     for (int e0 = threadIdx.x; e0 < F * T; e0 += NT * kBatch) {
         C ae[kBatch], ao[kBatch];
 #pragma unroll
@@ -687,10 +839,48 @@ void fft_backward_kernel(
             if (m > 0 && 2 * m != n) data[perm[n - m] * T + t] = zp;
         }
     }
+    }
 
     run_passes<T, +1, MF, NT>(data, twl, twp, d);
 
-    if (LK == GRID_COLS) {
+    if constexpr (VEC) {
+        // VG columns per lane: 16-byte grid stores
+        constexpr int VG = grid_vec<G, VEC>(), TQ = T / VG;
+        using V = typename Vec16<G>::type;
+        static_assert(LK == GRID_COLS && T % VG == 0 && NT % TQ == 0, "vector tile");
+        const int f = tile * T + ((int)threadIdx.x % TQ) * VG;
+        const int h = f / pm.ncol, c = f - h * pm.ncol;
+        if (h < pm.jh) {
+            const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
+            R W[VG];
+#pragma unroll
+            for (int k = 0; k < VG; ++k) {
+                W[k] = 1;
+                if (wlat) {
+                    W[k] = (R)(wlat[h] * (wrad ? wrad[c + k] : 1.0) * scale);
+                    if (jn == js) W[k] *= (R)center_factor;
+                }
+            }
+            G* col = g + c;
+            for (int e = threadIdx.x; e < n * TQ; e += NT) {
+                const int i = e / TQ, t0 = (e % TQ) * VG;
+                V a, b;
+#pragma unroll
+                for (int k = 0; k < VG; ++k) {
+                    C z = data[i * T + t0 + k];
+                    if (wlat) {
+                        z.x *= W[k];
+                        z.y *= W[k];
+                    }
+                    vset(a, k, (G)z.x);
+                    vset(b, k, (G)z.y);
+                }
+                G* row = col + (long)i * pm.lon_stride;
+                *reinterpret_cast<V*>(row + (long)jn * pm.lat_stride) = a;
+                if (js != jn) *reinterpret_cast<V*>(row + (long)js * pm.lat_stride) = b;
+            }
+        }
+    } else if (LK == GRID_COLS) {
         for (int e = threadIdx.x; e < n * T; e += NT) {
             const int i = e / T, t = e % T, h = th, c = tc;
             if (h >= pm.jh) continue;
@@ -783,6 +973,21 @@ K pick_variant(const FftPlan& f, K none, K general, K lean, K wide) {
     return f.lean ? lean : general;
 }
 
+// Whether the 16-byte-per-lane variant applies (see Vec16): Zernike-style
+// column grids, tiles of 8 or more columns, rows a whole number of vectors,
+// aligned instance bases, and an even FP32 spectrum stride.
+template <int T, class R, class G>
+bool use_vec(const FftLaunch& L, const PairMap& pm, const void* spec, long ss) {
+    if constexpr (T < 8) {
+        return false;
+    } else {
+        if (L.layout_kind != GRID_COLS || !L.grid_aligned16 || std::getenv("ZGL_FFT_SCALAR")) return false;
+        if (pm.ncol % grid_vec<G, true>() != 0) return false;
+        if (spec_vec<R, true>() == 2 && (reinterpret_cast<uintptr_t>(spec) % 16 != 0 || ss % 2 != 0)) return false;
+        return true;
+    }
+}
+
 template <int T, class R, class G>
 void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss, int nb,
                     hipStream_t s) {
@@ -791,14 +996,21 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(grid_tiles(pm), nb);
-    auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_forward_kernel<T, GRID_COLS, R, G, 0>,
-                                                      fft_forward_kernel<T, GRID_COLS, R, G, 1>,
-                                                      fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>,
-                                                      fft_forward_kernel<T, GRID_COLS, R, G, wide_mf<T>()>)
-                                        : pick_variant(f, fft_forward_kernel<T, GRID_ROWS, R, G, 0>,
-                                                      fft_forward_kernel<T, GRID_ROWS, R, G, 1>,
-                                                      fft_forward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>,
-                                                      fft_forward_kernel<T, GRID_ROWS, R, G, wide_mf<T>()>);
+    auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_forward_kernel<T, GRID_COLS, R, G, 0, false>,
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, 1, false>,
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>(), false>,
+                                                      fft_forward_kernel<T, GRID_COLS, R, G, wide_mf<T>(), false>)
+                                        : pick_variant(f, fft_forward_kernel<T, GRID_ROWS, R, G, 0, false>,
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, 1, false>,
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, lean_mf<T>(), false>,
+                                                      fft_forward_kernel<T, GRID_ROWS, R, G, wide_mf<T>(), false>);
+    if constexpr (T >= 8) {
+        if (use_vec<T, R, G>(L, pm, spec, ss))
+            k = pick_variant(f, fft_forward_kernel<T, GRID_COLS, R, G, 0, true>,
+                             fft_forward_kernel<T, GRID_COLS, R, G, 1, true>,
+                             fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>(), true>,
+                             fft_forward_kernel<T, GRID_COLS, R, G, wide_mf<T>(), true>);
+    }
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");
@@ -819,14 +1031,21 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     const PairMap pm = pair_map(*L.g, T);
     const size_t lds = lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C));
     dim3 grd(grid_tiles(pm), nb);
-    auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_backward_kernel<T, GRID_COLS, R, G, 0>,
-                                                      fft_backward_kernel<T, GRID_COLS, R, G, 1>,
-                                                      fft_backward_kernel<T, GRID_COLS, R, G, lean_mf<T>()>,
-                                                      fft_backward_kernel<T, GRID_COLS, R, G, wide_mf<T>()>)
-                                        : pick_variant(f, fft_backward_kernel<T, GRID_ROWS, R, G, 0>,
-                                                      fft_backward_kernel<T, GRID_ROWS, R, G, 1>,
-                                                      fft_backward_kernel<T, GRID_ROWS, R, G, lean_mf<T>()>,
-                                                      fft_backward_kernel<T, GRID_ROWS, R, G, wide_mf<T>()>);
+    auto k = L.layout_kind == GRID_COLS ? pick_variant(f, fft_backward_kernel<T, GRID_COLS, R, G, 0, false>,
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, 1, false>,
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, lean_mf<T>(), false>,
+                                                      fft_backward_kernel<T, GRID_COLS, R, G, wide_mf<T>(), false>)
+                                        : pick_variant(f, fft_backward_kernel<T, GRID_ROWS, R, G, 0, false>,
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, 1, false>,
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, lean_mf<T>(), false>,
+                                                      fft_backward_kernel<T, GRID_ROWS, R, G, wide_mf<T>(), false>);
+    if constexpr (T >= 8) {
+        if (use_vec<T, R, G>(L, pm, spec, ss))
+            k = pick_variant(f, fft_backward_kernel<T, GRID_COLS, R, G, 0, true>,
+                             fft_backward_kernel<T, GRID_COLS, R, G, 1, true>,
+                             fft_backward_kernel<T, GRID_COLS, R, G, lean_mf<T>(), true>,
+                             fft_backward_kernel<T, GRID_COLS, R, G, wide_mf<T>(), true>);
+    }
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                   "lds attribute");

@@ -186,6 +186,7 @@ struct FftLaunch {
     double scale;           // prefactor applied with the weights
     double center_factor;   // weight factor of an equator node (zest's SH quirk: 2)
     int tile = 0;           // column pairs per workgroup (0: the plan's choice)
+    bool grid_aligned16 = false;  // every instance's grid base is 16-byte aligned
 };
 // Tile widths the FFT kernels are built for that fit the LDS at this length
 // and arithmetic (widest first).
