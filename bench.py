@@ -429,10 +429,13 @@ def _timed_steps(step, steps, warmup):
 def run_ragged(args):
     """10,000 Zernike instances of orders N ~ U{8..96} (seed 2), coefficients
     N(0,1)/(1+n): one ragged backward + forward per step through
-    zgl_backward_ragged / zgl_forward_ragged (order-sorted queue over four
-    internal streams, HIP graph captured on the first call, replayed after),
-    against the same work as per-order batched zgl_backward / zgl_forward calls
-    on one stream."""
+    zgl_backward_ragged / zgl_forward_ragged (the default lanes pipeline: an
+    order-sorted queue of per-order pipelines over four internal streams; HIP
+    graph captured on the first call, replayed after), against the same work
+    as per-order batched zgl_backward / zgl_forward calls on one stream and as
+    the persistent pipeline (ZGL_RAGGED=persistent: the queue cut into
+    scratch-sized waves, one persistent launch per stage and wave over all
+    orders, items dealt to the resident workgroups longest first)."""
     import numpy as np
     import torch
     import zest_amd
@@ -483,12 +486,31 @@ def run_ragged(args):
         t_rag = _timed_steps(ragged_step, args.steps, args.warmup)
         costs = plan.ragged_lane_costs()
         t_plain = _timed_steps(plain_step, args.steps, args.warmup)
+        # the persistent pipeline (one persistent launch per stage and wave over
+        # all orders) on the same jobs
+        os.environ["ZGL_RAGGED"] = "persistent"
+        pers_plan = zest_amd.Plan(zest_amd.ZERNIKE, 8, precision=prec)
+
+        def pers_step():
+            pers_plan.backward_ragged(jb, stream=stream)
+            pers_plan.forward_ragged(jf, stream=stream)
+
+        t_pers = _timed_steps(pers_step, args.steps, args.warmup)
+        pcosts = pers_plan.ragged_lane_costs()
+        del os.environ["ZGL_RAGGED"]
+        del pers_plan
         out[name] = {"value": round(ninst * args.steps / t_rag, 2),
                      "ms_per_step": round(t_rag / args.steps * 1e3, 3),
                      "per_order_value": round(ninst * args.steps / t_plain, 2),
                      "per_order_ms_per_step": round(t_plain / args.steps * 1e3, 3),
                      "speedup_vs_per_order": round(t_plain / t_rag, 3),
-                     "lane_cost_max_over_mean": round(max(costs) / (sum(costs) / len(costs)), 4)}
+                     "lane_cost_max_over_mean": round(max(costs) / (sum(costs) / len(costs)), 4),
+                     "persistent_value": round(ninst * args.steps / t_pers, 2),
+                     "persistent_ms_per_step": round(t_pers / args.steps * 1e3, 3),
+                     "persistent_speedup_vs_per_order": round(t_plain / t_pers, 3),
+                     "persistent_workgroups": len(pcosts),
+                     "persistent_workgroup_cost_max_over_mean":
+                         round(max(pcosts) / (sum(pcosts) / len(pcosts)), 4)}
         del plan
         torch.cuda.synchronize()
     # accuracy of the mixed form on a sample: the FP64 grids forwarded in FP32

@@ -179,8 +179,10 @@ typedef struct zgl_job {
 
 zgl_status zgl_forward_ragged(zgl_plan *plan, const zgl_job *jobs, size_t njobs, void *stream);
 zgl_status zgl_backward_ragged(zgl_plan *plan, const zgl_job *jobs, size_t njobs, void *stream);
-/* Modelled work per internal stream of the last ragged call (its load
- * balance); no counterpart in zest. */
+/* The load balance of the last ragged call, as modelled work per unit of
+ * execution; no counterpart in zest. Persistent pipeline: per resident
+ * workgroup of its costliest launch; lanes pipeline: per internal stream.
+ * num_lanes receives the count (costs may hold fewer, max_lanes). */
 zgl_status zgl_ragged_lane_costs(const zgl_plan *plan, double *costs, int max_lanes,
                                  int *num_lanes);
 

@@ -455,11 +455,14 @@ class Plan:
         _check(LIB.zgl_backward_ragged(self._h, arr, len(jobs), _stream_ptr(stream, MEM_DEVICE)))
 
     def ragged_lane_costs(self) -> list:
-        """Modelled work per internal stream of the last ragged call."""
-        arr = (ctypes.c_double * 16)()
+        """Modelled work per unit of execution of the last ragged call: per
+        resident workgroup of the costliest persistent launch, or per
+        internal stream of the lanes pipeline."""
         n = ctypes.c_int()
-        _check(LIB.zgl_ragged_lane_costs(self._h, arr, 16, ctypes.byref(n)))
-        return [arr[i] for i in range(min(n.value, 16))]
+        _check(LIB.zgl_ragged_lane_costs(self._h, None, 0, ctypes.byref(n)))
+        arr = (ctypes.c_double * max(n.value, 1))()
+        _check(LIB.zgl_ragged_lane_costs(self._h, arr, n.value, ctypes.byref(n)))
+        return [arr[i] for i in range(n.value)]
 
     def backward_adjoint(self, grid, coeffs, grid_order: int, exp_order: int | None = None,
                          batch: int = 1, grid_stride: int = 0, coeff_stride: int = 0, stream=None):

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <map>
 #include <mutex>
@@ -77,6 +78,83 @@ void* dmalloc(size_t bytes, const char* what) {
         throw;
     }
     return p;
+}
+
+void release(PersistentLaunch& pl) {
+    for (void* b : pl.buffers) dfree(b);
+    pl = PersistentLaunch{};
+}
+
+void release(std::vector<PersistentLaunch>& v) {
+    for (auto& pl : v) release(pl);
+    v.clear();
+}
+
+void lpt_deal(const std::vector<int4>& items, const std::vector<double>& cost, int nbins,
+              std::vector<int4>& dealt, std::vector<int>& bounds, PersistentLaunch& pl) {
+    const size_t n = items.size();
+    nbins = std::max(1, std::min<int>(nbins, (int)std::max<size_t>(n, 1)));
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    // min-heap of (load, bin)
+    std::vector<std::pair<double, int>> heap;
+    heap.reserve(nbins);
+    for (int b = 0; b < nbins; ++b) heap.push_back({0.0, b});
+    auto cmp = [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+        return a.first > b.first || (a.first == b.first && a.second > b.second);
+    };
+    std::vector<int> bin_of(n);
+    std::vector<double> load(nbins, 0.0);
+    for (size_t k : order) {
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        auto& top = heap.back();
+        bin_of[k] = top.second;
+        top.first += cost[k];
+        load[top.second] = top.first;
+        std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+    // bin by bin; inside a bin by (group, x, y), so that consecutive items of
+    // one group share what the kernel keeps in LDS
+    std::vector<size_t> idx(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        if (bin_of[a] != bin_of[b]) return bin_of[a] < bin_of[b];
+        const int4 &p = items[a], &q = items[b];
+        if (p.x != q.x) return p.x < q.x;
+        if (p.y != q.y) return p.y < q.y;
+        return p.z < q.z;
+    });
+    dealt.resize(n);
+    bounds.assign(nbins + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        dealt[i] = items[idx[i]];
+        ++bounds[bin_of[idx[i]] + 1];
+    }
+    for (int b = 0; b < nbins; ++b) bounds[b + 1] += bounds[b];
+    pl.cost_max = 0;
+    double sum = 0;
+    for (double l : load) {
+        pl.cost_max = std::max(pl.cost_max, l);
+        sum += l;
+    }
+    pl.cost_mean = sum / nbins;
+    pl.bin_cost = load;
+}
+
+void* upload_owned(PersistentLaunch& pl, const void* data, size_t bytes) {
+    void* d = dmalloc(std::max<size_t>(bytes, 16), "persistent work list");
+    pl.buffers.push_back(d);
+    if (bytes) check_hip(hipMemcpy(d, data, bytes, hipMemcpyHostToDevice), "persistent work list upload");
+    return d;
+}
+
+int resident_blocks(const void* kernel, int threads, size_t lds) {
+    int dev = 0, cus = 0, per = 0;
+    check_hip(hipGetDevice(&dev), "device");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds), "occupancy");
+    return std::max(1, cus * std::max(1, per));
 }
 
 void dfree(void* p) noexcept {
@@ -191,7 +269,12 @@ struct zgl_plan {
             // `offs` or runs `exec`: both are rewritten or destroyed only once
             // that work has completed
             hipEvent_t done = nullptr;
+            // the persistent pipeline of the job list (valid && persistent):
+            // its launches in order, wave by wave, on `pscratch`
+            bool persistent = false;
+            std::vector<zgl::PersistentLaunch> plist;
         } cache[2];
+        std::vector<zgl::Scratch> pscratch;  // per lane: spectra and flm of one wave of the persistent pipeline
         std::vector<double> lane_cost;  // modelled work per lane of the last build
         bool grid16 = false;  // every grid of the job list being built is 16-byte aligned
     } rg;
@@ -280,6 +363,8 @@ void ragged_release(zgl_plan::Ragged& r) {
     for (auto e : r.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& sc : r.scratch) sc.release();
+    for (auto& c : r.cache) release(c.plist);
+    for (auto& sc : r.pscratch) sc.release();
     r = zgl_plan::Ragged{};
 }
 
@@ -874,13 +959,209 @@ zgl_status backward_entry(zgl_plan* plan, const double* coeffs, size_t exp_order
 }
 
 // ------------------------------------------------------------ ragged batches
-// Jobs of uneven orders: an order-sorted queue of per-order groups (largest
-// order first), dealt to kLanes internal streams by longest-processing-time
-// on a modelled cost; each group runs the batched pipeline of its order with
-// per-instance addresses (Inst offsets). On a non-null stream the whole
-// fork/join pipeline is captured once into a hipGraph and replayed while the
-// job list is unchanged (ZGL_RAGGED_GRAPH=0 runs it without a graph).
+// Jobs of uneven orders, two pipelines; both give the same bits as per-order
+// batched calls.
+//
+// Lanes (the default): the jobs form an order-sorted queue of per-order
+// groups (largest order first), dealt to kLanes internal streams by longest
+// processing time on a modelled cost; each group runs the batched pipeline of
+// its order with per-instance addresses (Inst offsets).
+//
+// Persistent (ZGL_RAGGED=persistent; Zernike, FP64 and FP32, where every
+// order has the streaming kernels): the order-sorted queue is cut into waves
+// whose spectra and flm fit the scratch budget. Each stage of a wave is one
+// persistent launch over all the groups of the wave -- the FFT one per kernel
+// variant among them -- whose work items (group, tile, instance group) are
+// dealt to the resident workgroups by longest processing time on a modelled
+// cost.
+//
+// Either way, on a non-null stream the whole pipeline is captured once into a
+// hipGraph and replayed while the job list is unchanged (ZGL_RAGGED_GRAPH=0
+// runs it without a graph); the persistent launch arrays are kept for the job
+// list either way.
 constexpr int kLanes = 4;
+
+// The internal streams and their fork/join events, created on first use.
+void ensure_lanes(zgl_plan::Ragged& R) {
+    if (!R.lanes.empty()) return;
+    R.lanes.resize(kLanes, nullptr);
+    R.ev.resize(kLanes + 1, nullptr);
+    R.scratch.resize(kLanes);
+    for (auto& l : R.lanes) check_hip(hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "lane stream");
+    for (auto& e : R.ev) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
+}
+
+// Streams of the persistent pipeline: its waves alternate between them
+// (ZGL_RAGGED_LANES, 1 to kLanes; default 1). A persistent launch fills every
+// resident workgroup slot, so a second stream gains nothing unless launches
+// are cut short (measured: 1 lane 102.9 ms, 2 lanes 105.0 ms per FP64 step of
+// 10,000 instances of orders 8..96).
+int persistent_lanes() {
+    const char* e = std::getenv("ZGL_RAGGED_LANES");
+    const int v = e && *e ? std::atoi(e) : 1;
+    return std::max(1, std::min(kLanes, v));
+}
+
+// The pipeline ragged calls take: ZGL_RAGGED=persistent selects the
+// persistent one, anything else the lanes one (measured faster: its per-order
+// launches of different stages overlap on the four streams, which a launch
+// filling the whole device cannot).
+bool ragged_persistent_selected() {
+    const char* e = std::getenv("ZGL_RAGGED");
+    return e && std::strcmp(e, "persistent") == 0;
+}
+
+struct RaggedGroup {
+    size_t order, first, count;
+    int lane;
+};
+
+// Builds the persistent pipeline of one direction into C.plist: false (and
+// nothing kept) where a group has no persistent form. `moved` reports a
+// reallocated wave scratch (the other direction's pipeline points at the old).
+bool build_persistent(zgl_plan* p, zgl_plan::Ragged::Cached& C, const std::vector<RaggedGroup>& groups,
+                      double* gbase, double* cbase, size_t n, bool forward, bool& moved) {
+    moved = false;
+    if (p->kind != ZGL_ZERNIKE || (p->precision != ZGL_FP64 && p->precision != ZGL_FP32) ||
+        !ragged_persistent_selected())
+        return false;
+    const bool fp32 = p->precision == ZGL_FP32;
+    const size_t csize = fp32 ? sizeof(float2) : sizeof(double2);
+    for (const auto& g : groups) {
+        const OrderTables& t = tables_for(p, g.order);
+        if (t.plm_in_kernel || (fp32 ? use_fused<float>(p, t) : use_fused<double>(p, t))) return false;
+    }
+    // waves: pieces of groups whose scratch fits the budget
+    struct Piece {
+        size_t order, first, count;
+        size_t spec_off, flm_off;  // bytes into the wave scratch
+    };
+    std::vector<std::vector<Piece>> waves(1);
+    const int nl = persistent_lanes();
+    const size_t budget = (env_mb("ZGL_SCRATCH_MB", 4096) << 20) / nl;  // per lane
+    size_t spec_use = 0, flm_use = 0, spec_max = 0, flm_max = 0;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    for (const auto& g : groups) {
+        const Chunking ch = chunking(p, tables_for(p, g.order).g, 1, csize);
+        const size_t per_spec = (size_t)ch.spec_stride * csize, per_flm = (size_t)ch.flm_stride * csize;
+        size_t done = 0;
+        while (done < g.count) {
+            const size_t room = budget > spec_use + flm_use ? budget - spec_use - flm_use : 0;
+            size_t take = std::min(g.count - done, room / (per_spec + per_flm + 512));
+            if (take == 0) {
+                if (!waves.back().empty()) {  // start a new wave
+                    waves.emplace_back();
+                    spec_use = flm_use = 0;
+                    continue;
+                }
+                take = 1;  // one instance above the budget: a wave of its own
+            }
+            waves.back().push_back(Piece{g.order, g.first + done, take, spec_use, flm_use});
+            spec_use = align(spec_use + take * per_spec);
+            flm_use = align(flm_use + take * per_flm);
+            spec_max = std::max(spec_max, spec_use);
+            flm_max = std::max(flm_max, flm_use);
+            done += take;
+        }
+    }
+    // wave w runs on lane w % nl, each lane on a scratch of its own
+    auto& R = p->rg;
+    if (nl > 1) ensure_lanes(R);
+    if ((int)R.pscratch.size() < nl) R.pscratch.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        Scratch& sc = R.pscratch[l];
+        const void *spec0 = sc.spec, *flm0 = sc.flm;
+        ensure_bytes(sc.spec, sc.spec_cap, spec_max);
+        ensure_bytes(sc.flm, sc.flm_cap, flm_max);
+        moved = moved || sc.spec != spec0 || sc.flm != flm0;
+    }
+
+    std::vector<PersistentLaunch> out;
+    try {
+        for (size_t w = 0; w < waves.size(); ++w) {
+            const auto& wave = waves[w];
+            const int lane = (int)(w % nl);
+            Scratch& sc = R.pscratch[lane];
+            const size_t first_launch = out.size();
+            std::vector<FftGroup> fg;
+            std::vector<ZtGemmArgs> ga;
+            for (const Piece& pc : wave) {
+                const OrderTables& t = tables_for(p, pc.order);
+                const Geom& g = t.g;
+                const Chunking ch = chunking(p, g, 1, csize);
+                char* spec = static_cast<char*>(sc.spec) + pc.spec_off;
+                char* flm = static_cast<char*>(sc.flm) + pc.flm_off;
+                FftLaunch L;
+                L.g = &g;
+                L.t = &t;
+                L.layout_kind = GRID_COLS;
+                L.grid_aligned16 = p->rg.grid16;
+                L.nm_used = (int)pc.order;
+                L.weighted = forward;
+                L.scale = forward ? forward_prefactor(p, g) : 1.0;
+                L.center_factor = 1.0;
+                fg.push_back(FftGroup{L, Inst<double>{gbase, 0, C.offs + pc.first}, spec, ch.spec_stride,
+                                      (int)pc.count});
+                ZtGemmArgs a{&t, (int)pc.order, forward && p->znorm == ZGL_ZNORM_UNNORMED ? 1 : 0, spec,
+                             ch.spec_stride, flm, ch.flm_stride, Inst<double>{cbase, 0, C.offs + n + pc.first},
+                             (int)pc.count};
+                ga.push_back(a);
+            }
+            bool ok;
+            if (forward) {
+                ok = fp32 ? fft_forward_persistent<float>(out, fg) : fft_forward_persistent<double>(out, fg);
+                ok = ok && (fp32 ? zt_legendre_forward_persistent<float>(out, ga)
+                                 : zt_legendre_forward_persistent<double>(out, ga));
+                ok = ok && (fp32 ? zt_radial_forward_persistent<float>(out, ga)
+                                 : zt_radial_forward_persistent<double>(out, ga));
+            } else {
+                ok = fp32 ? zt_radial_backward_persistent<float>(out, ga)
+                          : zt_radial_backward_persistent<double>(out, ga);
+                ok = ok && (fp32 ? zt_legendre_backward_persistent<float>(out, ga)
+                                 : zt_legendre_backward_persistent<double>(out, ga));
+                ok = ok && (fp32 ? fft_backward_persistent<float>(out, fg) : fft_backward_persistent<double>(out, fg));
+            }
+            if (!ok) {
+                release(out);
+                return false;
+            }
+            for (size_t k = first_launch; k < out.size(); ++k) out[k].lane = nl > 1 ? lane : -1;
+        }
+    } catch (...) {
+        release(out);
+        throw;
+    }
+    C.plist = std::move(out);
+    return true;
+}
+
+Stage stage_of(const char* name) {
+    for (int k = 0; k < ST_COUNT; ++k)
+        if (std::strcmp(kStageNames[k], name) == 0) return (Stage)k;
+    return ST_FFT_FWD;
+}
+
+// Launches on the caller's stream (lane -1) or forked over the lanes and
+// joined back.
+void run_persistent(zgl_plan* p, const zgl_plan::Ragged::Cached& C, hipStream_t s) {
+    auto& R = p->rg;
+    bool forked = false;
+    for (const PersistentLaunch& pl : C.plist) forked = forked || pl.lane >= 0;
+    if (forked) {
+        check_hip(hipEventRecord(R.ev[0], s), "fork");
+        for (auto l : R.lanes) check_hip(hipStreamWaitEvent(l, R.ev[0], 0), "fork");
+    }
+    for (const PersistentLaunch& pl : C.plist) {
+        hipStream_t ls = pl.lane >= 0 ? R.lanes[pl.lane] : s;
+        if (p->profiling) timed(p, stage_of(pl.stage), (int)pl.instances, ls, [&] { pl.run(ls); });
+        else pl.run(ls);
+    }
+    if (forked)
+        for (int l = 0; l < kLanes; ++l) {
+            check_hip(hipEventRecord(R.ev[1 + l], R.lanes[l]), "join");
+            check_hip(hipStreamWaitEvent(s, R.ev[1 + l], 0), "join");
+        }
+}
 
 bool ragged_graphs() {
     const char* e = std::getenv("ZGL_RAGGED_GRAPH");
@@ -912,6 +1193,8 @@ void drop_graph(zgl_plan::Ragged::Cached& c) {
     c.graph = nullptr;
     c.jobs.clear();
     c.valid = false;
+    release(c.plist);
+    c.persistent = false;
 }
 
 bool same_jobs(const std::vector<zgl_job>& a, const zgl_job* b, size_t n) {
@@ -958,8 +1241,14 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         auto& R = plan->rg;
         auto& C = R.cache[forward ? 1 : 0];
         const bool use_graph = s != nullptr && !plan->profiling && ragged_graphs();
-        if (use_graph && C.valid && same_jobs(C.jobs, jobs, njobs)) {
+        const bool same = C.valid && same_jobs(C.jobs, jobs, njobs);
+        if (same && use_graph && C.exec) {
             check_hip(hipGraphLaunch(C.exec, s), "ragged graph launch");
+            mark_cache(C, s);
+            return ZGL_SUCCESS;
+        }
+        if (same && !use_graph && C.persistent) {
+            run_persistent(plan, C, s);
             mark_cache(C, s);
             return ZGL_SUCCESS;
         }
@@ -974,11 +1263,10 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         if (idx.empty()) return ZGL_SUCCESS;
         std::stable_sort(idx.begin(), idx.end(),
                          [&](size_t a, size_t b) { return jobs[a].order > jobs[b].order; });
-        struct Group { size_t order, first, count; int lane; };
-        std::vector<Group> groups;
+        std::vector<RaggedGroup> groups;
         for (size_t k = 0; k < idx.size(); ++k) {
             if (groups.empty() || groups.back().order != jobs[idx[k]].order)
-                groups.push_back(Group{jobs[idx[k]].order, k, 0, 0});
+                groups.push_back(RaggedGroup{jobs[idx[k]].order, k, 0, 0});
             ++groups.back().count;
         }
 
@@ -1001,16 +1289,47 @@ zgl_status ragged_entry(zgl_plan* plan, const zgl_job* jobs, size_t njobs, void*
         check_hip(hipMemcpy(C.offs, offs.data(), 2 * n * sizeof(long), hipMemcpyHostToDevice),
                   "ragged offsets upload");
 
-        // lanes: streams, events, scratch; groups dealt by longest processing time
-        if (R.lanes.empty()) {
-            R.lanes.resize(kLanes, nullptr);
-            R.ev.resize(kLanes + 1, nullptr);
-            R.scratch.resize(kLanes);
-            for (auto& l : R.lanes) check_hip(hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "lane stream");
-            for (auto& e : R.ev) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "lane event");
-        }
-        R.lane_cost.assign(kLanes, 0.0);
         bool moved = false;
+        if (build_persistent(plan, C, groups, gbase, cbase, n, forward, moved)) {
+            // the other direction's pipeline points at the old wave scratch
+            if (moved) drop_graph(R.cache[forward ? 0 : 1]);
+            C.persistent = true;
+            C.jobs.assign(jobs, jobs + njobs);
+            C.valid = true;
+            R.lane_cost.clear();
+            double best = -1;
+            for (const auto& pl : C.plist) {  // the launch of the most modelled work
+                const double tot = pl.cost_mean * pl.bin_cost.size();
+                if (tot > best) {
+                    best = tot;
+                    R.lane_cost = pl.bin_cost;
+                }
+            }
+            if (!use_graph) {
+                run_persistent(plan, C, s);
+                mark_cache(C, s);
+                return ZGL_SUCCESS;
+            }
+            check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
+            try {
+                run_persistent(plan, C, s);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            check_hip(hipStreamEndCapture(s, &C.graph), "end capture");
+            check_hip(hipGraphInstantiate(&C.exec, C.graph, nullptr, nullptr, 0), "graph instantiate");
+            check_hip(hipGraphLaunch(C.exec, s), "ragged graph launch");
+            mark_cache(C, s);
+            return ZGL_SUCCESS;
+        }
+
+        // lanes: streams, events, scratch; groups dealt by longest processing time
+        ensure_lanes(R);
+        R.lane_cost.assign(kLanes, 0.0);
+        moved = false;
         for (auto& g : groups) {
             tables_for(plan, g.order);
             const int l = (int)(std::min_element(R.lane_cost.begin(), R.lane_cost.end()) - R.lane_cost.begin());

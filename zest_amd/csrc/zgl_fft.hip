@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "zgl_internal.hpp"
@@ -517,10 +518,12 @@ __host__ __device__ constexpr bool twiddles_in_lds(int T, size_t csize) {
     return T > (csize == 16 ? 8 : 4);
 }
 
+// copy = false: the LDS already holds these tables (a persistent workgroup's
+// previous item was of the same group); only the pointers are set.
 template <int T, int NT, class C>
 __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsigned short*& perm,
                                     const FftDesc& d, const C* twg, const C* twpg,
-                                    const unsigned short* permg) {
+                                    const unsigned short* permg, bool copy = true) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n = d.n;
     data = reinterpret_cast<C*>(smem_raw);
@@ -528,8 +531,10 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
         C* tl = data + n * T;
         C* tp = tl + n;
         perm = reinterpret_cast<unsigned short*>(tp + (d.npass > 0 ? d.npass - 1 : 0) * n);
-        for (int i = threadIdx.x; i < n; i += NT) tl[i] = twg[i];
-        for (int i = threadIdx.x; i < (d.npass - 1) * n; i += NT) tp[i] = twpg[i];
+        if (copy) {
+            for (int i = threadIdx.x; i < n; i += NT) tl[i] = twg[i];
+            for (int i = threadIdx.x; i < (d.npass - 1) * n; i += NT) tp[i] = twpg[i];
+        }
         twl = tl;
         twp = tp;
     } else {
@@ -537,7 +542,8 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
         twl = twg;
         twp = twpg;
     }
-    for (int i = threadIdx.x; i < n; i += NT) perm[i] = permg[i];
+    if (copy)
+        for (int i = threadIdx.x; i < n; i += NT) perm[i] = permg[i];
     __syncthreads();
 }
 
@@ -548,28 +554,65 @@ constexpr int lean_waves() {
     return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : (fft_threads<T, R, MF>() > kThreads ? (sizeof(R) == 8 ? 2 : 4) : 1);
 }
 
+// The arguments of one FFT launch over the instances of one order (also one
+// group of a persistent launch).
+template <class R, class G>
+struct FftFwdArgs {
+    FftDesc d;
+    const typename Cplx<R>::type* twg;
+    const typename Cplx<R>::type* twpg;
+    const unsigned short* permg;
+    Inst<const G> grid;
+    typename Cplx<R>::type* spec;
+    long spec_stride;
+    PairMap pm;
+    int nm;
+    double scale;
+    const double* wlat;
+    const double* wrad;
+    double center_factor;
+};
+
+template <class R, class G>
+struct FftBwdArgs {
+    FftDesc d;
+    const typename Cplx<R>::type* twg;
+    const typename Cplx<R>::type* twpg;
+    const unsigned short* permg;
+    const typename Cplx<R>::type* spec;
+    long spec_stride;
+    Inst<G> grid;
+    PairMap pm;
+    int nm;
+    double scale;
+    const double* wlat;
+    const double* wrad;
+    double center_factor;
+};
+
+// One tile of one instance; copy_tables = false when the LDS already holds
+// this order's tables.
 template <int T, int LK, class R, class G, int MF, bool VEC>
-__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
-void fft_forward_kernel(
-    FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
-    const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
-    Inst<const G> grid, typename Cplx<R>::type* __restrict__ spec,
-    long spec_stride, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
-    const double* __restrict__ wrad, double center_factor) {
+__device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, const int tile, const int inst,
+                                                 const bool copy_tables) {
     using C = typename Cplx<R>::type;
     constexpr int NT = fft_threads<T, R, MF>();
-    const int tile = xcd_tile(pm);
-    if (tile >= pm.ntiles) return;
+    const FftDesc& d = a.d;
+    const PairMap& pm = a.pm;
+    const int nm = a.nm;
+    const double scale = a.scale, center_factor = a.center_factor;
+    const double* __restrict__ wlat = a.wlat;
+    const double* __restrict__ wrad = a.wrad;
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
-    stage_tables<T, NT>(data, twl, twp, perm, d, twg, twpg, permg);
+    stage_tables<T, NT>(data, twl, twp, perm, d, a.twg, a.twpg, a.permg, copy_tables);
     const int n = d.n;
     int th, tc;
     thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
-    const G* g = grid.at(blockIdx.y);
-    C* sp = spec + (long)blockIdx.y * spec_stride;
+    const G* g = a.grid.at(inst);
+    C* sp = a.spec + (long)inst * a.spec_stride;
 
     // Loads are issued in batches of kBatch per thread before any LDS store,
     // so each thread keeps that many independent global reads in flight.
@@ -726,26 +769,26 @@ void fft_forward_kernel(
 }
 
 template <int T, int LK, class R, class G, int MF, bool VEC>
-__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
-void fft_backward_kernel(
-    FftDesc d, const typename Cplx<R>::type* __restrict__ twg,
-    const typename Cplx<R>::type* __restrict__ twpg, const unsigned short* __restrict__ permg,
-    const typename Cplx<R>::type* __restrict__ spec, long spec_stride, Inst<G> grid, PairMap pm, int nm, double scale, const double* __restrict__ wlat,
-    const double* __restrict__ wrad, double center_factor) {
+__device__ __forceinline__ void fft_backward_body(const FftBwdArgs<R, G>& a, const int tile, const int inst,
+                                                  const bool copy_tables) {
     using C = typename Cplx<R>::type;
     constexpr int NT = fft_threads<T, R, MF>();
-    const int tile = xcd_tile(pm);
-    if (tile >= pm.ntiles) return;
+    const FftDesc& d = a.d;
+    const PairMap& pm = a.pm;
+    const int nm = a.nm;
+    const double scale = a.scale, center_factor = a.center_factor;
+    const double* __restrict__ wlat = a.wlat;
+    const double* __restrict__ wrad = a.wrad;
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
-    stage_tables<T, NT>(data, twl, twp, perm, d, twg, twpg, permg);
+    stage_tables<T, NT>(data, twl, twp, perm, d, a.twg, a.twpg, a.permg, copy_tables);
     const int n = d.n;
     int th, tc;
     thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
-    G* g = grid.at(blockIdx.y);
-    const C* sp = spec + (long)blockIdx.y * spec_stride;
+    G* g = a.grid.at(inst);
+    const C* sp = a.spec + (long)inst * a.spec_stride;
     const int F = n / 2 + 1;
     if constexpr (spec_vec<R, VEC>() == 2 && kBwdSpecPairs) {
         // two consecutive columns per lane: 16-byte spectrum loads
@@ -915,6 +958,54 @@ void fft_backward_kernel(
     }
 }
 
+template <int T, int LK, class R, class G, int MF, bool VEC>
+__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
+void fft_forward_kernel(FftFwdArgs<R, G> a) {
+    const int tile = xcd_tile(a.pm);
+    if (tile >= a.pm.ntiles) return;
+    fft_forward_body<T, LK, R, G, MF, VEC>(a, tile, blockIdx.y, true);
+}
+
+template <int T, int LK, class R, class G, int MF, bool VEC>
+__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
+void fft_backward_kernel(FftBwdArgs<R, G> a) {
+    const int tile = xcd_tile(a.pm);
+    if (tile >= a.pm.ntiles) return;
+    fft_backward_body<T, LK, R, G, MF, VEC>(a, tile, blockIdx.y, true);
+}
+
+// Persistent forms (ragged batches of many orders): workgroup b walks the
+// items [bounds[b], bounds[b + 1]), item = (group, tile, instance), with the
+// arguments of each group in groups[]. Consecutive items of one group reuse
+// the tables already in LDS.
+template <int T, int LK, class R, class G, int MF, bool VEC>
+__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
+void fft_forward_persistent(const FftFwdArgs<R, G>* __restrict__ groups, const int4* __restrict__ items,
+                            const int* __restrict__ bounds) {
+    const int w1 = bounds[blockIdx.x + 1];
+    int prev = -1;
+    for (int w = bounds[blockIdx.x]; w < w1; ++w) {
+        const int4 it = items[w];
+        __syncthreads();  // the previous item is done with the LDS
+        fft_forward_body<T, LK, R, G, MF, VEC>(groups[it.x], it.y, it.z, it.x != prev);
+        prev = it.x;
+    }
+}
+
+template <int T, int LK, class R, class G, int MF, bool VEC>
+__global__ __launch_bounds__((fft_threads<T, R, MF>())) __attribute__((amdgpu_waves_per_eu(lean_waves<T, R, MF>())))
+void fft_backward_persistent(const FftBwdArgs<R, G>* __restrict__ groups, const int4* __restrict__ items,
+                             const int* __restrict__ bounds) {
+    const int w1 = bounds[blockIdx.x + 1];
+    int prev = -1;
+    for (int w = bounds[blockIdx.x]; w < w1; ++w) {
+        const int4 it = items[w];
+        __syncthreads();
+        fft_backward_body<T, LK, R, G, MF, VEC>(groups[it.x], it.y, it.z, it.x != prev);
+        prev = it.x;
+    }
+}
+
 int lds_limit() {
     static int limit = 0;
     if (!limit) {
@@ -1017,9 +1108,10 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
     const int nt = !f.mfma_pass ? fft_threads<T, R, 0>()
                    : f.wide   ? fft_threads<T, R, wide_mf<T>()>()
                    : f.lean   ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
-    k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
-                                 L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
-                                 L.center_factor);
+    const FftFwdArgs<R, G> a{f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, grid, (C*)spec, ss, pm,
+                             L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
+                             L.center_factor};
+    k<<<grd, nt, lds, s>>>(a);
     check_hip(hipGetLastError(), "fft_forward launch");
 }
 
@@ -1052,9 +1144,10 @@ void launch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> grid
     const int nt = !f.mfma_pass ? fft_threads<T, R, 0>()
                    : f.wide   ? fft_threads<T, R, wide_mf<T>()>()
                    : f.lean   ? fft_threads<T, R, lean_mf<T>()>() : fft_threads<T, R, 1>();
-    k<<<grd, nt, lds, s>>>(f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
-                                 pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
-                                 L.center_factor);
+    const FftBwdArgs<R, G> a{f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)spec, ss, grid,
+                             pm, L.nm_used, L.scale, L.weighted ? L.t->wlat : nullptr, L.t->wrad,
+                             L.center_factor};
+    k<<<grd, nt, lds, s>>>(a);
     check_hip(hipGetLastError(), "fft_backward launch");
 }
 
@@ -1082,6 +1175,151 @@ void dispatch_backward(const FftLaunch& L, const void* spec, long ss, Inst<G> gr
         case 2: launch_backward<2, R>(L, spec, ss, grid, nb, s); break;
         default: launch_backward<1, R>(L, spec, ss, grid, nb, s); break;
     }
+}
+
+// ---- persistent launches over many orders (ragged batches) ----
+// The kernel of one (width T, variant, vector form): variant 0 plain, 1 MFMA
+// pass, 2 lean, 3 wide, as pick_variant chooses them.
+template <int T, class R, bool VEC>
+struct FftPersistentKernels {
+    using FA = FftFwdArgs<R, double>;
+    using BA = FftBwdArgs<R, double>;
+    using FK = void (*)(const FA*, const int4*, const int*);
+    using BK = void (*)(const BA*, const int4*, const int*);
+    static FK fwd(int var) {
+        switch (var) {
+            case 0: return fft_forward_persistent<T, GRID_COLS, R, double, 0, VEC>;
+            case 1: return fft_forward_persistent<T, GRID_COLS, R, double, 1, VEC>;
+            case 2: return fft_forward_persistent<T, GRID_COLS, R, double, lean_mf<T>(), VEC>;
+            default: return fft_forward_persistent<T, GRID_COLS, R, double, wide_mf<T>(), VEC>;
+        }
+    }
+    static BK bwd(int var) {
+        switch (var) {
+            case 0: return fft_backward_persistent<T, GRID_COLS, R, double, 0, VEC>;
+            case 1: return fft_backward_persistent<T, GRID_COLS, R, double, 1, VEC>;
+            case 2: return fft_backward_persistent<T, GRID_COLS, R, double, lean_mf<T>(), VEC>;
+            default: return fft_backward_persistent<T, GRID_COLS, R, double, wide_mf<T>(), VEC>;
+        }
+    }
+    static int threads(int var) {
+        switch (var) {
+            case 0: return fft_threads<T, R, 0>();
+            case 1: return fft_threads<T, R, 1>();
+            case 2: return fft_threads<T, R, lean_mf<T>()>();
+            default: return fft_threads<T, R, wide_mf<T>()>();
+        }
+    }
+};
+
+inline int fft_variant(const FftPlan& f) { return !f.mfma_pass ? 0 : f.wide ? 3 : f.lean ? 2 : 1; }
+
+// One persistent launch over the groups `ids` (all of width T, variant var,
+// vector form VEC): items (group, tile, instance), modelled cost n * T each.
+template <int T, class R, bool VEC>
+void fft_persistent_bucket(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& groups,
+                           const std::vector<int>& ids, int var, bool forward) {
+    using C = typename Cplx<R>::type;
+    using K = FftPersistentKernels<T, R, VEC>;
+    PersistentLaunch pl;
+    pl.stage = forward ? "fft_forward" : "fft_backward";
+    std::vector<typename K::FA> fa;
+    std::vector<typename K::BA> ba;
+    std::vector<int4> items;
+    std::vector<double> cost;
+    size_t lds = 0;
+    for (int j = 0; j < (int)ids.size(); ++j) {
+        const FftGroup& gr = groups[ids[j]];
+        const FftLaunch& L = gr.L;
+        const FftPlan& f = L.t->fft;
+        const PairMap pm = pair_map(*L.g, T);
+        lds = std::max(lds, lds_bytes(f.desc.n, f.desc.npass, T, sizeof(C)));
+        if (forward)
+            fa.push_back(typename K::FA{f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, Inst<const double>(gr.grid),
+                                        (C*)gr.spec, gr.spec_stride, pm, L.nm_used, L.scale,
+                                        L.weighted ? L.t->wlat : nullptr, L.t->wrad, L.center_factor});
+        else
+            ba.push_back(typename K::BA{f.desc, tw_of<R>(f), twp_of<R>(f), f.perm, (const C*)gr.spec,
+                                        gr.spec_stride, gr.grid, pm, L.nm_used, L.scale,
+                                        L.weighted ? L.t->wlat : nullptr, L.t->wrad, L.center_factor});
+        pl.instances += gr.nb;
+        for (int b = 0; b < gr.nb; ++b)
+            for (int t = 0; t < pm.ntiles; ++t) {
+                items.push_back(make_int4(j, t, b, 0));
+                cost.push_back((double)f.desc.n * T);
+            }
+    }
+    if (items.empty()) return;
+    const int nt = K::threads(var);
+    const void* kern = forward ? reinterpret_cast<const void*>(K::fwd(var)) : reinterpret_cast<const void*>(K::bwd(var));
+    if (lds > 65536)
+        check_hip(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "lds attribute");
+    std::vector<int4> dealt;
+    std::vector<int> bounds;
+    lpt_deal(items, cost, resident_blocks(kern, nt, lds), dealt, bounds, pl);
+    const int blocks = (int)bounds.size() - 1;
+    const int4* di = upload_owned(pl, dealt);
+    const int* db = upload_owned(pl, bounds);
+    pl.items = (long)items.size();
+    pl.blocks = blocks;
+    if (forward) {
+        const typename K::FA* dg = upload_owned(pl, fa);
+        const typename K::FK k = K::fwd(var);
+        pl.run = [=](hipStream_t s) {
+            k<<<blocks, nt, lds, s>>>(dg, di, db);
+            check_hip(hipGetLastError(), "persistent fft_forward launch");
+        };
+    } else {
+        const typename K::BA* dg = upload_owned(pl, ba);
+        const typename K::BK k = K::bwd(var);
+        pl.run = [=](hipStream_t s) {
+            k<<<blocks, nt, lds, s>>>(dg, di, db);
+            check_hip(hipGetLastError(), "persistent fft_backward launch");
+        };
+    }
+    out.push_back(std::move(pl));
+}
+
+template <int T, class R>
+void fft_persistent_width(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& groups,
+                          const std::vector<int>& ids, bool forward) {
+    std::map<std::pair<int, bool>, std::vector<int>> buckets;
+    for (int i : ids) {
+        const FftGroup& gr = groups[i];
+        const PairMap pm = pair_map(*gr.L.g, T);
+        bool vec = false;
+        if constexpr (T >= 8) vec = use_vec<T, R, double>(gr.L, pm, gr.spec, gr.spec_stride);
+        buckets[{fft_variant(gr.L.t->fft), vec}].push_back(i);
+    }
+    for (auto& kv : buckets) {
+        if constexpr (T >= 8)
+            if (kv.first.second) {
+                fft_persistent_bucket<T, R, true>(out, groups, kv.second, kv.first.first, forward);
+                continue;
+            }
+        fft_persistent_bucket<T, R, false>(out, groups, kv.second, kv.first.first, forward);
+    }
+}
+
+// Widths below 8 (long sequences, orders above those of ragged batches) have
+// no persistent form: false, nothing appended.
+template <class R>
+bool fft_persistent(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& groups, bool forward) {
+    std::map<int, std::vector<int>> by_width;
+    for (int i = 0; i < (int)groups.size(); ++i) {
+        const FftLaunch& L = groups[i].L;
+        const int T = L.tile ? L.tile : L.t->fft.tile[sizeof(R) == 4];
+        if (L.layout_kind != GRID_COLS || T < 8) return false;
+        by_width[T].push_back(i);
+    }
+    for (auto& kv : by_width) {
+        switch (kv.first) {
+            case 32: fft_persistent_width<32, R>(out, groups, kv.second, forward); break;
+            case 16: fft_persistent_width<16, R>(out, groups, kv.second, forward); break;
+            default: fft_persistent_width<8, R>(out, groups, kv.second, forward); break;
+        }
+    }
+    return true;
 }
 
 // Whether T column pairs per workgroup fit the LDS and the per-thread item
@@ -1258,11 +1496,25 @@ void fft_backward(const FftLaunch& L, const void* spec, long spec_stride, Inst<G
     dispatch_backward<R>(L, spec, spec_stride, grid, nb, s);
 }
 
+template <class R>
+bool fft_forward_persistent(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& g) {
+    return fft_persistent<R>(out, g, true);
+}
+
+template <class R>
+bool fft_backward_persistent(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& g) {
+    return fft_persistent<R>(out, g, false);
+}
+
 // FP64 grids with FP64 or FP32 arithmetic; FP32 grids with FP32 arithmetic.
 #if ZGL_FFT_PART == 0
+template bool fft_forward_persistent<double>(std::vector<PersistentLaunch>&, const std::vector<FftGroup>&);
+template bool fft_backward_persistent<double>(std::vector<PersistentLaunch>&, const std::vector<FftGroup>&);
 template void fft_forward<double, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
 template void fft_backward<double, double>(const FftLaunch&, const void*, long, Inst<double>, int, hipStream_t);
 #elif ZGL_FFT_PART == 1
+template bool fft_forward_persistent<float>(std::vector<PersistentLaunch>&, const std::vector<FftGroup>&);
+template bool fft_backward_persistent<float>(std::vector<PersistentLaunch>&, const std::vector<FftGroup>&);
 template void fft_forward<float, double>(const FftLaunch&, Inst<const double>, void*, long, int, hipStream_t);
 template void fft_backward<float, double>(const FftLaunch&, const void*, long, Inst<double>, int, hipStream_t);
 #else

@@ -352,4 +352,64 @@ void coeff_accumulate(Inst<double> c, const double* d, long ds, long n, int nb, 
 void coeff_kappa_scale(Inst<const double> c, double* out, long os, int nt, int unnormed, int nb,
                        hipStream_t s);
 
+// ---- persistent launches over many orders (ragged batches) ----
+// One launch of a stage kernel over groups of instances of different orders:
+// the arguments of every group in a device array, and the work items
+// (group, x, y) -- x, y being the stage's own workgroup coordinates within the
+// group -- dealt to a fixed number of resident workgroups, longest first, each
+// walking its own contiguous range of items. `run` enqueues the launch; the
+// device arrays stay valid until release().
+struct PersistentLaunch {
+    std::vector<void*> buffers;
+    std::function<void(hipStream_t)> run;
+    const char* stage = "";
+    long items = 0;
+    long instances = 0;  // instances the launch covers (stage statistics)
+    int lane = 0;        // internal stream it runs on (ragged pipelines)
+    int blocks = 0;
+    double cost_max = 0, cost_mean = 0;  // modelled work of the busiest and of the mean workgroup
+    std::vector<double> bin_cost;        // modelled work of each workgroup
+};
+void release(PersistentLaunch& pl);
+void release(std::vector<PersistentLaunch>& v);
+
+// Deals the items, with their modelled costs, to nbins bins by longest
+// processing time first (each to the least loaded bin); returns them bin by
+// bin in `dealt` with the nbins + 1 bin offsets in `bounds`.
+void lpt_deal(const std::vector<int4>& items, const std::vector<double>& cost, int nbins,
+              std::vector<int4>& dealt, std::vector<int>& bounds, PersistentLaunch& pl);
+// Copies v to new device memory owned by pl (synchronous).
+void* upload_owned(PersistentLaunch& pl, const void* data, size_t bytes);
+template <class T>
+T* upload_owned(PersistentLaunch& pl, const std::vector<T>& v) {
+    return static_cast<T*>(upload_owned(pl, v.data(), v.size() * sizeof(T)));
+}
+// Resident workgroups of a kernel on the current device (CUs x per-CU occupancy).
+int resident_blocks(const void* kernel, int threads, size_t lds);
+
+// The FFT of one group: FftLaunch as for a batched call, the group's grids
+// (Inst) and its spectrum block on the wave's scratch.
+struct FftGroup {
+    FftLaunch L;
+    Inst<double> grid;
+    void* spec;
+    long spec_stride;
+    int nb;
+};
+// Appends the persistent launches of one FFT direction (one per kernel
+// variant among the groups) to out. R: the arithmetic; grids are FP64.
+// False (nothing appended) for groups whose FFT width has no persistent form.
+template <class R> bool fft_forward_persistent(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& g);
+template <class R> bool fft_backward_persistent(std::vector<PersistentLaunch>& out, const std::vector<FftGroup>& g);
+// The streaming Legendre and radial stages (zgl_stream.hip); false when a
+// group does not fit the streaming kernels (nothing appended then).
+template <class T> bool zt_legendre_forward_persistent(std::vector<PersistentLaunch>& out,
+                                                       const std::vector<ZtGemmArgs>& g);
+template <class T> bool zt_legendre_backward_persistent(std::vector<PersistentLaunch>& out,
+                                                        const std::vector<ZtGemmArgs>& g);
+template <class T> bool zt_radial_forward_persistent(std::vector<PersistentLaunch>& out,
+                                                     const std::vector<ZtGemmArgs>& g);
+template <class T> bool zt_radial_backward_persistent(std::vector<PersistentLaunch>& out,
+                                                      const std::vector<ZtGemmArgs>& g);
+
 }  // namespace zgl

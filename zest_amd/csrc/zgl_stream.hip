@@ -529,19 +529,19 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
 
 // PLM: the P block is recomputed in the workgroup (no table)
 // X3: the BF16x3 tile loop (FP32 stages only)
-template <class S, int NS, bool PLM, bool X3 = false>
-__global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb) {
+// One work item wk (problem, first row) for instance group `ig` of nb instances.
+template <class S, int NS, bool PLM, bool X3>
+__device__ __forceinline__ void stream_body(const S& st, const int4 wk, const int ig, const int nb) {
     using T = typename S::T;
     constexpr int TM = 16 * NS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* As = reinterpret_cast<T*>(smem_raw);
 
-    const int4 wk = work[blockIdx.x];
     const int m0 = wk.y;
     const typename S::Prob P = st.problem(wk.x, m0);
     const int rows = min(TM, P.M - m0);
     if (rows <= 0) return;
-    const int g0 = blockIdx.y * st.G;
+    const int g0 = ig * st.G;
     const int gn = min(st.G, nb - g0);
     if (gn <= 0) return;
 
@@ -593,6 +593,35 @@ __global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* _
         default: if constexpr (NS >= 4) ZGL_TILES(4); break;
     }
 #undef ZGL_TILES
+}
+
+template <class S, int NS, bool PLM, bool X3 = false>
+__global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb) {
+    stream_body<S, NS, PLM, X3>(st, work[blockIdx.x], blockIdx.y, nb);
+}
+
+// Persistent form over many orders (ragged batches): workgroup b walks items
+// [bounds[b], bounds[b + 1]), item = (group, work index, instance group).
+template <class S>
+struct StreamGroup {
+    S st;
+    const int4* work;
+    int nb;
+    int nwork;  // items of the work list
+};
+
+template <class S, int NS, bool PLM>
+__global__ __launch_bounds__(kThreads, 3) void stream_persistent(const StreamGroup<S>* __restrict__ groups,
+                                                                 const int4* __restrict__ items,
+                                                                 const int* __restrict__ bounds) {
+    const int w1 = bounds[blockIdx.x + 1];
+    for (int w = bounds[blockIdx.x]; w < w1; ++w) {
+        const int4 it = items[w];
+        const StreamGroup<S>& gr = groups[it.x];
+        const S st = gr.st;
+        __syncthreads();  // the previous item is done with the LDS
+        stream_body<S, NS, PLM, false>(st, gr.work[it.y], it.z, gr.nb);
+    }
 }
 
 // Two workgroups of 80 KB fit one CU's 160 KB of LDS.
@@ -756,9 +785,10 @@ __device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* A
 // FP64 operands need more registers than three waves per SIMD allow.
 template <class T> constexpr int kRadWaves = sizeof(T) == 8 ? 2 : 3;
 
+// bx: the workgroup index of the one-dimensional launch
 template <class T, int NS>
-__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdArgs<T> a, const int4* __restrict__ work,
-                                                             int nb) {
+__device__ __forceinline__ void rad_fwd_body(const RadFwdArgs<T>& a, const int4* __restrict__ work, const int bx,
+                                             const int nb) {
     constexpr int TM = 16 * NS;
     constexpr int KI = 4 * RadCfg<T>::KV;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -766,13 +796,13 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdA
     // one-dimensional grid, instance group fastest: every group starts the
     // costliest items (the list is sorted by cost) first
     const int ng = (nb + a.G - 1) / a.G;
-    const int wi = blockIdx.x / ng;
+    const int wi = bx / ng;
     const int4 wk = work[wi];
     const int l = wk.x, i0 = wk.y;
     const int M = l < a.nt ? (a.nt - l + 1) / 2 : 0;
     const int rows = min(TM, M - i0);
     if (rows <= 0) return;
-    const int g0 = (blockIdx.x - wi * ng) * a.G;
+    const int g0 = (bx - wi * ng) * a.G;
     const int gn = min(a.G, nb - g0);
     if (gn <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -791,6 +821,12 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdA
     }
 }
 
+template <class T, int NS>
+__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdArgs<T> a, const int4* __restrict__ work,
+                                                             int nb) {
+    rad_fwd_body<T, NS>(a, work, blockIdx.x, nb);
+}
+
 // Backward: flm[l,m][k] = sum_i R_nl(r_k) c[n,l,m], n = l + 2i.
 // A workgroup owns one l and G instances; the R block (rows i, contiguous k)
 // sits in LDS. A wave's tile is one instance and 16 values of m: it holds the
@@ -807,7 +843,7 @@ struct RadBwdArgs {
 };
 
 template <class T>
-__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdArgs<T> a, int nb) {
+__device__ __forceinline__ void rad_bwd_body(const RadBwdArgs<T>& a, const int bx, const int nb) {
     using M = SM<T>;
     using V = typename M::V;
     constexpr int VW = M::VW, TNW = 16 * VW;
@@ -817,12 +853,12 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdA
     // from the middle outwards (c, c-1, c+1, ..., c = nt/2), roughly by
     // decreasing cost (l+1)(nt-l+1)/2, and covers [0, nt) exactly once
     const int ng = (nb + a.G - 1) / a.G;
-    const int r = blockIdx.x / ng, d = (r + 1) >> 1;
+    const int r = bx / ng, d = (r + 1) >> 1;
     const int l = (r & 1) ? a.nt / 2 - d : a.nt / 2 + d;
     if (l < 0 || l >= a.nt) return;
     const int K = (a.nt - l + 1) / 2;
     const int steps = (K + 3) >> 2;
-    const int g0 = (blockIdx.x - r * ng) * a.G;
+    const int g0 = (bx - r * ng) * a.G;
     const int gn = min(a.G, nb - g0);
     if (gn <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -894,6 +930,53 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdA
                 }
             }
         }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdArgs<T> a, int nb) {
+    rad_bwd_body<T>(a, blockIdx.x, nb);
+}
+
+// Persistent forms of the radial kernels (ragged batches): item = (group,
+// workgroup index of the group's one-dimensional launch).
+template <class T>
+struct RadFwdGroup {
+    RadFwdArgs<T> a;
+    const int4* work;
+    int nb;
+};
+template <class T>
+struct RadBwdGroup {
+    RadBwdArgs<T> a;
+    int nb;
+};
+
+template <class T, int NS>
+__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_persistent(const RadFwdGroup<T>* __restrict__ groups,
+                                                                  const int4* __restrict__ items,
+                                                                  const int* __restrict__ bounds) {
+    const int w1 = bounds[blockIdx.x + 1];
+    for (int w = bounds[blockIdx.x]; w < w1; ++w) {
+        const int4 it = items[w];
+        const RadFwdGroup<T>& gr = groups[it.x];
+        const RadFwdArgs<T> a = gr.a;
+        __syncthreads();
+        rad_fwd_body<T, NS>(a, gr.work, it.y, gr.nb);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_persistent(const RadBwdGroup<T>* __restrict__ groups,
+                                                                  const int4* __restrict__ items,
+                                                                  const int* __restrict__ bounds) {
+    const int w1 = bounds[blockIdx.x + 1];
+    for (int w = bounds[blockIdx.x]; w < w1; ++w) {
+        const int4 it = items[w];
+        const RadBwdGroup<T>& gr = groups[it.x];
+        const RadBwdArgs<T> a = gr.a;
+        __syncthreads();
+        rad_bwd_body<T>(a, it.y, gr.nb);
     }
 }
 
@@ -983,6 +1066,162 @@ bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
                                        a.nb, a.group & 15, s, legendre_x3(a.group));
 }
 
+// ---- persistent launches over many orders (ragged batches) ----
+namespace {
+
+// Uploads the group array and the dealt items of one persistent launch and
+// appends it to out.
+template <class Group, class Kernel>
+void persistent_launch(std::vector<PersistentLaunch>& out, const char* stage, Kernel* k, size_t lds,
+                       const std::vector<Group>& groups, const std::vector<int4>& items,
+                       const std::vector<double>& cost) {
+    if (items.empty()) return;
+    PersistentLaunch pl;
+    pl.stage = stage;
+    for (const Group& g : groups) pl.instances += g.nb;
+    allow_lds(k, lds);
+    std::vector<int4> dealt;
+    std::vector<int> bounds;
+    lpt_deal(items, cost, resident_blocks(reinterpret_cast<const void*>(k), kThreads, lds), dealt, bounds, pl);
+    const int blocks = (int)bounds.size() - 1;
+    const Group* dg = upload_owned(pl, groups);
+    const int4* di = upload_owned(pl, dealt);
+    const int* db = upload_owned(pl, bounds);
+    pl.items = (long)items.size();
+    pl.blocks = blocks;
+    pl.run = [=](hipStream_t s) {
+        k<<<blocks, kThreads, lds, s>>>(dg, di, db);
+        check_hip(hipGetLastError(), stage);
+    };
+    out.push_back(std::move(pl));
+}
+
+// Items (group, work index, instance group) of a streaming Legendre stage;
+// the modelled cost of an item is its full tile: 16 NS rows x kmax x ncol x G.
+template <int NS, class S>
+bool legendre_persistent(std::vector<PersistentLaunch>& out, const char* stage, std::vector<StreamGroup<S>>& groups,
+                         const std::vector<int>& kmax) {
+    using T = typename S::T;
+    size_t lds = 0;
+    std::vector<int4> items;
+    std::vector<double> cost;
+    for (int j = 0; j < (int)groups.size(); ++j) {
+        StreamGroup<S>& gr = groups[j];
+        if (!gr.st.ptab) return false;  // P blocks recomputed in the kernel: no persistent form
+        const int kp = (kmax[j] + 3) & ~3;
+        gr.st.as = kp + 1;
+        const size_t l = (size_t)(16 * NS + (S::kCenter ? 1 : 0)) * gr.st.as * sizeof(T);
+        if (l > kMaxStreamLds) return false;
+        lds = std::max(lds, l);
+    }
+    for (int j = 0; j < (int)groups.size(); ++j) {
+        const StreamGroup<S>& gr = groups[j];
+        const int ng = (gr.nb + gr.st.G - 1) / gr.st.G;
+        const double c = 16.0 * NS * kmax[j] * gr.st.ncol * gr.st.G;
+        for (int w = 0; w < gr.nwork; ++w)
+            for (int ig = 0; ig < ng; ++ig) {
+                items.push_back(make_int4(j, w, ig, 0));
+                cost.push_back(c);
+            }
+    }
+    persistent_launch(out, stage, &stream_persistent<S, NS, false>, lds, groups, items, cost);
+    return true;
+}
+
+}  // namespace
+
+template <class T>
+bool zt_legendre_forward_persistent(std::vector<PersistentLaunch>& out, const std::vector<ZtGemmArgs>& g) {
+    std::vector<StreamGroup<LegFwd<T>>> groups;
+    std::vector<int> kmax;
+    for (const ZtGemmArgs& a : g) {
+        const Geom& G = a.t->g;
+        LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
+                     a.flm_stride * 2, G.order, a.nt, G.jh, G.ncol, group_of(a.group & 15, a.nb), 0,
+                     a.t->zh, a.t->plm_qm, a.t->plm_phase};
+        groups.push_back({st, a.t->work_lf, a.nb, a.t->n_lf});
+        kmax.push_back(G.jh);
+    }
+    return legendre_persistent<kNsLegendreFwd>(out, "legendre_forward", groups, kmax);
+}
+
+template <class T>
+bool zt_legendre_backward_persistent(std::vector<PersistentLaunch>& out, const std::vector<ZtGemmArgs>& g) {
+    std::vector<StreamGroup<LegBwd<T>>> groups;
+    std::vector<int> kmax;
+    for (const ZtGemmArgs& a : g) {
+        const Geom& G = a.t->g;
+        LegBwd<T> st{ptab_sel<T>(*a.t), (const T*)a.flm, (T*)a.spec, a.spec_stride * 2,
+                     a.flm_stride * 2, G.order, a.nt, G.jh, G.ncol, group_of(a.group & 15, a.nb), 0, G.nth,
+                     a.t->zh, a.t->plm_qm, a.t->plm_phase};
+        groups.push_back({st, a.t->work_sb, a.nb, a.t->n_sb});
+        kmax.push_back(plm_count(G.order, 0, 0));
+    }
+    return legendre_persistent<kNsStreamBwd>(out, "legendre_backward", groups, kmax);
+}
+
+template <class T>
+bool zt_radial_forward_persistent(std::vector<PersistentLaunch>& out, const std::vector<ZtGemmArgs>& g) {
+    constexpr int KI = 4 * RadCfg<T>::KV;
+    std::vector<RadFwdGroup<T>> groups;
+    std::vector<int4> items;
+    std::vector<double> cost;
+    size_t lds = 0;
+    for (int j = 0; j < (int)g.size(); ++j) {
+        const ZtGemmArgs& a = g[j];
+        const Geom& G = a.t->g;
+        const int kp = (G.ncol + KI - 1) / KI * KI;
+        RadFwdArgs<T> r{rtab_sel<T>(*a.t), (const T*)a.flm, a.coeffs, a.flm_stride * 2,
+                        a.nt, G.ncol, a.zernike_unnormed, group_of(a.group, a.nb), kp + 2};
+        const size_t l = (size_t)16 * kNsRadial * r.as * sizeof(T);
+        if (l > kMaxStreamLds) return false;
+        lds = std::max(lds, l);
+        groups.push_back({r, a.t->work_rf, a.nb});
+        const int ng = (a.nb + r.G - 1) / r.G;
+        // an item: up to 16 NS rows of one l against its l + 1 values of m, G instances
+        const double c = 16.0 * kNsRadial * G.ncol * (0.5 * a.nt + 1) * r.G;
+        for (int bx = 0; bx < a.t->n_rf * ng; ++bx) {
+            items.push_back(make_int4(j, bx, 0, 0));
+            cost.push_back(c);
+        }
+    }
+    persistent_launch(out, "radial_forward", &rad_fwd_persistent<T, kNsRadial>, lds, groups, items, cost);
+    return true;
+}
+
+template <class T>
+bool zt_radial_backward_persistent(std::vector<PersistentLaunch>& out, const std::vector<ZtGemmArgs>& g) {
+    constexpr int TNW = 16 * SM<T>::VW;
+    std::vector<RadBwdGroup<T>> groups;
+    std::vector<int4> items;
+    std::vector<double> cost;
+    size_t lds = 0;
+    for (int j = 0; j < (int)g.size(); ++j) {
+        const ZtGemmArgs& a = g[j];
+        const Geom& G = a.t->g;
+        const int kmax = (G.order + 1) / 2;
+        if (kmax > 4 * kRadBwdSteps) return false;
+        const int rs = G.ncol;
+        const size_t l = ((size_t)4 * ((kmax + 3) / 4) * rs + TNW) * sizeof(T);
+        if (l > kMaxStreamLds) return false;
+        lds = std::max(lds, l);
+        RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
+                        a.nt, G.ncol, group_of(a.group, a.nb), rs};
+        groups.push_back({r, a.nb});
+        const int ng = (a.nb + r.G - 1) / r.G;
+        for (int bx = 0; bx < a.nt * ng; ++bx) {
+            // rank r of the kernel's middle-out order of l, as rad_bwd_body maps it
+            const int rk = bx / ng, d = (rk + 1) >> 1;
+            const int lv = (rk & 1) ? a.nt / 2 - d : a.nt / 2 + d;
+            if (lv < 0 || lv >= a.nt) continue;
+            items.push_back(make_int4(j, bx, 0, 0));
+            cost.push_back((double)(lv + 1) * ((a.nt - lv + 1) / 2) * G.ncol * r.G);
+        }
+    }
+    persistent_launch(out, "radial_backward", &rad_bwd_persistent<T>, lds, groups, items, cost);
+    return true;
+}
+
 bool stream_legendre_fits(int order, int jh) {
     return stream_lds(16 * kNsLegendreFwd, false, jh, sizeof(double)) <= kMaxStreamLds &&
            stream_lds(16 * kNsStreamBwd, true, plm_count(order, 0, 0), sizeof(double)) <= kMaxStreamLds;
@@ -1023,5 +1262,13 @@ template bool zt_radial_forward_stream<double>(const ZtGemmArgs&, hipStream_t);
 template bool zt_radial_forward_stream<float>(const ZtGemmArgs&, hipStream_t);
 template bool zt_radial_backward_stream<double>(const ZtGemmArgs&, hipStream_t);
 template bool zt_radial_backward_stream<float>(const ZtGemmArgs&, hipStream_t);
+template bool zt_legendre_forward_persistent<double>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_legendre_forward_persistent<float>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_legendre_backward_persistent<double>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_legendre_backward_persistent<float>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_radial_forward_persistent<double>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_radial_forward_persistent<float>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_radial_backward_persistent<double>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
+template bool zt_radial_backward_persistent<float>(std::vector<PersistentLaunch>&, const std::vector<ZtGemmArgs>&);
 
 }  // namespace zgl
