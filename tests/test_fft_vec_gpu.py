@@ -68,3 +68,4 @@ def test_misaligned_stride_takes_scalar_path(device, monkeypatch):
         ref = oracle.zt_backward(c[1], 24)
         tol = 1e-12 if precision == 0 else 1e-6
         assert np.abs(g[1] - ref).max() <= tol * np.abs(ref).max()
+

@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "zgl_internal.hpp"
@@ -590,6 +591,10 @@ struct FftBwdArgs {
     double center_factor;
 };
 
+template <int T, class R, class G, int NT, bool VEC>
+__device__ __forceinline__ void fft_forward_output(const FftFwdArgs<R, G>& a, int tile, int inst,
+                                                   const typename Cplx<R>::type* data);
+
 // One tile of one instance; copy_tables = false when the LDS already holds
 // this order's tables.
 template <int T, int LK, class R, class G, int MF, bool VEC>
@@ -599,10 +604,6 @@ __device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, cons
     constexpr int NT = fft_threads<T, R, MF>();
     const FftDesc& d = a.d;
     const PairMap& pm = a.pm;
-    const int nm = a.nm;
-    const double scale = a.scale, center_factor = a.center_factor;
-    const double* __restrict__ wlat = a.wlat;
-    const double* __restrict__ wrad = a.wrad;
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
@@ -612,7 +613,6 @@ __device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, cons
     thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
     const G* g = a.grid.at(inst);
-    C* sp = a.spec + (long)inst * a.spec_stride;
 
     // Loads are issued in batches of kBatch per thread before any LDS store,
     // so each thread keeps that many independent global reads in flight.
@@ -699,7 +699,24 @@ __device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, cons
     }
 
     run_passes<T, -1, MF, NT>(data, twl, twp, d);
+    fft_forward_output<T, R, G, NT, VEC>(a, tile, inst, data);
+}
 
+// The output phase of the forward FFT: the parity split, weights and the
+// spectrum stores of one tile, from the transformed sequences in LDS.
+template <int T, class R, class G, int NT, bool VEC>
+__device__ __forceinline__ void fft_forward_output(const FftFwdArgs<R, G>& a, const int tile, const int inst,
+                                                   const typename Cplx<R>::type* data) {
+    using C = typename Cplx<R>::type;
+    const PairMap& pm = a.pm;
+    const int n = a.d.n;
+    const int nm = a.nm;
+    const double scale = a.scale, center_factor = a.center_factor;
+    const double* __restrict__ wlat = a.wlat;
+    const double* __restrict__ wrad = a.wrad;
+    C* sp = a.spec + (long)inst * a.spec_stride;
+    int th, tc;
+    thread_column<T, NT>(pm, tile, th, tc);
     if constexpr (spec_vec<R, VEC>() == 2) {
         // two consecutive columns per lane: 16-byte spectrum stores
         constexpr int TP = T / 2;
@@ -1096,11 +1113,12 @@ void launch_forward(const FftLaunch& L, Inst<const G> grid, void* spec, long ss,
                                                       fft_forward_kernel<T, GRID_ROWS, R, G, lean_mf<T>(), false>,
                                                       fft_forward_kernel<T, GRID_ROWS, R, G, wide_mf<T>(), false>);
     if constexpr (T >= 8) {
-        if (use_vec<T, R, G>(L, pm, spec, ss))
+        if (use_vec<T, R, G>(L, pm, spec, ss)) {
             k = pick_variant(f, fft_forward_kernel<T, GRID_COLS, R, G, 0, true>,
                              fft_forward_kernel<T, GRID_COLS, R, G, 1, true>,
                              fft_forward_kernel<T, GRID_COLS, R, G, lean_mf<T>(), true>,
                              fft_forward_kernel<T, GRID_COLS, R, G, wide_mf<T>(), true>);
+        }
     }
     if (lds > 65536)
         check_hip(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
