@@ -67,6 +67,36 @@ template <> struct SM<double> {
 
 constexpr int kThreads = 256;
 constexpr int kKC = 6;  // k-steps of 4 per register chunk (K = 65 -> 3 chunks)
+constexpr int kMaxGroup = 16;  // instances per workgroup
+
+// Raw buffer accesses (a wave-uniform base and byte range, 32-bit lane
+// offsets): a lane whose offset lies past the range loads zeros and its store
+// is dropped by the hardware, so masked accesses need no branch and every
+// lane issues the same loads and stores -- the counts of memory operations in
+// flight stay static, and a wait for a load never waits for stores issued
+// after it.
+typedef unsigned int bu4 __attribute__((vector_size(16)));
+constexpr unsigned kOutOfRange = 0x80000000u;  // past every range used here (< 2 GiB)
+__device__ inline const void* wave_uniform(const void* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<const void*>(((unsigned long long)hi << 32) | lo);
+}
+__device__ inline __amdgpu_buffer_rsrc_t buffer_range(const void* base, unsigned bytes) {
+    // dword 3 of a raw gfx9 buffer resource: 32-bit data format, no swizzle
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(wave_uniform(base)), 0, (int)bytes, 0x00020000);
+}
+template <class V>
+__device__ inline V buffer_load16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    static_assert(sizeof(V) == 16, "16-byte accesses");
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+template <class V>
+__device__ inline void buffer_store16(__amdgpu_buffer_rsrc_t r, unsigned off, V v) {
+    static_assert(sizeof(V) == 16, "16-byte accesses");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bu4, v), r, (int)off, 0, 0);
+}
 
 // Legendre forward: flm[l,m][c] = sum_h P_lm(z_h) X^par_m[h][c], l = m+par+2i.
 template <class TT>
@@ -97,6 +127,8 @@ struct LegFwd {
     __device__ const T* b_base(int inst) const { return spec + inst * spec_stride; }
     __device__ unsigned b_off(const Prob& p, int k) const { return (unsigned)(p.boff + k * 2 * ncol); }
     __device__ T* c_base(int inst) const { return flm + inst * flm_stride; }
+    __host__ __device__ long b_stride() const { return spec_stride; }
+    __host__ __device__ long c_stride() const { return flm_stride; }
     __device__ unsigned c_off(const Prob& p, int i) const {
         return (unsigned)(tri_idx(p.m + p.par + 2 * i, p.m) * 2 * ncol);
     }
@@ -154,6 +186,8 @@ struct LegBwd {
         return (unsigned)(tri_idx(p.m + p.par + 2 * k, p.m) * 2 * ncol);
     }
     __device__ T* c_base(int inst) const { return spec + inst * spec_stride; }
+    __host__ __device__ long b_stride() const { return flm_stride; }
+    __host__ __device__ long c_stride() const { return spec_stride; }
     // output row r is half latitude h = r + hoff (r = -hoff: the equator)
     __device__ unsigned c_off(const Prob& p, int r) const {
         return (unsigned)(p.boff + (r + p.hoff) * 2 * ncol);
@@ -215,6 +249,8 @@ struct LegEval {
         return (unsigned)(tri_idx(p.m + l0 + k, p.m) * 2 * ncol);
     }
     __device__ T* c_base(int inst) const { return spec + inst * spec_stride; }
+    __host__ __device__ long b_stride() const { return flm_stride; }
+    __host__ __device__ long c_stride() const { return spec_stride; }
     __device__ unsigned c_off(const Prob& p, int r) const { return (unsigned)(p.boff + (long)r * 2 * ncol); }
     // As[r][k] = P_{m+l0+k,m}(z_{m0+r})
     __device__ void plm_fill(T* As, int AS, const Prob& p, int m0, int rows, int) const {
@@ -271,23 +307,35 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
         col = fc - gi * N;
     };
 
+    // Buffer accesses relative to the group's first instance (the host keeps
+    // a group's operand and output spans below 2 GiB): lanes past the group
+    // or the K axis load zeros, so every lane issues all kKC loads, the
+    // number of loads in flight is static, and the MFMAs of a chunk wait
+    // only for their own operands.
+    const __amdgpu_buffer_rsrc_t rb =
+        buffer_range(st.b_base(g0), (unsigned)(gn * st.b_stride() * (long)sizeof(T)));
+    const __amdgpu_buffer_rsrc_t rc =
+        buffer_range(st.c_base(g0), (unsigned)(gn * st.c_stride() * (long)sizeof(T)));
     auto load = [&](V (&buf)[kKC], int u) {
         int inst, col, ch;
         unit(u, inst, col, ch);
         const bool colok = inst < g0 + gn;
-        const T* bb = st.b_base(inst);
+        const unsigned ib = (unsigned)((inst - g0) * st.b_stride()) + col;
 #pragma unroll
         for (int s = 0; s < kKC; ++s) {
             const int k = 4 * (ch * kKC + s) + g;
-            V x = {};
-            if (colok && k < K) x = *reinterpret_cast<const V*>(bb + (st.b_off(P, k) + col));
-            buf[s] = x;
+            const bool ok = colok && k < K;
+            buf[s] = buffer_load16<V>(rb, ok ? (ib + st.b_off(P, k)) * (unsigned)sizeof(T) : kOutOfRange);
         }
     };
 
+    // Every chunk issues its tile's stores -- dropped (out of range) except
+    // after the tile's last chunk -- so the count of stores between a
+    // prefetch and its use is static and its wait never covers stores.
     auto compute = [&](const V (&buf)[kKC], int u) {
         int inst, col, ch;
         unit(u, inst, col, ch);
+        const bool colok = inst < g0 + gn;
 #pragma unroll
         for (int s = 0; s < kKC; ++s) {
             const int ks = ch * kKC + s;
@@ -305,50 +353,75 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
                 }
             }
         }
-        if (ch != nchunks - 1) return;
-        // store the finished tile and clear the accumulators
-        const bool colok = inst < g0 + gn;
-        T* cb = st.c_base(inst);
+        const bool last = ch == nchunks - 1;
+        if constexpr (S::kAccum) {
+            // evaluation passes after the first add to the stored sums
+            if (!last) return;
+            T* cb = st.c_base(inst);
 #pragma unroll
-        for (int q = 0; q < NSA; ++q) {
+            for (int q = 0; q < NSA; ++q) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * q + M::row(lane, r);
-                if (colok && i < rows) {
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * q + M::row(lane, r);
+                    if (colok && i < rows) {
+                        V o;
+#pragma unroll
+                        for (int v = 0; v < VW; ++v) o[v] = acc[q][v][r];
+                        V* dst = reinterpret_cast<V*>(cb + (st.c_off(P, m0 + i) + col));
+                        if (st.accum) o += *dst;
+                        *dst = o;
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < VW; ++v) acc[q][v] = typename M::acc_t{0, 0, 0, 0};
+            }
+        } else {
+            const unsigned ic = (unsigned)((inst - g0) * st.c_stride()) + col;
+#pragma unroll
+            for (int q = 0; q < NSA; ++q) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * q + M::row(lane, r);
                     V o;
 #pragma unroll
                     for (int v = 0; v < VW; ++v) o[v] = acc[q][v][r];
-                    V* dst = reinterpret_cast<V*>(cb + (st.c_off(P, m0 + i) + col));
-                    if constexpr (S::kAccum)
-                        if (st.accum) o += *dst;
-                    *dst = o;
+                    const bool ok = last && colok && i < rows;
+                    buffer_store16(rc, ok ? (ic + st.c_off(P, m0 + i)) * (unsigned)sizeof(T) : kOutOfRange, o);
                 }
-            }
+                if (last)
 #pragma unroll
-            for (int v = 0; v < VW; ++v) acc[q][v] = typename M::acc_t{0, 0, 0, 0};
+                    for (int v = 0; v < VW; ++v) acc[q][v] = typename M::acc_t{0, 0, 0, 0};
+            }
         }
         if constexpr (CENTER) {
             // sum the four lane groups (k = 4s + g) and store from group 0
+            V c = cpart;
 #pragma unroll
             for (int v = 0; v < VW; ++v) {
-                T x = cpart[v];
+                T x = c[v];
                 x += __shfl_xor(x, 16);
                 x += __shfl_xor(x, 32);
-                cpart[v] = x;
+                c[v] = x;
             }
-            if (colok && g == 0)
-                *reinterpret_cast<V*>(cb + (st.c_off(P, -P.hoff) + col)) = cpart;
-            cpart = V{};
+            const bool ok = last && colok && g == 0;
+            buffer_store16(rc, ok ? ((unsigned)((inst - g0) * st.c_stride()) + col + st.c_off(P, -P.hoff)) *
+                                        (unsigned)sizeof(T)
+                                  : kOutOfRange,
+                           c);
+            if (last) cpart = V{};
         }
     };
 
+    // the prefetch is unconditional too: past the last unit it re-reads that
+    // unit (cache hits) and the values are not used
+    if (U <= 0) return;
     V b0[kKC], b1[kKC];
-    if (U > 0) load(b0, 0);
+    load(b0, 0);
     for (int u = 0; u < U; u += 2) {
-        if (u + 1 < U) load(b1, u + 1);
+        load(b1, min(u + 1, U - 1));
         compute(b0, u);
         if (u + 1 >= U) break;
-        if (u + 2 < U) load(b0, u + 2);
+        load(b0, min(u + 2, U - 1));
         compute(b1, u + 1);
     }
 }
@@ -416,6 +489,7 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
         col = (t - gi * ntile) * TNW + VW * j;
     };
     // rows k = 16 (chunk step) + 4 g + r, r < 4
+    // branch-free loads, as in stream_tiles
     auto load = [&](V (&buf)[kKC3][4], int u) {
         int inst, col, ch;
         unit(u, inst, col, ch);
@@ -426,9 +500,8 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 16 * (ch * kKC3 + s) + 4 * g + r;
-                V x = {};
-                if (colok && k < K) x = *reinterpret_cast<const V*>(bb + (st.b_off(P, k) + col));
-                buf[s][r] = x;
+                const bool ok = colok && k < K;
+                buf[s][r] = *reinterpret_cast<const V*>(bb + (ok ? st.b_off(P, k) + col : 0u));
             }
     };
     auto compute = [&](const V (&buf)[kKC3][4], int u) {
@@ -438,13 +511,16 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
         for (int s = 0; s < kKC3; ++s) {
             const int ks = ch * kKC3 + s;
             if (ks < steps) {
+                V b[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) b[r] = (col < N && 16 * ks + 4 * g + r < K) ? buf[s][r] : V{};
                 s4 bh[VW], bm[VW], bl[VW];
 #pragma unroll
                 for (int v = 0; v < VW; ++v)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         short h, m, l;
-                        split3(buf[s][r][v], h, m, l);
+                        split3(b[r][v], h, m, l);
                         bh[v][r] = h;
                         bm[v][r] = m;
                         bl[v][r] = l;
@@ -477,7 +553,7 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
                     for (int r = 0; r < 4; ++r) {
                         const float a = Ac[16 * ks + 4 * g + r];
 #pragma unroll
-                        for (int v = 0; v < VW; ++v) cpart[v] += a * buf[s][r][v];
+                        for (int v = 0; v < VW; ++v) cpart[v] += a * b[r][v];
                     }
                 }
             }
@@ -516,13 +592,14 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
         }
     };
 
+    if (U <= 0) return;
     V b0[kKC3][4], b1[kKC3][4];
-    if (U > 0) load(b0, 0);
+    load(b0, 0);
     for (int u = 0; u < U; u += 2) {
-        if (u + 1 < U) load(b1, u + 1);
+        load(b1, min(u + 1, U - 1));
         compute(b0, u);
         if (u + 1 >= U) break;
-        if (u + 2 < U) load(b0, u + 2);
+        load(b0, min(u + 2, U - 1));
         compute(b1, u + 1);
     }
 }
@@ -637,8 +714,18 @@ void allow_lds(F* kernel, size_t lds) {
 }
 
 int group_of(int requested, int nb) {
-    const int g = requested > 0 ? requested : 4;
+    const int g = requested > 0 ? std::min(requested, kMaxGroup) : 4;
     return nb < g ? nb : g;
+}
+
+// Instances per workgroup of a streaming stage: the requested group, capped
+// so that the group's operand and output spans stay within the 2 GiB reach of
+// the kernels' 32-bit buffer offsets; 0 when one instance exceeds it.
+template <class S>
+int stream_group(const S& st, int requested, int nb) {
+    const long span = std::max(st.b_stride(), st.c_stride()) * (long)sizeof(typename S::T);
+    if (span >= (long)kOutOfRange) return 0;
+    return (int)std::min<long>(group_of(requested, nb), ((long)kOutOfRange - 1) / span);
 }
 
 template <int NS, class S>
@@ -651,7 +738,8 @@ bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, int grou
     const size_t lds = (size_t)(16 * NS + (S::kCenter ? 1 : 0)) * st.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (nwork == 0 || nb <= 0) return true;
-    st.G = group_of(group, nb);
+    st.G = stream_group(st, group, nb);
+    if (st.G <= 0) return false;
     dim3 grid(nwork, (nb + st.G - 1) / st.G);
     auto go = [&](auto kernel) {
         allow_lds(kernel, lds);
@@ -695,8 +783,8 @@ struct RadFwdArgs {
 };
 
 template <int NSA, class T>
-__device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* As, int l, int i0,
-                                              int rows, int g0, int gn, int wave, int lane) {
+__device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* As, double* const* cbase, int l,
+                                              int i0, int rows, int g0, int gn, int wave, int lane) {
     using M = SM<T>;
     using V = typename M::V;
     constexpr int KV = RadCfg<T>::KV, KI = 4 * KV;
@@ -708,35 +796,40 @@ __device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* A
     const int ntw = tiles > wave ? (tiles - wave + 3) >> 2 : 0;
     const int nchunks = (iters + kKC - 1) / kKC;
     const int U = ntw * nchunks;
-    const long rowlen = 2L * a.nr;
+    const unsigned rowbytes = 2u * a.nr * sizeof(T);
+    const unsigned fbytes = (unsigned)(a.flm_stride * sizeof(T));
+    const unsigned cbytes = (unsigned)tet_idx(a.nt, 0, 0) * 16u;  // coefficients n < nt
 
     typename M::acc_t are[NSA], aim[NSA];
 #pragma unroll
     for (int q = 0; q < NSA; ++q) are[q] = aim[q] = typename M::acc_t{0, 0, 0, 0};
 
-    auto unit = [&](int u, int& inst, int& m, int& ch) {
+    auto unit = [&](int u, int& gi, int& m, int& ch) {
         const int tw = u / nchunks;
         ch = u - tw * nchunks;
         const int t = wave + 4 * tw;
-        const int gi = t / nmb;
-        inst = g0 + gi;
+        gi = t / nmb;
         m = 16 * (t - gi * nmb) + j;
     };
+    // buffer loads: lanes past the block (m > l) or the k axis load zeros
     auto load = [&](V (&buf)[kKC], int u) {
-        int inst, m, ch;
-        unit(u, inst, m, ch);
-        const T* row = a.flm + inst * a.flm_stride + tri_idx(l, m) * rowlen;
+        int gi, m, ch;
+        unit(u, gi, m, ch);
+        const __amdgpu_buffer_rsrc_t rf = buffer_range(a.flm + (g0 + gi) * a.flm_stride, fbytes);
+        const unsigned rowoff = (unsigned)tri_idx(l, m) * rowbytes;
 #pragma unroll
         for (int s = 0; s < kKC; ++s) {
             const int k = (ch * kKC + s) * KI + KV * g;
-            V x = {};
-            if (m <= l && k < a.nr) x = *reinterpret_cast<const V*>(row + 2 * k);
-            buf[s] = x;
+            const bool ok = m <= l && k < a.nr;
+            buf[s] = buffer_load16<V>(rf, ok ? rowoff + 2u * k * sizeof(T) : kOutOfRange);
         }
     };
+    // Every chunk issues the tile's coefficient stores -- dropped (out of
+    // range) except after its last chunk -- so the count of stores between a
+    // prefetch and its use is static and its wait never covers the stores.
     auto compute = [&](const V (&buf)[kKC], int u) {
-        int inst, m, ch;
-        unit(u, inst, m, ch);
+        int gi, m, ch;
+        unit(u, gi, m, ch);
 #pragma unroll
         for (int s = 0; s < kKC; ++s) {
             const int it = ch * kKC + s;
@@ -752,32 +845,31 @@ __device__ __forceinline__ void rad_fwd_tiles(const RadFwdArgs<T>& a, const T* A
                 }
             }
         }
-        if (ch != nchunks - 1) return;
-        double* cb = a.coeffs.at(inst);
+        const bool last = ch == nchunks - 1;
+        const __amdgpu_buffer_rsrc_t rc = buffer_range(cbase[gi], cbytes);
 #pragma unroll
         for (int q = 0; q < NSA; ++q) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = 16 * q + M::row(lane, r);
-                if (i < rows && m <= l) {
-                    const int n = l + 2 * (i0 + i);
-                    const double kappa = a.unnormed ? (double)(2 * n + 3) : 1.0;
-                    double* o = cb + tet_idx(n, l, m) * 2;
-                    o[0] = (double)are[q][r] * kappa;
-                    o[1] = (double)aim[q][r] * kappa;
-                }
+                const int n = l + 2 * (i0 + i);
+                const double kappa = a.unnormed ? (double)(2 * n + 3) : 1.0;
+                const bool ok = last && i < rows && m <= l;
+                const d2 o = {(double)are[q][r] * kappa, (double)aim[q][r] * kappa};
+                buffer_store16(rc, ok ? (unsigned)tet_idx(n, l, m) * 16u : kOutOfRange, o);
             }
-            are[q] = aim[q] = typename M::acc_t{0, 0, 0, 0};
+            if (last) are[q] = aim[q] = typename M::acc_t{0, 0, 0, 0};
         }
     };
 
+    if (U <= 0) return;
     V b0[kKC], b1[kKC];
-    if (U > 0) load(b0, 0);
+    load(b0, 0);
     for (int u = 0; u < U; u += 2) {
-        if (u + 1 < U) load(b1, u + 1);
+        load(b1, min(u + 1, U - 1));
         compute(b0, u);
         if (u + 1 >= U) break;
-        if (u + 2 < U) load(b0, u + 2);
+        load(b0, min(u + 2, U - 1));
         compute(b1, u + 1);
     }
 }
@@ -812,12 +904,15 @@ __device__ __forceinline__ void rad_fwd_body(const RadFwdArgs<T>& a, const int4*
         const int i = e / KP, k = e - i * KP;
         As[i * a.as + k] = (i < rows && k < a.nr) ? a.rtab[rad_idx(l + 2 * (i0 + i), l) * a.nr + k] : T(0);
     }
+    // coefficient bases of the group's instances, looked up once
+    __shared__ double* cbase[kMaxGroup];
+    if (tid < gn) cbase[tid] = a.coeffs.at(g0 + tid);
     __syncthreads();
     switch ((rows + 15) >> 4) {
-        case 1: rad_fwd_tiles<1>(a, As, l, i0, rows, g0, gn, wave, lane); break;
-        case 2: if constexpr (NS >= 2) rad_fwd_tiles<2>(a, As, l, i0, rows, g0, gn, wave, lane); break;
-        case 3: if constexpr (NS >= 3) rad_fwd_tiles<3>(a, As, l, i0, rows, g0, gn, wave, lane); break;
-        default: if constexpr (NS >= 4) rad_fwd_tiles<4>(a, As, l, i0, rows, g0, gn, wave, lane); break;
+        case 1: rad_fwd_tiles<1>(a, As, cbase, l, i0, rows, g0, gn, wave, lane); break;
+        case 2: if constexpr (NS >= 2) rad_fwd_tiles<2>(a, As, cbase, l, i0, rows, g0, gn, wave, lane); break;
+        case 3: if constexpr (NS >= 3) rad_fwd_tiles<3>(a, As, cbase, l, i0, rows, g0, gn, wave, lane); break;
+        default: if constexpr (NS >= 4) rad_fwd_tiles<4>(a, As, cbase, l, i0, rows, g0, gn, wave, lane); break;
     }
 }
 
@@ -833,7 +928,7 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_kernel(RadFwdA
 // (re, im) coefficients of its 16 m for every i in registers (the A operand)
 // and sweeps the k axis in tiles of 16*VW columns with the column permutation
 // of the Legendre kernels, storing VW interleaved (re, im) pairs per lane.
-constexpr int kRadBwdSteps = 16;  // i < 64
+constexpr int kRadBwdSteps = 16;  // i < 64; the R block holds 16 * ceil(K / 16) rows
 
 template <class T>
 struct RadBwdArgs {
@@ -842,11 +937,102 @@ struct RadBwdArgs {
     int nt, nr, G, rs;
 };
 
-template <class T>
-__device__ __forceinline__ void rad_bwd_body(const RadBwdArgs<T>& a, const int bx, const int nb) {
+// The tiles of one wave with SC*4 MFMA steps (a compile-time count, so the
+// coefficient loads and the MFMA sequence are branch-free).
+template <int SC, class T>
+__device__ __forceinline__ void rad_bwd_tiles(const RadBwdArgs<T>& a, const T* Rs, const double* const* cbase,
+                                              int l, int K, int g0, int gn, int wave, int lane) {
     using M = SM<T>;
     using V = typename M::V;
     constexpr int VW = M::VW, TNW = 16 * VW;
+    constexpr int NSTEP = 4 * SC;
+    const int steps = (K + 3) >> 2;  // MFMA steps (the loads cover NSTEP)
+    const int j = lane & 15, g = lane >> 4;
+    const int RS = a.rs;
+    const int nmb = (l + 16) >> 4;
+    const int tiles = gn * nmb;
+    if (wave >= tiles) return;
+    const int ntile = (a.nr + TNW - 1) / TNW;  // <= NT: nr <= 136 (order <= 128)
+    constexpr int NT = (136 + TNW - 1) / TNW;
+    const unsigned rowbytes = 2u * a.nr * sizeof(T);
+    const unsigned cbytes = (unsigned)tet_idx(a.nt, 0, 0) * 16u;  // coefficients n < nt
+    const unsigned fbytes = (unsigned)(a.flm_stride * sizeof(T));
+
+    // A operand of tile t: lane (row j -> m = mb + j, group g -> i = 4s + g),
+    // the (re, im) FP64 pairs of its coefficients; lanes past the block load
+    // zeros (out-of-range offsets)
+    d2 cv[NSTEP];
+    auto load = [&](int t) {
+        const int gi = t / nmb;
+        const int mb = 16 * (t - gi * nmb);
+        const __amdgpu_buffer_rsrc_t rc = buffer_range(cbase[gi], cbytes);
+#pragma unroll
+        for (int s = 0; s < NSTEP; ++s) {
+            const int i = 4 * s + g, m = mb + j;
+            const bool ok = i < K && m <= l;
+            cv[s] = buffer_load16<d2>(rc, ok ? (unsigned)tet_idx(l + 2 * i, l, m) * 16u : kOutOfRange);
+        }
+    };
+
+    load(wave);
+    for (int t = wave; t < tiles; t += 4) {
+        T xre[NSTEP], xim[NSTEP];
+#pragma unroll
+        for (int s = 0; s < NSTEP; ++s) {
+            xre[s] = (T)cv[s][0];
+            xim[s] = (T)cv[s][1];
+        }
+        // the next tile's coefficients are requested before this tile's
+        // stores, so waiting for them never waits for the stores (past the
+        // last tile this re-reads the current one)
+        load(t + 4 < tiles ? t + 4 : t);
+        const int gi = t / nmb;
+        const int mb = 16 * (t - gi * nmb);
+        const __amdgpu_buffer_rsrc_t rf = buffer_range(a.flm + (g0 + gi) * a.flm_stride, fbytes);
+        // a static count of column tiles (the stores of tiles past ntile are
+        // dropped), so the count of stores after the prefetch is static too
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+            typename M::acc_t are[VW], aim[VW];
+#pragma unroll
+            for (int v = 0; v < VW; ++v) are[v] = aim[v] = typename M::acc_t{0, 0, 0, 0};
+            const int col = ct * TNW + VW * j;
+#pragma unroll
+            for (int s = 0; s < NSTEP; ++s) {
+                if (s < steps && ct < ntile) {
+                    const V b = *reinterpret_cast<const V*>(Rs + (4 * s + g) * RS + col);
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) {
+                        are[v] = M::mma(xre[s], b[v], are[v]);
+                        aim[v] = M::mma(xim[s], b[v], aim[v]);
+                    }
+                }
+            }
+            // 2*VW reals per row: (re, im) of columns col .. col+VW-1, two
+            // 16-byte stores; rows past l and columns past nr are dropped
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = mb + M::row(lane, r);
+                const bool ok = m <= l && col < a.nr;
+                const unsigned o = (unsigned)tri_idx(l, m) * rowbytes + 2u * col * sizeof(T);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    V w;
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) {
+                        const int v = (VW * h + e) >> 1;
+                        w[e] = ((VW * h + e) & 1) ? aim[v][r] : are[v][r];
+                    }
+                    buffer_store16(rf, ok ? o + 16u * h : kOutOfRange, w);
+                }
+            }
+        }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void rad_bwd_body(const RadBwdArgs<T>& a, const int bx, const int nb) {
+    constexpr int TNW = 16 * SM<T>::VW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* Rs = reinterpret_cast<T*>(smem_raw);
     // one-dimensional grid, instance group fastest; the rank r enumerates l
@@ -857,84 +1043,40 @@ __device__ __forceinline__ void rad_bwd_body(const RadBwdArgs<T>& a, const int b
     const int l = (r & 1) ? a.nt / 2 - d : a.nt / 2 + d;
     if (l < 0 || l >= a.nt) return;
     const int K = (a.nt - l + 1) / 2;
-    const int steps = (K + 3) >> 2;
+    const int sc = (K + 15) >> 4;  // groups of four MFMA steps, K <= 4 * kRadBwdSteps
     const int g0 = (bx - r * ng) * a.G;
     const int gn = min(a.G, nb - g0);
     if (gn <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int j = lane & 15, g = lane >> 4;
     const int RS = a.rs;
-    // Rs[i * RS + k], RS = nr; columns of the last (partial) tile past nr read
-    // the next row or the zero tail, and their results are not stored
-    for (int e = tid; e < 4 * steps * RS + TNW; e += kThreads) {
+    // Rs[i * RS + k], RS = nr, rows i < 16 sc (zero past K); columns of the
+    // last (partial) tile past nr read the next row or the zero tail, and
+    // their results are not stored
+    for (int e = tid; e < 16 * sc * RS + TNW; e += kThreads) {
         const int i = e / RS, k = e - i * RS;
         Rs[e] = (i < K && k < a.nr) ? a.rtab[rad_idx(l + 2 * i, l) * a.nr + k] : T(0);
     }
+    // coefficient bases of the group's instances, looked up once (a ragged
+    // batch reads them from its offset table), so the tile loop has no
+    // dependent global load
+    __shared__ const double* cbase[kMaxGroup];
+    if (tid < gn) cbase[tid] = a.coeffs.at(g0 + tid);
     __syncthreads();
-
-    const int nmb = (l + 16) >> 4;
-    const int tiles = gn * nmb;
-    const int ntile = (a.nr + TNW - 1) / TNW;
-    const long rowlen = 2L * a.nr;
-    for (int t = wave; t < tiles; t += 4) {
-        const int gi = t / nmb;
-        const int inst = g0 + gi;
-        const int mb = 16 * (t - gi * nmb);
-        // A operand: lane (row j -> m = mb + j, group g -> i = 4s + g)
-        const double* cb = a.coeffs.at(inst);
-        T xre[kRadBwdSteps], xim[kRadBwdSteps];
-#pragma unroll
-        for (int s = 0; s < kRadBwdSteps; ++s) {
-            const int i = 4 * s + g, m = mb + j;
-            xre[s] = xim[s] = T(0);
-            if (s < steps && i < K && m <= l) {
-                const double* c = cb + tet_idx(l + 2 * i, l, m) * 2;
-                xre[s] = (T)c[0];
-                xim[s] = (T)c[1];
-            }
-        }
-        T* ob = a.flm + inst * a.flm_stride;
-        for (int ct = 0; ct < ntile; ++ct) {
-            typename M::acc_t are[VW], aim[VW];
-#pragma unroll
-            for (int v = 0; v < VW; ++v) are[v] = aim[v] = typename M::acc_t{0, 0, 0, 0};
-            const int col = ct * TNW + VW * j;
-#pragma unroll
-            for (int s = 0; s < kRadBwdSteps; ++s) {
-                if (s < steps) {
-                    const V b = *reinterpret_cast<const V*>(Rs + (4 * s + g) * RS + col);
-#pragma unroll
-                    for (int v = 0; v < VW; ++v) {
-                        are[v] = M::mma(xre[s], b[v], are[v]);
-                        aim[v] = M::mma(xim[s], b[v], aim[v]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = mb + M::row(lane, r);
-                if (m <= l && col < a.nr) {
-                    // 2*VW reals: (re, im) of columns col .. col+VW-1, two 16-byte stores
-                    T* o = ob + tri_idx(l, m) * rowlen + 2 * col;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        V w;
-#pragma unroll
-                        for (int e = 0; e < VW; ++e) {
-                            const int v = (VW * h + e) >> 1;
-                            w[e] = ((VW * h + e) & 1) ? aim[v][r] : are[v][r];
-                        }
-                        *reinterpret_cast<V*>(o + VW * h) = w;
-                    }
-                }
-            }
-        }
+    switch (sc) {
+        case 1: rad_bwd_tiles<1>(a, Rs, cbase, l, K, g0, gn, wave, lane); break;
+        case 2: rad_bwd_tiles<2>(a, Rs, cbase, l, K, g0, gn, wave, lane); break;
+        case 3: rad_bwd_tiles<3>(a, Rs, cbase, l, K, g0, gn, wave, lane); break;
+        default: rad_bwd_tiles<4>(a, Rs, cbase, l, K, g0, gn, wave, lane); break;
     }
 }
 
+// The backward keeps the next tile's coefficients in flight beside the
+// current tile's: two waves per SIMD.
+constexpr int kRadBwdWaves = 2;
+
 template <class T>
-__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_kernel(RadBwdArgs<T> a, int nb) {
+__global__ __launch_bounds__(kThreads, kRadBwdWaves) void rad_bwd_kernel(RadBwdArgs<T> a, int nb) {
     rad_bwd_body<T>(a, blockIdx.x, nb);
 }
 
@@ -967,7 +1109,7 @@ __global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_fwd_persistent(con
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads, kRadWaves<T>) void rad_bwd_persistent(const RadBwdGroup<T>* __restrict__ groups,
+__global__ __launch_bounds__(kThreads, kRadBwdWaves) void rad_bwd_persistent(const RadBwdGroup<T>* __restrict__ groups,
                                                                   const int4* __restrict__ items,
                                                                   const int* __restrict__ bounds) {
     const int w1 = bounds[blockIdx.x + 1];
@@ -1006,6 +1148,7 @@ bool zt_radial_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
                     a.nt, g.ncol, a.zernike_unnormed, 1, kp + 2};
     const size_t lds = (size_t)16 * kNsRadial * r.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
+    if ((size_t)a.flm_stride * 2 * sizeof(T) >= kOutOfRange) return false;  // buffer ranges
     if (a.t->n_rf == 0 || a.nb <= 0) return true;
     r.G = group_of(a.group, a.nb);
     dim3 grid((unsigned)a.t->n_rf * (unsigned)((a.nb + r.G - 1) / r.G));
@@ -1020,11 +1163,11 @@ bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     constexpr int TNW = 16 * SM<T>::VW;
     const int kmax = (g.order + 1) / 2;  // i values of l = 0
-    if (kmax > 4 * kRadBwdSteps) return false;
+    if (kmax > 4 * kRadBwdSteps || g.ncol > 136) return false;
     // rows of ncol reals, plus one column tile of zeros so that the last
     // tile of the last row reads inside the array
     const int rs = g.ncol;
-    const size_t lds = ((size_t)4 * ((kmax + 3) / 4) * rs + TNW) * sizeof(T);
+    const size_t lds = ((size_t)16 * ((kmax + 15) / 16) * rs + TNW) * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (a.nt == 0 || a.nb <= 0) return true;
     RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
@@ -1137,8 +1280,10 @@ bool zt_legendre_forward_persistent(std::vector<PersistentLaunch>& out, const st
     for (const ZtGemmArgs& a : g) {
         const Geom& G = a.t->g;
         LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
-                     a.flm_stride * 2, G.order, a.nt, G.jh, G.ncol, group_of(a.group & 15, a.nb), 0,
+                     a.flm_stride * 2, G.order, a.nt, G.jh, G.ncol, 0, 0,
                      a.t->zh, a.t->plm_qm, a.t->plm_phase};
+        st.G = stream_group(st, a.group & 15, a.nb);
+        if (st.G <= 0) return false;
         groups.push_back({st, a.t->work_lf, a.nb, a.t->n_lf});
         kmax.push_back(G.jh);
     }
@@ -1152,8 +1297,10 @@ bool zt_legendre_backward_persistent(std::vector<PersistentLaunch>& out, const s
     for (const ZtGemmArgs& a : g) {
         const Geom& G = a.t->g;
         LegBwd<T> st{ptab_sel<T>(*a.t), (const T*)a.flm, (T*)a.spec, a.spec_stride * 2,
-                     a.flm_stride * 2, G.order, a.nt, G.jh, G.ncol, group_of(a.group & 15, a.nb), 0, G.nth,
+                     a.flm_stride * 2, G.order, a.nt, G.jh, G.ncol, 0, 0, G.nth,
                      a.t->zh, a.t->plm_qm, a.t->plm_phase};
+        st.G = stream_group(st, a.group & 15, a.nb);
+        if (st.G <= 0) return false;
         groups.push_back({st, a.t->work_sb, a.nb, a.t->n_sb});
         kmax.push_back(plm_count(G.order, 0, 0));
     }
@@ -1200,9 +1347,9 @@ bool zt_radial_backward_persistent(std::vector<PersistentLaunch>& out, const std
         const ZtGemmArgs& a = g[j];
         const Geom& G = a.t->g;
         const int kmax = (G.order + 1) / 2;
-        if (kmax > 4 * kRadBwdSteps) return false;
+        if (kmax > 4 * kRadBwdSteps || G.ncol > 136) return false;
         const int rs = G.ncol;
-        const size_t l = ((size_t)4 * ((kmax + 3) / 4) * rs + TNW) * sizeof(T);
+        const size_t l = ((size_t)16 * ((kmax + 15) / 16) * rs + TNW) * sizeof(T);
         if (l > kMaxStreamLds) return false;
         lds = std::max(lds, l);
         RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
