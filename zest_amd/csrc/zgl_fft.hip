@@ -1387,15 +1387,17 @@ C* upload(const std::vector<C>& v, hipStream_t s, const char* what) {
 
 #if ZGL_FFT_PART == 0
 std::vector<int> fft_tiles(const FftPlan& p, bool fp32) {
-    // the plan's width, half of it, and the double width where it fits
-    // (longer contiguous segments per longitude row, fewer workgroups per CU)
+    // the plan's width (first: the autotuner's preference on a near tie),
+    // the double width where it fits (longer contiguous segments per
+    // longitude row, fewer workgroups per CU) and half of it
     std::vector<int> out;
     const size_t cs = fp32 ? sizeof(float2) : sizeof(double2);
     const int T0 = p.tile[fp32 ? 1 : 0];
-    if (T0 < 32 && tile_fits(p.desc.n, p.desc.radix, p.desc.npass, 2 * T0, cs, p.mfma_pass, p.wide))
-        out.push_back(2 * T0);
-    for (int T = T0; T >= 1 && out.size() < 3; T /= 2)
-        if (tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs, p.mfma_pass, p.wide)) out.push_back(T);
+    auto fits = [&](int T) { return tile_fits(p.desc.n, p.desc.radix, p.desc.npass, T, cs, p.mfma_pass, p.wide); };
+    if (fits(T0)) out.push_back(T0);
+    if (T0 < 32 && fits(2 * T0)) out.push_back(2 * T0);
+    for (int T = T0 / 2; T >= 1 && out.size() < 3; T /= 2)
+        if (fits(T)) out.push_back(T);
     return out;
 }
 

@@ -98,6 +98,38 @@ __device__ inline void buffer_store16(__amdgpu_buffer_rsrc_t r, unsigned off, V 
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bu4, v), r, (int)off, 0, 0);
 }
 
+// Fills n elements of a workgroup's table block in LDS from global memory:
+// map(e) gives element e's LDS index and source (ok false: a zero, src then
+// any readable address); consecutive e should be consecutive in memory. B loads per thread are issued before any of them
+// is used, so the fill costs a few memory latencies instead of one per
+// element (a load under a branch inside the loop would serialise them).
+template <class T>
+struct FillSrc {
+    const T* src;
+    bool ok;
+    int dst;  // LDS index
+};
+template <int B, class T, class F>
+__device__ __forceinline__ void lds_fill(T* lds, int n, F&& map) {
+    for (int e0 = threadIdx.x; e0 < n; e0 += B * kThreads) {
+        T v[B];
+        bool ok[B];
+        int d[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const FillSrc<T> f = map(min(e0 + u * kThreads, n - 1));
+            ok[u] = f.ok;
+            d[u] = f.dst;
+            v[u] = *f.src;
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int e = e0 + u * kThreads;
+            if (e < n) lds[d[u]] = ok[u] ? v[u] : T(0);
+        }
+    }
+}
+
 // Legendre forward: flm[l,m][c] = sum_h P_lm(z_h) X^par_m[h][c], l = m+par+2i.
 template <class TT>
 struct LegFwd {
@@ -635,20 +667,23 @@ __device__ __forceinline__ void stream_body(const S& st, const int4 wk, const in
         for (int e = tid; e < nfill; e += kThreads) As[e] = T(0);
         __syncthreads();
         st.plm_fill(As, AS, P, m0, rows, TM);
-    } else if (P.a_sk == 1) {
-        for (int e = tid; e < TM * KP; e += kThreads) {
-            const int i = e / KP, k = e - i * KP;
-            As[i * AS + k] = (i < rows && k < K) ? P.A[(long)i * P.a_si + k] : T(0);
-        }
     } else {
-        for (int e = tid; e < TM * KP; e += kThreads) {
-            const int k = e / TM, i = e - k * TM;
-            As[i * AS + k] = (i < rows && k < K) ? P.A[(long)k * P.a_sk + i] : T(0);
-        }
+        // rows of AS (the padding columns past KP are never read; they get
+        // zeros too); the equator row (P.center) is row TM
+        const int nrow = TM + (center ? 1 : 0);
+        // zeros read the table's first value: P.A itself lies past the end of
+        // the table for a problem without rows (K = 0)
+        // e runs along the table's contiguous axis: k for row-major blocks
+        // (a_sk = 1), the rows i otherwise
+        const bool kfast = P.a_sk == 1;
+        lds_fill<8>(As, nrow * AS, [&](int e) {
+            const int i = kfast ? e / AS : e % nrow, k = kfast ? e - i * AS : e / nrow;
+            const bool eq = i == TM;
+            const bool ok = k < K && (eq || i < rows);
+            const T* src = (eq ? P.Ac : P.A) + (eq ? 0L : (long)i * P.a_si) + (long)k * P.a_sk;
+            return FillSrc<T>{ok ? src : st.ptab, ok, i * AS + k};
+        });
     }
-    if constexpr (!PLM)
-        if (center)
-            for (int k = tid; k < KP; k += kThreads) As[TM * AS + k] = k < K ? P.Ac[(long)k * P.a_sk] : T(0);
     __syncthreads();
 
     const int nstrips = (rows + 15) >> 4;
@@ -899,11 +934,12 @@ __device__ __forceinline__ void rad_fwd_body(const RadFwdArgs<T>& a, const int4*
     if (gn <= 0) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int KP = (a.nr + KI - 1) / KI * KI;
-    for (int e = tid; e < TM * KP; e += kThreads) {
-        const int i = e / KP, k = e - i * KP;
-        As[i * a.as + k] = (i < rows && k < a.nr) ? a.rtab[rad_idx(l + 2 * (i0 + i), l) * a.nr + k] : T(0);
-    }
+    const int AS = a.as;  // >= KP: columns past nr are zeros
+    lds_fill<8>(As, TM * AS, [&](int e) {
+        const int i = e / AS, k = e - i * AS;
+        const bool ok = i < rows && k < a.nr;
+        return FillSrc<T>{a.rtab + (ok ? rad_idx(l + 2 * (i0 + i), l) * a.nr + k : 0L), ok, e};
+    });
     // coefficient bases of the group's instances, looked up once
     __shared__ double* cbase[kMaxGroup];
     if (tid < gn) cbase[tid] = a.coeffs.at(g0 + tid);
@@ -1053,10 +1089,11 @@ __device__ __forceinline__ void rad_bwd_body(const RadBwdArgs<T>& a, const int b
     // Rs[i * RS + k], RS = nr, rows i < 16 sc (zero past K); columns of the
     // last (partial) tile past nr read the next row or the zero tail, and
     // their results are not stored
-    for (int e = tid; e < 16 * sc * RS + TNW; e += kThreads) {
+    lds_fill<8>(Rs, 16 * sc * RS + TNW, [&](int e) {
         const int i = e / RS, k = e - i * RS;
-        Rs[e] = (i < K && k < a.nr) ? a.rtab[rad_idx(l + 2 * i, l) * a.nr + k] : T(0);
-    }
+        const bool ok = i < K && k < a.nr;
+        return FillSrc<T>{a.rtab + (ok ? rad_idx(l + 2 * i, l) * a.nr + k : 0L), ok, e};
+    });
     // coefficient bases of the group's instances, looked up once (a ragged
     // batch reads them from its offset table), so the tile loop has no
     // dependent global load
