@@ -513,7 +513,9 @@ int tuned_choice(zgl_plan* p, size_t order, int rbytes, int gbytes, int stage, h
     return c;
 }
 
-const std::vector<int> kGroupCandidates = {4, 2, 8, 1};  // the untuned default first
+const std::vector<int> kGroupCandidates = {4, 2, 8, 1, 3, 6, 12};  // the untuned default first
+// the radial stages also take 16 instances per workgroup (no flag bits there)
+const std::vector<int> kRadialCandidates = {4, 2, 8, 1, 16};
 // BF16x3 Legendre products (kGroupX3) at every group size: the candidates of
 // ZGL_BF16X3 plans. The FP32 modes keep the FP32 MFMA products: timed alone,
 // a BF16x3 backward stage at order 128 looked faster than the FP32 one, but
@@ -592,7 +594,7 @@ void run_forward(zgl_plan* p, Scratch& sc, Inst<const G> grid, size_t N, Inst<do
             });
             a.group |= legendre_math<T>(p);
             timed(p, ST_LEG_FWD, nb, s, [&] { zt_legendre_forward<T>(a, s); });
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_FWD, s, [&] { return kGroupCandidates; }, [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_FWD, s, [&] { return kRadialCandidates; }, [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_radial_forward<T>(b, s);
@@ -644,7 +646,7 @@ void run_backward(zgl_plan* p, Scratch& sc, Inst<const double> coeffs, size_t nt
             timed(p, ST_FUSED_BWD, nb, s, [&] { zt_fused_backward(a, s); });
         } else if (p->kind == ZGL_ZERNIKE) {
             ZtGemmArgs a{&t, (int)nt, 0, sc.spec, ch.spec_stride, sc.flm, ch.flm_stride, cin, nb};
-            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_BWD, s, [&] { return kGroupCandidates; }, [&](int c) {
+            a.group = tuned_choice(p, N, (int)sizeof(T), (int)sizeof(G), ST_RAD_BWD, s, [&] { return kRadialCandidates; }, [&](int c) {
                 ZtGemmArgs b = a;
                 b.group = c;
                 zt_radial_backward<T>(b, s);

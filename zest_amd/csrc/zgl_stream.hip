@@ -97,6 +97,20 @@ __device__ inline void buffer_store16(__amdgpu_buffer_rsrc_t r, unsigned off, V 
     static_assert(sizeof(V) == 16, "16-byte accesses");
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bu4, v), r, (int)off, 0, 0);
 }
+typedef unsigned int bu2 __attribute__((vector_size(8)));
+// one (re, im) pair of T: 8 bytes (float) or 16 bytes (double)
+template <class T>
+__device__ inline void buffer_store_pair(__amdgpu_buffer_rsrc_t r, unsigned off, T re, T im) {
+    if constexpr (sizeof(T) == 8) {
+        typedef double dd2 __attribute__((ext_vector_type(2)));
+        const dd2 v = {re, im};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bu4, v), r, (int)off, 0, 0);
+    } else {
+        typedef float ff2 __attribute__((ext_vector_type(2)));
+        const ff2 v = {re, im};
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bu2, v), r, (int)off, 0, 0);
+    }
+}
 
 // Fills n elements of a workgroup's table block in LDS from global memory:
 // map(e) gives element e's LDS index and source (ok false: a zero, src then
@@ -1029,6 +1043,30 @@ __device__ __forceinline__ void rad_bwd_tiles(const RadBwdArgs<T>& a, const T* R
         // dropped), so the count of stores after the prefetch is static too
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
+            // A last tile of at most 16 columns (8 of 64 at order 128 in
+            // FP32) takes one column per lane: 2 MFMAs per step instead of
+            // 2 VW, and as many stores as a full tile (4 of them dropped)
+            if (ct < ntile && a.nr - ct * TNW <= 16) {
+                typename M::acc_t ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
+                const int col = ct * TNW + j;
+#pragma unroll
+                for (int s = 0; s < NSTEP; ++s) {
+                    if (s < steps) {
+                        const T b = Rs[(4 * s + g) * RS + col];
+                        ar1 = M::mma(xre[s], b, ar1);
+                        ai1 = M::mma(xim[s], b, ai1);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = mb + M::row(lane, r);
+                    const bool ok = m <= l && col < a.nr;
+                    const unsigned o = (unsigned)tri_idx(l, m) * rowbytes + 2u * col * sizeof(T);
+                    buffer_store_pair<T>(rf, ok ? o : kOutOfRange, ar1[r], ai1[r]);
+                    buffer_store16(rf, kOutOfRange, V{});
+                }
+                continue;
+            }
             typename M::acc_t are[VW], aim[VW];
 #pragma unroll
             for (int v = 0; v < VW; ++v) are[v] = aim[v] = typename M::acc_t{0, 0, 0, 0};
