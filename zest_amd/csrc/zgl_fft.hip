@@ -548,10 +548,16 @@ __device__ inline void stage_tables(C*& data, const C*& twl, const C*& twp, unsi
     __syncthreads();
 }
 
-// Waves per SIMD asked of the lean variant (3 in FP64, 4 in FP32) and of the
-// 512-thread one (4: two workgroups per CU); none of the others.
+// Waves per SIMD asked of the lean variant (3 in FP64, 4 in FP32), of the
+// 512-thread one (4: two workgroups per CU) and of the narrow FP64 tiles with
+// the matrix-core pass (T <= 8, long sequences such as SH L = 512: 2); none
+// of the others. Unasked, the compiler gives those narrow kernels one
+// register more than two waves allow (a wave per SIMD less: SH L = 512 FFTs
+// 4.3 -> 7.5 ms per 1024 instances). Asking three of the narrow FP64 kernels
+// without the pass makes them spill.
 template <int T, class R, int MF>
 constexpr int lean_waves() {
+    if constexpr (sizeof(R) == 8 && T <= 8 && MF == 1) return 2;
     return MF == 2 ? (sizeof(R) == 8 ? 3 : 4) : (fft_threads<T, R, MF>() > kThreads ? (sizeof(R) == 8 ? 2 : 4) : 1);
 }
 
@@ -607,12 +613,18 @@ __device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, cons
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
-    stage_tables<T, NT>(data, twl, twp, perm, d, a.twg, a.twpg, a.permg, copy_tables);
+    // the tables and operands through restrict-qualified locals: no store of
+    // the kernel aliases them (without this the FP64 kernels hold ~30 more
+    // VGPRs and lose a wave per SIMD)
+    const C* __restrict__ twg = a.twg;
+    const C* __restrict__ twpg = a.twpg;
+    const unsigned short* __restrict__ permg = a.permg;
+    stage_tables<T, NT>(data, twl, twp, perm, d, twg, twpg, permg, copy_tables);
     const int n = d.n;
     int th, tc;
     thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
-    const G* g = a.grid.at(inst);
+    const G* __restrict__ g = a.grid.at(inst);
 
     // Loads are issued in batches of kBatch per thread before any LDS store,
     // so each thread keeps that many independent global reads in flight.
@@ -714,7 +726,7 @@ __device__ __forceinline__ void fft_forward_output(const FftFwdArgs<R, G>& a, co
     const double scale = a.scale, center_factor = a.center_factor;
     const double* __restrict__ wlat = a.wlat;
     const double* __restrict__ wrad = a.wrad;
-    C* sp = a.spec + (long)inst * a.spec_stride;
+    C* __restrict__ sp = a.spec + (long)inst * a.spec_stride;
     int th, tc;
     thread_column<T, NT>(pm, tile, th, tc);
     if constexpr (spec_vec<R, VEC>() == 2) {
@@ -799,13 +811,19 @@ __device__ __forceinline__ void fft_backward_body(const FftBwdArgs<R, G>& a, con
     C* data;
     const C *twl, *twp;
     unsigned short* perm;
-    stage_tables<T, NT>(data, twl, twp, perm, d, a.twg, a.twpg, a.permg, copy_tables);
+    // the tables and operands through restrict-qualified locals: no store of
+    // the kernel aliases them (without this the FP64 kernels hold ~30 more
+    // VGPRs and lose a wave per SIMD)
+    const C* __restrict__ twg = a.twg;
+    const C* __restrict__ twpg = a.twpg;
+    const unsigned short* __restrict__ permg = a.permg;
+    stage_tables<T, NT>(data, twl, twp, perm, d, twg, twpg, permg, copy_tables);
     const int n = d.n;
     int th, tc;
     thread_column<T, NT>(pm, tile, th, tc);
     const int h0 = tile * T;  // first latitude pair of a row-layout (SH) tile
-    G* g = a.grid.at(inst);
-    const C* sp = a.spec + (long)inst * a.spec_stride;
+    G* __restrict__ g = a.grid.at(inst);
+    const C* __restrict__ sp = a.spec + (long)inst * a.spec_stride;
     const int F = n / 2 + 1;
     if constexpr (spec_vec<R, VEC>() == 2 && kBwdSpecPairs) {
         // two consecutive columns per lane: 16-byte spectrum loads

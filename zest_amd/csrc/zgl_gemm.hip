@@ -66,18 +66,37 @@ static_assert(TK == 32 && TN == 64 && kThreads == 256, "loader thread mapping");
 // Operand kinds (see the header comment).
 enum { ROWS_OUT = 0, ROWS_K = 1 };
 
-// Workgroup b runs on XCD b % 8 (each XCD has its own L2). The grid deals the
-// column tiles of one work item to one XCD, consecutively: the A block of the
-// item (a P_lm or R_nl table block, re-staged for every column tile) is then
-// read from HBM once per item instead of once per tile.
+// Workgroup b runs on XCD b % 8 (each XCD has its own L2). The work items
+// form groups of kGroupItems consecutive items (the row tiles of one problem,
+// as the work lists are built), dealt to the XCDs in turn (neighbouring
+// problems, of similar cost, on different XCDs); an XCD walks each of its
+// groups column tile by column tile, the items of a group consecutively: the
+// workgroups of one (group, column tile) then read the same B tile and run
+// side by side, so it comes from HBM once, and the group's A blocks (P_lm or
+// R_nl table blocks, re-staged for every column tile) stay in the L2 while
+// the group's column tiles pass. (ZGL_GEMM_GROUP=n sets another group size,
+// for comparison runs; 1: every item's column tiles in turn.)
+constexpr int kGroupItems = 4;
+
+int gemm_group() {
+    static const int g = [] {
+        const char* e = std::getenv("ZGL_GEMM_GROUP");
+        const int v = e ? std::atoi(e) : kGroupItems;
+        return v >= 1 && v <= 64 ? v : kGroupItems;
+    }();
+    return g;
+}
+
 template <class S, int NS>
 __global__ __launch_bounds__(kThreads) void gemm_kernel(S st, const int4* __restrict__ work, int nwork,
-                                                        int nct) {
+                                                        int nct, int gsz) {
     using T = typename S::T;
     using M = Mfma<T>;
     constexpr int TM = 16 * NS;
     const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
-    const int ct = q % nct, wi = (q / nct) * 8 + xcd;
+    const int gl = q / (nct * gsz), rq = q - gl * (nct * gsz);
+    const int ct = rq / gsz, r = rq - ct * gsz;
+    const int wi = (gl * 8 + xcd) * gsz + r;  // group gl * 8 + xcd, its item r
     if (wi >= nwork) return;
     const int4 wk = work[wi];
     const int n0 = ct * TN;
@@ -501,8 +520,10 @@ template <int NS, class S>
 void launch(const S& st, const int4* work, int nwork, int ncols, int nz, hipStream_t s) {
     if (nwork == 0 || ncols <= 0 || nz <= 0) return;
     const int nct = (ncols + TN - 1) / TN;
-    dim3 grid((unsigned)(8 * nct) * (unsigned)((nwork + 7) / 8), 1, nz);
-    gemm_kernel<S, NS><<<grid, kThreads, 0, s>>>(st, work, nwork, nct);
+    const int gsz = gemm_group();
+    const int groups = (nwork + gsz - 1) / gsz;  // dealt to the 8 XCDs in turn
+    dim3 grid((unsigned)(8 * nct * gsz) * (unsigned)((groups + 7) / 8), 1, nz);
+    gemm_kernel<S, NS><<<grid, kThreads, 0, s>>>(st, work, nwork, nct, gsz);
     check_hip(hipGetLastError(), "gemm launch");
 }
 
