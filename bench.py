@@ -561,7 +561,10 @@ def run_sh512(args):
     c *= torch.from_numpy((1.0 + l) ** -1.5).to(dev)[None, :, None]
     c[:, torch.from_numpy(m == 0).to(dev), 1] = 0.0
     g = torch.empty((B,) + zest_amd.st_shape(L), dtype=torch.float64, device=dev)
-    forms = (("fp64", zest_amd.FP64), ("fp32", zest_amd.FP32), ("fp32+refine", zest_amd.FP32_REFINE))
+    # fp32_grid32: FP32 arithmetic on grids stored in FP32 (the forward reads
+    # the FP32 rounding of the FP64 backward's grid)
+    forms = (("fp64", zest_amd.FP64), ("fp32", zest_amd.FP32), ("fp32_grid32", zest_amd.FP32),
+             ("fp32+refine", zest_amd.FP32_REFINE))
     if args.only == "fp64":  # profiling runs: the FP64 form alone
         forms = forms[:1]
     plans = {name: zest_amd.Plan(zest_amd.SH, L, precision=p) for name, p in forms}
@@ -573,12 +576,14 @@ def run_sh512(args):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
            "config": {"workload": "sh_glq_backward_forward", "order": L, "instances": B,
                       "grid": list(zest_amd.st_shape(L)), "conventions": "geo,none"}}
-    gw = torch.empty_like(g)
     cw = torch.zeros_like(c)
     for name, plan in plans.items():
+        gin = g.float() if name == "fp32_grid32" else g
+        gw = torch.empty_like(gin)
         cx = torch.zeros_like(c)
-        plan.forward(g, cx, L, batch=B)
+        plan.forward(gin, cx, L, batch=B)
         torch.cuda.synchronize()
+        del gin
         d = cx - c64
         maxrel = float((d.abs().amax(dim=(1, 2)) / c64.abs().amax(dim=(1, 2))).max())
         rms = float((d.square().sum(dim=(1, 2)).sqrt() / c64.square().sum(dim=(1, 2)).sqrt()).max())

@@ -255,8 +255,11 @@ class Plan:
         _check(LIB.zgl_plan_tuning(self._h, None, 0, ctypes.byref(n)))
         arr = (ctypes.c_int * (5 * max(1, n.value)))()
         _check(LIB.zgl_plan_tuning(self._h, arr, n.value, ctypes.byref(n)))
+        # the library's stage order (kStageNames, zgl_api.hip)
         names = ("fft_forward", "legendre_forward", "radial_forward", "radial_backward",
-                 "legendre_backward", "fft_backward")
+                 "legendre_backward", "fft_backward", "eval_tables", "eval_radial", "eval_legendre",
+                 "eval_longitude", "rotate", "rotate_wide", "polar_rotate", "power_spectrum",
+                 "legendre_radial_forward", "radial_legendre_backward")
         return [(arr[5 * i], arr[5 * i + 1], arr[5 * i + 2], names[arr[5 * i + 3]], arr[5 * i + 4])
                 for i in range(n.value)]
 
