@@ -1070,11 +1070,22 @@ __device__ __forceinline__ void rad_bwd_tiles(const RadBwdArgs<T>& a, const T* R
             typename M::acc_t are[VW], aim[VW];
 #pragma unroll
             for (int v = 0; v < VW; ++v) are[v] = aim[v] = typename M::acc_t{0, 0, 0, 0};
-            const int col = ct * TNW + VW * j;
+            // column of value v of lane j: half q = v / HV of the tile, then
+            // HV consecutive columns per lane, so that the 16-byte (re, im)
+            // store of each half is contiguous across the 16 lanes of a row
+            // (256 bytes per store instruction and row, not 16-byte pieces
+            // 32 bytes apart)
+            constexpr int HV = VW / 2;
+            const int col0 = ct * TNW + HV * j;
 #pragma unroll
             for (int s = 0; s < NSTEP; ++s) {
                 if (s < steps && ct < ntile) {
-                    const V b = *reinterpret_cast<const V*>(Rs + (4 * s + g) * RS + col);
+                    const T* rrow = Rs + (4 * s + g) * RS + col0;
+                    T b[VW];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int e = 0; e < HV; ++e) b[q * HV + e] = rrow[q * (TNW / 2) + e];
 #pragma unroll
                     for (int v = 0; v < VW; ++v) {
                         are[v] = M::mma(xre[s], b[v], are[v]);
@@ -1082,22 +1093,23 @@ __device__ __forceinline__ void rad_bwd_tiles(const RadBwdArgs<T>& a, const T* R
                     }
                 }
             }
-            // 2*VW reals per row: (re, im) of columns col .. col+VW-1, two
-            // 16-byte stores; rows past l and columns past nr are dropped
+            // per row and half: the (re, im) pairs of HV columns, one 16-byte
+            // store; rows past l and columns past nr are dropped
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = mb + M::row(lane, r);
-                const bool ok = m <= l && col < a.nr;
-                const unsigned o = (unsigned)tri_idx(l, m) * rowbytes + 2u * col * sizeof(T);
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
+                    const int c = col0 + h * (TNW / 2);
+                    const bool ok = m <= l && c < a.nr;
+                    const unsigned o = (unsigned)tri_idx(l, m) * rowbytes + 2u * c * sizeof(T);
                     V w;
 #pragma unroll
-                    for (int e = 0; e < VW; ++e) {
-                        const int v = (VW * h + e) >> 1;
-                        w[e] = ((VW * h + e) & 1) ? aim[v][r] : are[v][r];
+                    for (int e = 0; e < HV; ++e) {
+                        w[2 * e] = are[h * HV + e][r];
+                        w[2 * e + 1] = aim[h * HV + e][r];
                     }
-                    buffer_store16(rf, ok ? o + 16u * h : kOutOfRange, w);
+                    buffer_store16(rf, ok ? o : kOutOfRange, w);
                 }
             }
         }
