@@ -35,7 +35,7 @@ def test_tuned_results_identical(device, precision):
     assert out[False][1] == []
     tuning = out[True][1]
     print("tuning:", tuning)
-    x3 = any(st.startswith("legendre") and choice >= 16 for _, _, _, st, choice in tuning)
+    x3 = any(st.startswith("legendre") and choice & 16 for _, _, _, st, choice in tuning)
     for (g0, c0), (g1, c1) in zip(out[False][0], out[True][0]):
         if x3:
             # two FP32-level results against each other, each within the 1e-6 gate
@@ -53,10 +53,42 @@ def test_tuned_results_identical(device, precision):
         if st.startswith("fft"):
             assert choice in (32, 16, 8, 4, 2, 1)
         elif st.startswith("legendre"):
-            # instances per workgroup (+16: BF16x3 products, FP32 stages)
-            assert choice in (1, 2, 3, 4, 6, 8, 12) or (precision == 1 and choice in (17, 18, 20, 24))
+            # instances per workgroup (+16: BF16x3 products, FP32 stages), or
+            # 32 + tiles per wave (runs of equal tile counts)
+            assert (choice in (1, 2, 3, 4, 6, 8, 12, 34, 35, 36, 38)
+                    or (precision == 1 and choice in (17, 18, 20, 24)))
         else:
             assert choice in (1, 2, 4, 8, 16)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_legendre_tile_runs_identical(device, monkeypatch, precision):
+    """Workgroups over runs of 4 t column tiles of the flattened (instance,
+    column) index (ZGL_LEGENDRE_GROUP = 32 + t) compute every output with the
+    same products in the same order as instance groups: bit-identical grids
+    and coefficients, at orders whose rows fill a whole number of tiles or
+    not, batches smaller than one run and runs that end mid-instance."""
+    import torch
+    import zest_amd
+    from zest_amd import synth
+    for order, batch in ((128, 3), (40, 7), (9, 5), (64, 1)):
+        c = np.stack([synth.zernike_coeffs(order, 30 + b) for b in range(batch)])
+        dc = torch.from_numpy(c).to(device)
+        res = {}
+        for grp in ("", "4", "33", "35", "36", "47"):
+            monkeypatch.setenv("ZGL_LEGENDRE_GROUP", grp)
+            plan = zest_amd.Plan(zest_amd.ZERNIKE, order, precision=precision)
+            dg = torch.empty((batch,) + zest_amd.zt_shape(order), dtype=torch.float64, device=device)
+            co = torch.zeros_like(dc)
+            plan.backward(dc, dg, order, batch=batch)
+            plan.forward(dg, co, order, batch=batch)
+            torch.cuda.synchronize()
+            res[grp] = (dg.cpu().numpy(), co.cpu().numpy())
+        g0, c0 = res[""]
+        assert np.abs(c0 - c).max() <= (1e-10 if precision == 0 else 1e-5) * np.abs(c).max()
+        for grp, (g1, c1) in res.items():
+            assert np.array_equal(g0, g1), (order, batch, grp)
+            assert np.array_equal(c0, c1), (order, batch, grp)
 
 
 @pytest.mark.parametrize("kind,layout", [("zernike", 0), ("sh", 0), ("sh", 1)])

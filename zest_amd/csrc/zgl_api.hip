@@ -513,7 +513,11 @@ int tuned_choice(zgl_plan* p, size_t order, int rbytes, int gbytes, int stage, h
     return c;
 }
 
-const std::vector<int> kGroupCandidates = {4, 2, 8, 1, 3, 6, 12};  // the untuned default first
+// the untuned default first; kGroupTiles | t: workgroups of 4 t column tiles
+// of the flattened (instance, column) index (every wave the same count of
+// tiles, where G instances per workgroup leave a partial tile at order 128)
+const std::vector<int> kGroupCandidates = {4, 2, 8, 1, 3, 6, 12, kGroupTiles | 2, kGroupTiles | 3,
+                                           kGroupTiles | 4, kGroupTiles | 6};
 // the radial stages also take 16 instances per workgroup (no flag bits there)
 const std::vector<int> kRadialCandidates = {4, 2, 8, 1, 16};
 // BF16x3 Legendre products (kGroupX3) at every group size: the candidates of

@@ -212,9 +212,11 @@ struct ZtGemmArgs {
     Inst<double> coeffs;    // strides in doubles
     int nb;
     int group = 0;          // instances per workgroup of the streaming kernels (0: default);
-                            // Legendre stages: | kGroupX3 for the BF16x3 products (FP32)
+                            // Legendre stages: | kGroupX3 for the BF16x3 products (FP32),
+                            // or kGroupTiles | tiles per wave (runs of equal tile counts)
 };
 constexpr int kGroupX3 = 16;
+constexpr int kGroupTiles = 32;
 template <class T> void zt_legendre_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_forward(const ZtGemmArgs& a, hipStream_t s);
 template <class T> void zt_radial_backward(const ZtGemmArgs& a, hipStream_t s);

@@ -311,11 +311,13 @@ struct LegEval {
 };
 
 // The tile loop of one wave, for NSA active 16-row strips (a compile-time
-// count, so the MFMA sequence has no per-strip branches).
+// count, so the MFMA sequence has no per-strip branches). The workgroup's
+// tiles start at column c0 of instance g0 and run over the flattened
+// (instance, column) index.
 template <int NSA, bool CENTER, class S>
 __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob& P,
                                              const typename S::T* As, int AS, int m0, int rows,
-                                             int g0, int gn, int wave, int lane) {
+                                             int g0, int gn, int c0, int tiles, int wave, int lane) {
     using T = typename S::T;
     using M = SM<T>;
     using V = typename M::V;
@@ -324,7 +326,6 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
     const int j = lane & 15, g = lane >> 4;
     const int K = P.K, steps = (K + 3) >> 2;
     const int N = P.N;
-    const int tiles = (gn * N + TNW - 1) / TNW;
     const int ntw = tiles > wave ? (tiles - wave + 3) >> 2 : 0;  // tiles of this wave
     const int nchunks = max(1, (steps + kKC - 1) / kKC);
     const int U = ntw * nchunks;
@@ -341,13 +342,14 @@ __device__ __forceinline__ void stream_tiles(const S& st, const typename S::Prob
 
     // unit u -> (instance, first column of this lane, chunk). A tile's TNW
     // columns run over the flattened (instance, column) index of the group,
-    // so a tile may end one instance and start the next and only the group's
-    // last tile is partial (N is a multiple of 16, so a lane's VW columns
-    // never straddle two instances); lanes past the group get inst >= g0 + gn
+    // so a tile may end one instance and start the next and only the
+    // workgroup's last tile can be partial (N and c0 are multiples of 16, so a lane's
+    // VW columns never straddle two instances); lanes past the group get
+    // inst >= g0 + gn
     auto unit = [&](int u, int& inst, int& col, int& ch) {
         const int tw = u / nchunks;
         ch = u - tw * nchunks;
-        const int fc = (wave + 4 * tw) * TNW + VW * j;
+        const int fc = c0 + (wave + 4 * tw) * TNW + VW * j;
         const int gi = fc / N;
         inst = g0 + gi;
         col = fc - gi * N;
@@ -652,11 +654,16 @@ __device__ __forceinline__ void stream_tiles_x3(const S& st, const typename S::P
 
 // PLM: the P block is recomputed in the workgroup (no table)
 // X3: the BF16x3 tile loop (FP32 stages only)
-// One work item wk (problem, first row) for instance group `ig` of nb instances.
+// One work item wk (problem, first row) for instance group `ig` of nb
+// instances: st.G instances, or with tpw > 0 the ig-th run of 4 tpw tiles of
+// the flattened (instance, column) index of all nb instances -- the same
+// number of tiles for every wave of the workgroup, whatever the row length.
 template <class S, int NS, bool PLM, bool X3>
-__device__ __forceinline__ void stream_body(const S& st, const int4 wk, const int ig, const int nb) {
+__device__ __forceinline__ void stream_body(const S& st, const int4 wk, const int ig, const int nb,
+                                            const int tpw = 0) {
     using T = typename S::T;
     constexpr int TM = 16 * NS;
+    constexpr int TNW = 16 * (X3 ? SM<float>::VW : SM<T>::VW);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* As = reinterpret_cast<T*>(smem_raw);
 
@@ -664,16 +671,29 @@ __device__ __forceinline__ void stream_body(const S& st, const int4 wk, const in
     const typename S::Prob P = st.problem(wk.x, m0);
     const int rows = min(TM, P.M - m0);
     if (rows <= 0) return;
-    const int g0 = ig * st.G;
-    const int gn = min(st.G, nb - g0);
-    if (gn <= 0) return;
+    int g0, gn, c0, tiles;
+    if (tpw > 0) {
+        const long f0 = (long)ig * 4 * tpw * TNW;  // first flattened column
+        const long f1 = min((long)nb * P.N, f0 + 4L * tpw * TNW);
+        if (f0 >= f1) return;
+        g0 = (int)(f0 / P.N);
+        c0 = (int)(f0 - (long)g0 * P.N);
+        gn = (int)((f1 - 1) / P.N) - g0 + 1;
+        tiles = (int)((f1 - f0 + TNW - 1) / TNW);
+    } else {
+        g0 = ig * st.G;
+        gn = min(st.G, nb - g0);
+        if (gn <= 0) return;
+        c0 = 0;
+        tiles = (gn * P.N + TNW - 1) / TNW;
+    }
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int K = P.K, KP = X3 ? (K + 15) & ~15 : (K + 3) & ~3;
+    const int K = P.K;
     const int AS = st.as;
 
-    // A tile: TM rows x KP, zero padded, As[i * AS + k]; the equator row
+    // A tile: TM rows x KP (K rounded up to the step of the tile loop), zero padded, As[i * AS + k]; the equator row
     // (P.center) goes to As row TM. Without a table the block is recomputed.
     const bool center = S::kCenter && P.center;
     if constexpr (PLM) {
@@ -708,8 +728,8 @@ __device__ __forceinline__ void stream_body(const S& st, const int4 wk, const in
             if (center) stream_tiles_x3<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane); \
             else stream_tiles_x3<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);             \
         } else {                                                                                     \
-            if (center) stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, wave, lane);    \
-            else stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, wave, lane);                \
+            if (center) stream_tiles<n, S::kCenter>(st, P, As, AS, m0, rows, g0, gn, c0, tiles, wave, lane); \
+            else stream_tiles<n, false>(st, P, As, AS, m0, rows, g0, gn, c0, tiles, wave, lane);             \
         }                                                                                            \
     }
     switch (nstrips) {
@@ -722,8 +742,8 @@ __device__ __forceinline__ void stream_body(const S& st, const int4 wk, const in
 }
 
 template <class S, int NS, bool PLM, bool X3 = false>
-__global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb) {
-    stream_body<S, NS, PLM, X3>(st, work[blockIdx.x], blockIdx.y, nb);
+__global__ __launch_bounds__(kThreads, 3) void stream_kernel(S st, const int4* __restrict__ work, int nb, int tpw) {
+    stream_body<S, NS, PLM, X3>(st, work[blockIdx.x], blockIdx.y, nb, tpw);
 }
 
 // Persistent form over many orders (ragged batches): workgroup b walks items
@@ -777,6 +797,9 @@ int stream_group(const S& st, int requested, int nb) {
     return (int)std::min<long>(group_of(requested, nb), ((long)kOutOfRange - 1) / span);
 }
 
+// group: instances per workgroup in the low four bits, or with kGroupTiles
+// set, tiles per wave (the workgroups then split the flattened (instance,
+// column) index of the batch into runs of 4 such tiles; not with x3)
 template <int NS, class S>
 bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, int group, hipStream_t s,
                    bool x3 = false) {
@@ -787,12 +810,30 @@ bool launch_stream(S st, int kmax, const int4* work, int nwork, int nb, int grou
     const size_t lds = (size_t)(16 * NS + (S::kCenter ? 1 : 0)) * st.as * sizeof(T);
     if (lds > kMaxStreamLds) return false;
     if (nwork == 0 || nb <= 0) return true;
-    st.G = stream_group(st, group, nb);
-    if (st.G <= 0) return false;
-    dim3 grid(nwork, (nb + st.G - 1) / st.G);
+    int tpw = 0, ngroups = 0;
+    if ((group & kGroupTiles) && !x3 && (group & 15) > 0) {
+        // a run of 4 tpw tiles touches at most `span` instances (every
+        // problem has N = 2 ncol columns); their operand and output spans
+        // must stay within the 32-bit offsets, else instance groups are used
+        constexpr int TNW = 16 * SM<T>::VW;
+        const long n = 2L * st.ncol, run = 4L * (group & 15) * TNW;
+        const long span = std::min<long>((run + n - 1) / n + 1, nb);
+        const long ibytes = std::max(st.b_stride(), st.c_stride()) * (long)sizeof(T);
+        if (span * ibytes < (long)kOutOfRange) {
+            tpw = group & 15;
+            st.G = (int)span;  // not read by the kernels in this mode
+            ngroups = (int)((nb * n + run - 1) / run);
+        }
+    }
+    if (tpw == 0) {
+        st.G = stream_group(st, group & 15, nb);
+        if (st.G <= 0) return false;
+        ngroups = (nb + st.G - 1) / st.G;
+    }
+    dim3 grid(nwork, ngroups);
     auto go = [&](auto kernel) {
         allow_lds(kernel, lds);
-        kernel<<<grid, kThreads, lds, s>>>(st, work, nb);
+        kernel<<<grid, kThreads, lds, s>>>(st, work, nb, tpw);
     };
     if constexpr (std::is_same_v<T, float>) {
         if (x3) {
@@ -931,7 +972,6 @@ template <class T, int NS>
 __device__ __forceinline__ void rad_fwd_body(const RadFwdArgs<T>& a, const int4* __restrict__ work, const int bx,
                                              const int nb) {
     constexpr int TM = 16 * NS;
-    constexpr int KI = 4 * RadCfg<T>::KV;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* As = reinterpret_cast<T*>(smem_raw);
     // one-dimensional grid, instance group fastest: every group starts the
@@ -1276,13 +1316,24 @@ static bool legendre_x3(int group) {
     return e && std::strcmp(e, "bf16x3") == 0;
 }
 
+// The grouping of an untuned Legendre stage: ZGL_LEGENDRE_GROUP forces one
+// (1..16 instances per workgroup, or 33..47: kGroupTiles | tiles per wave),
+// else the default (4 instances).
+static int legendre_group(int group) {
+    if (group & ~kGroupX3) return group & ~kGroupX3;
+    const char* e = std::getenv("ZGL_LEGENDRE_GROUP");
+    if (!e || !*e) return 0;
+    const int v = std::atoi(e);
+    return (v >= 1 && v <= kMaxGroup) || (v > kGroupTiles && v < kGroupTiles + 16) ? v : 0;
+}
+
 template <class T>
 bool zt_legendre_forward_stream(const ZtGemmArgs& a, hipStream_t s) {
     const Geom& g = a.t->g;
     LegFwd<T> st{ptab_sel<T>(*a.t), (const T*)a.spec, (T*)a.flm, a.spec_stride * 2,
                  a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0,
                  a.t->zh, a.t->plm_qm, a.t->plm_phase};
-    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, a.group & 15, s,
+    return launch_stream<kNsLegendreFwd>(st, g.jh, a.t->work_lf, a.t->n_lf, a.nb, legendre_group(a.group), s,
                                          legendre_x3(a.group));
 }
 
@@ -1293,7 +1344,7 @@ bool zt_legendre_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
                  a.flm_stride * 2, g.order, a.nt, g.jh, g.ncol, 1, 0, g.nth,
                  a.t->zh, a.t->plm_qm, a.t->plm_phase};
     return launch_stream<kNsStreamBwd>(st, plm_count(g.order, 0, 0), a.t->work_sb, a.t->n_sb,
-                                       a.nb, a.group & 15, s, legendre_x3(a.group));
+                                       a.nb, legendre_group(a.group), s, legendre_x3(a.group));
 }
 
 // ---- persistent launches over many orders (ragged batches) ----
