@@ -425,11 +425,15 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
             for (int m0 = 0; m0 < ncount; m0 += tm_r) rf.push_back(make_int4(l, m0, 0, 0));
             for (int m0 = 0; m0 < 2 * (l + 1); m0 += tm_r) rb.push_back(make_int4(l, m0, 0, 0));
         }
-        // radial forward items, costliest first (rows of the tile x (l + 1)
-        // values of m): the short items fill in behind the long ones
+        // radial forward items, costliest first, so that the short items
+        // fill in behind the long ones. An item streams its flm rows m <= l
+        // (blocks of 16 m) once, whatever its row count, and runs one MFMA
+        // strip per 16 rows for each block: cost (blocks of m) x (2 + strips),
+        // the flm stream weighted as two strips.
         auto cost = [&](const int4& w) {
             const int ncount = (order - w.x + 1) / 2;
-            return (long)std::min(tm_r, ncount - w.y) * (w.x + 1);
+            const int strips = (std::min(tm_r, ncount - w.y) + 15) / 16;
+            return (long)((w.x + 16) / 16) * (2 + strips);
         };
         std::stable_sort(rf.begin(), rf.end(), [&](const int4& a, const int4& b) { return cost(a) > cost(b); });
     }
