@@ -109,6 +109,7 @@ struct OrderTables {
     int4* work_rf = nullptr; int n_rf = 0;  // forward radial
     int4* work_rb = nullptr; int n_rb = 0;  // backward radial
     int4* work_sb = nullptr; int n_sb = 0;  // backward Legendre, streaming kernel (Zernike)
+    int4* work_rs = nullptr; int n_rs = 0;  // backward radial, streaming kernel: l, costliest first
     // fused FP32 colatitude + radial kernels (Zernike, orders <= 128): R_nl(r_k)
     // by 16-node chunk, [l][kc][d][16] (forward) and [l][kc][16][d] (backward),
     // n = l + 2d, d < fuse_dp; (m, par) problems, costliest first

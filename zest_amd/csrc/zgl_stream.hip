@@ -1025,6 +1025,7 @@ struct RadBwdArgs {
     const T* rtab; Inst<const double> coeffs; T* flm;
     long flm_stride;
     int nt, nr, G, rs;
+    const int4* lorder = nullptr;  // l of rank r (nt entries, costliest first), or null
 };
 
 // The tiles of one wave with SC*4 MFMA steps (a compile-time count, so the
@@ -1164,9 +1165,10 @@ __device__ __forceinline__ void rad_bwd_body(const RadBwdArgs<T>& a, const int b
     // one-dimensional grid, instance group fastest; the rank r enumerates l
     // from the middle outwards (c, c-1, c+1, ..., c = nt/2), roughly by
     // decreasing cost (l+1)(nt-l+1)/2, and covers [0, nt) exactly once
+    // (or, given, from the list of l by decreasing cost)
     const int ng = (nb + a.G - 1) / a.G;
     const int r = bx / ng, d = (r + 1) >> 1;
-    const int l = (r & 1) ? a.nt / 2 - d : a.nt / 2 + d;
+    const int l = a.lorder ? a.lorder[r].x : (r & 1) ? a.nt / 2 - d : a.nt / 2 + d;
     if (l < 0 || l >= a.nt) return;
     const int K = (a.nt - l + 1) / 2;
     const int sc = (K + 15) >> 4;  // groups of four MFMA steps, K <= 4 * kRadBwdSteps
@@ -1299,6 +1301,8 @@ bool zt_radial_backward_stream(const ZtGemmArgs& a, hipStream_t s) {
     if (a.nt == 0 || a.nb <= 0) return true;
     RadBwdArgs<T> r{rtab_sel<T>(*a.t), a.coeffs, (T*)a.flm, a.flm_stride * 2,
                     a.nt, g.ncol, group_of(a.group, a.nb), rs};
+    // the cost-ordered list covers l < order: used when nothing is truncated
+    if (a.t->n_rs == a.nt) r.lorder = a.t->work_rs;
     dim3 grid((unsigned)a.nt * (unsigned)((a.nb + r.G - 1) / r.G));
     allow_lds(&rad_bwd_kernel<T>, lds);
     rad_bwd_kernel<T><<<grid, kThreads, lds, s>>>(r, a.nb);

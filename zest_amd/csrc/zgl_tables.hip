@@ -415,7 +415,7 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
             for (int m0 = 0; m0 < cnt; m0 += tm_lf) lf.push_back(make_int4(prob, m0, 0, 0));
             for (int m0 = 0; m0 < g.jh; m0 += tm_lb) lb.push_back(make_int4(prob, m0, 0, 0));
         }
-    std::vector<int4> sb;
+    std::vector<int4> sb, rs;
     if (kind == ZGL_ZERNIKE)
         for (int prob = 0; prob < 2 * order; ++prob)
             for (int m0 = 0; m0 < g.nth / 2; m0 += 16 * kNsStreamBwd) sb.push_back(make_int4(prob, m0, 0, 0));
@@ -436,12 +436,22 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
             return (long)((w.x + 16) / 16) * (2 + strips);
         };
         std::stable_sort(rf.begin(), rf.end(), [&](const int4& a, const int4& b) { return cost(a) > cost(b); });
+        // radial backward (streaming kernel) items, one per l, costliest
+        // first: an item writes its flm rows m <= l (blocks of 16 m) and runs
+        // ceil(K / 16) groups of four MFMA steps per block, K = (order - l + 1) / 2
+        for (int l = 0; l < order; ++l) rs.push_back(make_int4(l, 0, 0, 0));
+        auto cost_b = [&](const int4& w) {
+            const int steps = ((order - w.x + 1) / 2 + 15) / 16;
+            return (long)((w.x + 16) / 16) * (2 + steps);
+        };
+        std::stable_sort(rs.begin(), rs.end(), [&](const int4& a, const int4& b) { return cost_b(a) > cost_b(b); });
     }
     t.work_lf = upload_work(lf, s); t.n_lf = (int)lf.size();
     t.work_lb = upload_work(lb, s); t.n_lb = (int)lb.size();
     t.work_rf = upload_work(rf, s); t.n_rf = (int)rf.size();
     t.work_rb = upload_work(rb, s); t.n_rb = (int)rb.size();
     t.work_sb = upload_work(sb, s); t.n_sb = (int)sb.size();
+    t.work_rs = upload_work(rs, s); t.n_rs = (int)rs.size();
     check_hip(hipStreamSynchronize(s), "work lists");
     if (want_fp32) ensure_fp32_tables(t, s);
 }
@@ -449,7 +459,7 @@ void build_tables(OrderTables& t, zgl_kind kind, int zernike_norm, int sh_norm, 
 void free_tables(OrderTables& t) {
     for (void* p : {(void*)t.wlat, (void*)t.wrad, (void*)t.ptab, (void*)t.zh, (void*)t.rtab, (void*)t.ptab32,
                     (void*)t.rtab32, (void*)t.work_lf, (void*)t.work_lb, (void*)t.work_rf,
-                    (void*)t.work_rb, (void*)t.work_sb, (void*)t.rfuse_f, (void*)t.rfuse_b,
+                    (void*)t.work_rb, (void*)t.work_sb, (void*)t.work_rs, (void*)t.rfuse_f, (void*)t.rfuse_b,
                     (void*)t.work_ff, (void*)t.work_fb})
         dfree(p);
     fft_plan_destroy(t.fft);
