@@ -770,16 +770,19 @@ __device__ __forceinline__ void fft_forward_output(const FftFwdArgs<R, G>& a, co
         }
         return;
     }
+    // a thread keeps one column (h, c) for every m: its quadrature weight
+    // (none for the transpose of the backward transform) is read once, not
+    // per m (a global load the compiler cannot hoist out of the guarded loop)
+    const int h = th, c = tc;
+    if (h >= pm.jh) return;
+    const R W = (R)((wlat ? wlat[h] * (wrad ? wrad[c] : 1.0) : 1.0) * scale);
     for (int e = threadIdx.x; e < nm * T; e += NT) {
-        const int m = e / T, t = e % T, h = th, c = tc;
-        if (h >= pm.jh) continue;
+        const int m = e / T, t = e % T;
         const C zm = data[m * T + t];
         const C zc = data[(m ? n - m : 0) * T + t];
         // two-for-one split: A = (Z_m + conj Z_{n-m}) / 2, B = (Z_m - conj Z_{n-m}) / (2i)
         const C A = mk<C>(R(0.5) * (zm.x + zc.x), R(0.5) * (zm.y - zc.y));
         const C B = mk<C>(R(0.5) * (zm.y + zc.y), R(-0.5) * (zm.x - zc.x));
-        // quadrature weight (none for the transpose of the backward transform)
-        const R W = (R)((wlat ? wlat[h] * (wrad ? wrad[c] : 1.0) : 1.0) * scale);
         // H = weight * conj(scale * X)
         const C Ha = mk<C>(W * A.x, -W * A.y);
         const C Hb = mk<C>(W * B.x, -W * B.y);
@@ -959,20 +962,27 @@ __device__ __forceinline__ void fft_backward_body(const FftBwdArgs<R, G>& a, con
             }
         }
     } else if (LK == GRID_COLS) {
-        for (int e = threadIdx.x; e < n * T; e += NT) {
-            const int i = e / T, t = e % T, h = th, c = tc;
-            if (h >= pm.jh) continue;
-            C z = data[i * T + t];
+        // one column (h, c) per thread: its output weight (the transpose of
+        // the forward transform) is read once
+        const int h = th, c = tc;
+        if (h < pm.jh) {
             const int jn = pm.nth / 2 + h, js = pm.jh - 1 - h;
-            if (wlat) {  // output weights (the transpose of the forward transform)
-                R W = (R)(wlat[h] * (wrad ? wrad[c] : 1.0) * scale);
+            R W = 1;
+            if (wlat) {
+                W = (R)(wlat[h] * (wrad ? wrad[c] : 1.0) * scale);
                 if (jn == js) W *= (R)center_factor;
-                z.x *= W;
-                z.y *= W;
             }
-            G* row = g + (long)i * pm.lon_stride + c;
-            row[(long)jn * pm.lat_stride] = (G)z.x;
-            if (js != jn) row[(long)js * pm.lat_stride] = (G)z.y;
+            for (int e = threadIdx.x; e < n * T; e += NT) {
+                const int i = e / T, t = e % T;
+                C z = data[i * T + t];
+                if (wlat) {
+                    z.x *= W;
+                    z.y *= W;
+                }
+                G* row = g + (long)i * pm.lon_stride + c;
+                row[(long)jn * pm.lat_stride] = (G)z.x;
+                if (js != jn) row[(long)js * pm.lat_stride] = (G)z.y;
+            }
         }
     } else {
         for (int e = threadIdx.x; e < n * T; e += NT) {
