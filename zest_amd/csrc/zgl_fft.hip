@@ -629,8 +629,12 @@ __device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, cons
     // Loads are issued in batches of kBatch per thread before any LDS store,
     // so each thread keeps that many independent global reads in flight.
     if constexpr (VEC) {
-        // VG columns per lane; the lane's column group is threadIdx.x % (T / VG)
-        constexpr int VG = grid_vec<G, VEC>(), TQ = T / VG, KB = kBatch / 2;
+        // VG columns per lane; the lane's column group is threadIdx.x % (T / VG).
+        // KB loads of each latitude in flight per thread: with FP64 arithmetic
+        // on FP64 grids a whole tile at n = 255, T = 16 (8 per thread) goes out
+        // in one batch, one memory round trip instead of two
+        constexpr int VG = grid_vec<G, VEC>(), TQ = T / VG;
+        constexpr int KB = (sizeof(R) == 8 && VG == 2) ? kBatch : kBatch / 2;
         using V = typename Vec16<G>::type;
         static_assert(LK == GRID_COLS && T % VG == 0 && NT % TQ == 0, "vector tile");
         const int f = tile * T + ((int)threadIdx.x % TQ) * VG;
