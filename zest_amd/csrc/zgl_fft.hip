@@ -630,11 +630,12 @@ __device__ __forceinline__ void fft_forward_body(const FftFwdArgs<R, G>& a, cons
     // so each thread keeps that many independent global reads in flight.
     if constexpr (VEC) {
         // VG columns per lane; the lane's column group is threadIdx.x % (T / VG).
-        // KB loads of each latitude in flight per thread: with FP64 arithmetic
-        // on FP64 grids a whole tile at n = 255, T = 16 (8 per thread) goes out
-        // in one batch, one memory round trip instead of two
+        // KB loads of each latitude in flight per thread: on FP64 grids with
+        // FP64 arithmetic, or with 256 threads, a whole tile at n = 255, T = 16
+        // (8 per thread) goes out in one batch, one memory round trip instead
+        // of two
         constexpr int VG = grid_vec<G, VEC>(), TQ = T / VG;
-        constexpr int KB = (sizeof(R) == 8 && VG == 2) ? kBatch : kBatch / 2;
+        constexpr int KB = (VG == 2 && (sizeof(R) == 8 || NT == kThreads)) ? kBatch : kBatch / 2;
         using V = typename Vec16<G>::type;
         static_assert(LK == GRID_COLS && T % VG == 0 && NT % TQ == 0, "vector tile");
         const int f = tile * T + ((int)threadIdx.x % TQ) * VG;
