@@ -51,18 +51,18 @@ HBM_PEAK_GBS = 8000.0
 # -> (fetch bytes, write bytes). Reported as roofline.traffic only for that workload;
 # the commands and the full table are in profiles/zernike_n128_b64_kernels.md.
 MEASURED_TRAFFIC = {
-    ("fft_forward", 128, 64, "mixed_grid32"): (1192e6, 1161e6),
+    ("fft_forward", 128, 64, "mixed_grid32"): (1192e6, 1160e6),
     ("fft_backward", 128, 64, "mixed_grid32"): (1168e6, 1155e6),
     ("legendre_forward", 128, 64, "mixed_grid32"): (1303e6, 589e6),
-    ("legendre_backward", 128, 64, "mixed_grid32"): (697e6, 1179e6),
-    ("radial_forward", 128, 64, "mixed_grid32"): (618e6, 197e6),
-    ("radial_backward", 128, 64, "mixed_grid32"): (239e6, 589e6),
-    ("fft_forward", 128, 64, "fp64"): (2325e6, 2317e6),
+    ("legendre_backward", 128, 64, "mixed_grid32"): (688e6, 1183e6),
+    ("radial_forward", 128, 64, "mixed_grid32"): (596e6, 196e6),
+    ("radial_backward", 128, 64, "mixed_grid32"): (225e6, 589e6),
+    ("fft_forward", 128, 64, "fp64"): (2326e6, 2317e6),
     ("fft_backward", 128, 64, "fp64"): (2318e6, 2308e6),
     ("legendre_forward", 128, 64, "fp64"): (2511e6, 1150e6),
-    ("legendre_backward", 128, 64, "fp64"): (1291e6, 2300e6),
-    ("radial_forward", 128, 64, "fp64"): (1187e6, 195e6),
-    ("radial_backward", 128, 64, "fp64"): (285e6, 1152e6),
+    ("legendre_backward", 128, 64, "fp64"): (1298e6, 2300e6),
+    ("radial_forward", 128, 64, "fp64"): (1187e6, 199e6),
+    ("radial_backward", 128, 64, "fp64"): (284e6, 1152e6),
 }
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
